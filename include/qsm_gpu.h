@@ -1,0 +1,217 @@
+/*
+ * qsm_gpu.h -- C ABI of the MI355X (gfx950) backend for QSM.rs's TV-ADMM
+ * dipole inversion and V-SHARP background-field removal.
+ *
+ * Conventions shared with the reference crate (qsm-core):
+ *   - volumes are flat arrays in Fortran order, idx = i + j*nx + k*nx*ny
+ *     (src/fft.rs:236-241);
+ *   - voxel sizes are in mm, B0 direction is normalised internally;
+ *   - masks are u8, 0 = outside; the eroded mask returned by V-SHARP is 0/1.
+ * Every entry point is synchronous, returns a status code (0 = QSM_OK) and
+ * never aborts the process; qsm_last_error_message() describes the last
+ * failure on the calling thread.  Progress callbacks run on the calling
+ * thread with exactly the reference's (current, total) sequence.
+ *
+ * Host-pointer entry points allocate their device memory per call.  The
+ * batch entry points (qsm_tv_batch_*) take device pointers, keep their
+ * workspace between runs and are the throughput path.
+ */
+#ifndef QSM_GPU_H
+#define QSM_GPU_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes ------------------------------------------------------ */
+#define QSM_OK 0
+#define QSM_EDIM 1          /* length mismatch: PipelineError::DimensionMismatch
+                               (src/pipeline/config.rs:344-368) */
+#define QSM_EPARAM 2        /* invalid argument (null pointer, bad parameter) */
+#define QSM_EHIP 3          /* HIP runtime error */
+#define QSM_ENOMEM 4        /* device allocation failed */
+#define QSM_ECONFIG 5       /* PipelineError::InvalidConfig: algorithm not served here */
+#define QSM_EUNSUPPORTED 6  /* axis length outside what the FFT supports */
+
+/* ---- types ------------------------------------------------------------- */
+
+/* Grid, src/grid.rs:9-14: dims (nx, ny, nz) and voxel size (mm). */
+typedef struct qsm_grid {
+    size_t nx, ny, nz;
+    double vsx, vsy, vsz;
+} qsm_grid;
+
+/* TvParams, src/inversion/tv.rs:21-43 (defaults 2e-4, 2e-2, 1e-3, 250). */
+typedef struct qsm_tv_params {
+    double lambda;
+    double rho;
+    double tol;
+    size_t max_iter;
+} qsm_tv_params;
+
+/* VsharpParams, src/bgremove/vsharp.rs:20-39 (defaults 0.05, 12 mm, 1 mm). */
+typedef struct qsm_vsharp_params {
+    double threshold;
+    double max_radius;
+    double min_radius;
+} qsm_vsharp_params;
+
+/* Progress callback: the reference's `impl FnMut(usize, usize)`. */
+typedef void (*qsm_progress_fn)(size_t current, size_t total, void* user);
+
+/* ScanMetadata, src/pipeline/config.rs:300-318 (the fields the TV / V-SHARP
+ * arms read: dims, voxel size, B0 direction). */
+typedef struct qsm_scan_metadata {
+    size_t nx, ny, nz;
+    double vsx, vsy, vsz;
+    const double* echo_times;
+    size_t n_echo_times;
+    double field_strength;
+    double b0_direction[3];
+} qsm_scan_metadata;
+
+/* InversionAlgorithm, src/pipeline/config.rs:47-58 (same order). */
+typedef enum qsm_inversion_algorithm {
+    QSM_INV_TKD = 0, QSM_INV_TSVD, QSM_INV_TIKHONOV, QSM_INV_TV, QSM_INV_RTS,
+    QSM_INV_NLTV, QSM_INV_MEDI, QSM_INV_ILSQR, QSM_INV_TGV, QSM_INV_QSMART
+} qsm_inversion_algorithm;
+
+/* BgRemovalAlgorithm, src/pipeline/config.rs:33-43 (same order). */
+typedef enum qsm_bg_removal_algorithm {
+    QSM_BG_VSHARP = 0, QSM_BG_PDF, QSM_BG_LBV, QSM_BG_ISMV, QSM_BG_SHARP,
+    QSM_BG_RESHARP, QSM_BG_HARPERELLA, QSM_BG_IHARPERELLA
+} qsm_bg_removal_algorithm;
+
+/* ---- defaults and diagnostics ------------------------------------------ */
+
+/* TvParams::default(), src/inversion/tv.rs:34-43 */
+void qsm_tv_params_default(qsm_tv_params* p);
+/* VsharpParams::default(), src/bgremove/vsharp.rs:31-39 */
+void qsm_vsharp_params_default(qsm_vsharp_params* p);
+/* Message of the last failed call on this thread ("" if none). */
+const char* qsm_last_error_message(void);
+/* Device name, architecture and library build info into buf. */
+int qsm_backend_info(char* buf, size_t len);
+/* 1 if n is an axis length the GPU FFT supports (2^a 3^b 5^c 7^d 11^e 13^f,
+ * n <= 512), else 0. */
+int qsm_fft_length_supported(size_t n);
+
+/* ---- drop-in algorithm entry points (host pointers) -------------------- */
+
+/* inversion::tv_admm, src/inversion/tv.rs:63-70.
+ * local_field: N f64; mask: N u8; bdir: 3 f64; chi_out: N f64 (caller-owned).
+ * Progress: (iter+1, max_iter) before each iteration, (k, k) on convergence. */
+int qsm_tv_admm(const double* local_field, const uint8_t* mask, const qsm_grid* grid,
+                const double bdir[3], const qsm_tv_params* params, qsm_progress_fn progress,
+                void* user, double* chi_out);
+
+/* Same as qsm_tv_admm, also reporting the number of iterations run and the
+ * last relative change (diagnostics; no reference counterpart). */
+int qsm_tv_admm_ex(const double* local_field, const uint8_t* mask, const qsm_grid* grid,
+                   const double bdir[3], const qsm_tv_params* params, qsm_progress_fn progress,
+                   void* user, double* chi_out, size_t* iters_out, double* relchange_out);
+
+/* bgremove::vsharp, src/bgremove/vsharp.rs:55-61.
+ * local_out: N f64; mask_out: N u8 (eroded mask, 0/1). */
+int qsm_vsharp(const double* field, const uint8_t* mask, const qsm_grid* grid,
+               const qsm_vsharp_params* params, qsm_progress_fn progress, void* user,
+               double* local_out, uint8_t* mask_out);
+
+/* vsharp_with_radii, src/bgremove/vsharp.rs:88-95 (explicit radii in mm, any
+ * order; empty -> zeros + copy of mask; one radius -> SHARP). */
+int qsm_vsharp_with_radii(const double* field, const uint8_t* mask, const qsm_grid* grid,
+                          const double* radii, size_t n_radii, double threshold,
+                          qsm_progress_fn progress, void* user, double* local_out,
+                          uint8_t* mask_out);
+
+/* ---- pipeline entry points --------------------------------------------- */
+
+/* run_dipole_inversion, src/pipeline/inversion.rs:28-35.  Serves the Tv arm
+ * (tv_params); other algorithms return QSM_ECONFIG.  Lengths are checked as
+ * the reference does (DimensionMismatch -> QSM_EDIM). */
+int qsm_run_dipole_inversion(const double* local_field_ppm, size_t field_len,
+                             const uint8_t* mask, size_t mask_len,
+                             const qsm_scan_metadata* metadata, qsm_inversion_algorithm algorithm,
+                             const qsm_tv_params* tv_params, qsm_progress_fn progress,
+                             void* user, double* chi_out);
+
+/* run_bg_removal, src/pipeline/bg_removal.rs:20-26.  Serves the Vsharp arm;
+ * other algorithms return QSM_ECONFIG. */
+int qsm_run_bg_removal(const double* field_ppm, size_t field_len, const uint8_t* mask,
+                       size_t mask_len, const qsm_scan_metadata* metadata,
+                       qsm_bg_removal_algorithm algorithm,
+                       const qsm_vsharp_params* vsharp_params, qsm_progress_fn progress,
+                       void* user, double* local_out, uint8_t* mask_out);
+
+/* ---- device-resident batched TV-ADMM (no reference counterpart) -------- */
+
+/* One volume of a batch: device pointers, caller-owned. */
+typedef struct qsm_tv_volume {
+    qsm_grid grid;
+    double bdir[3];
+    const double* d_field;  /* N f64 */
+    const uint8_t* d_mask;  /* N u8 */
+    double* d_chi;          /* N f64 output */
+} qsm_tv_volume;
+
+typedef struct qsm_tv_batch qsm_tv_batch;
+
+/* Plan a batch of volumes of any (supported) shapes sharing one TvParams.
+ * stream: a hipStream_t, or NULL for a stream owned by the batch.  Allocates
+ * the device workspace once. */
+int qsm_tv_batch_create(const qsm_tv_volume* vols, size_t n_vols, const qsm_tv_params* params,
+                        void* stream, qsm_tv_batch** out);
+/* Run the whole solve (spectral setup, iterations, masking) for every volume
+ * in one launch per kernel.  Progress: (iter+1, max_iter) per iteration of
+ * the batch, (k, k) once every volume has converged.  Blocks until done. */
+int qsm_tv_batch_run(qsm_tv_batch* batch, qsm_progress_fn progress, void* user);
+/* Iterations run and last relative change of volume v after a run. */
+int qsm_tv_batch_status(const qsm_tv_batch* batch, size_t v, size_t* iters, double* relchange);
+void qsm_tv_batch_destroy(qsm_tv_batch* batch);
+/* Mean duration (ms, HIP events on the batch's stream) of each kernel of the
+ * iteration over `iters` timed iterations after a fresh spectral setup:
+ * ms_out[0] z/u update + divergence + R2C x, [1] C2C y, [2] C2C z + x-update
+ * + inverse z, [3] inverse C2C y, [4] C2R x + relative-change sums,
+ * [5] relative-change reduction.  n_out >= 6.  Leaves the batch ready for
+ * qsm_tv_batch_run. */
+int qsm_tv_batch_profile(qsm_tv_batch* batch, size_t iters, double* ms_out, size_t n_out);
+
+/* ---- device memory helpers (no reference counterpart) ------------------ */
+/* For callers that hold no HIP runtime of their own (tests, benchmarks,
+ * bindings): plain device allocations and blocking copies on device 0's
+ * null stream. */
+int qsm_device_alloc(size_t bytes, void** out);
+int qsm_device_free(void* ptr);
+int qsm_memcpy_htod(void* dst, const void* src, size_t bytes);
+int qsm_memcpy_dtoh(void* dst, const void* src, size_t bytes);
+int qsm_device_synchronize(void);
+
+/* ---- device-resident batched V-SHARP (no reference counterpart) ------- */
+
+typedef struct qsm_vsharp_volume {
+    qsm_grid grid;
+    const double* d_field;  /* N f64 total field */
+    const uint8_t* d_mask;  /* N u8 */
+    double* d_local;        /* N f64 output local field */
+    uint8_t* d_mask_out;    /* N u8 output eroded mask */
+} qsm_vsharp_volume;
+
+typedef struct qsm_vsharp_batch qsm_vsharp_batch;
+
+/* Plan V-SHARP over a batch of volumes sharing one radius list (mm, any
+ * order, at least one; one radius gives SHARP, src/bgremove/sharp.rs:64-131)
+ * and threshold.  Allocates the device workspace once. */
+int qsm_vsharp_batch_create(const qsm_vsharp_volume* vols, size_t n_vols, const double* radii,
+                            size_t n_radii, double threshold, void* stream,
+                            qsm_vsharp_batch** out);
+/* Run every volume; progress (r+1, R) per radius as vsharp does. */
+int qsm_vsharp_batch_run(qsm_vsharp_batch* batch, qsm_progress_fn progress, void* user);
+void qsm_vsharp_batch_destroy(qsm_vsharp_batch* batch);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* QSM_GPU_H */

@@ -1,0 +1,520 @@
+"""Host-side mirror of QSM.rs's TV-ADMM / V-SHARP API over the gfx950 C ABI.
+
+Every public name mirrors the reference crate (qsm-core):
+
+=====================================  =========================================
+this module                            reference
+=====================================  =========================================
+``Grid``                               ``Grid`` (src/grid.rs:9-44)
+``TvParams``                           ``inversion::TvParams`` (src/inversion/tv.rs:21-43)
+``VsharpParams``                       ``bgremove::VsharpParams`` (src/bgremove/vsharp.rs:20-39)
+``tv_admm``                            ``inversion::tv_admm`` (src/inversion/tv.rs:63-97)
+``vsharp``                             ``bgremove::vsharp`` (src/bgremove/vsharp.rs:55-64)
+``vsharp_with_radii``                  ``vsharp_with_radii`` (src/bgremove/vsharp.rs:88-199)
+``ScanMetadata``, ``InversionConfig``,
+``BgRemovalConfig``, ``BgRemovalResult``,
+``PipelineError``                      ``pipeline::config`` (src/pipeline/config.rs)
+``run_dipole_inversion``               src/pipeline/inversion.rs:28-114
+``run_bg_removal``                     src/pipeline/bg_removal.rs:20-92
+=====================================  =========================================
+
+Arrays are flat numpy vectors in Fortran voxel order (``idx = i + j*nx +
+k*nx*ny``); a 3-D array is accepted and flattened with ``order="F"``.  All
+computation runs in ``libqsm_gpu.so`` on the GPU; there is no CPU fallback:
+importing this package fails if the library has not been built.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import enum
+import os
+from dataclasses import dataclass, field
+from typing import Callable, Optional, Sequence
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libqsm_gpu.so")
+
+if not os.path.exists(LIB_PATH):
+    raise ImportError(
+        f"{LIB_PATH} is missing: build it with `python qsm.rs_amd/build.py` "
+        "(or __graft_entry__.build()); there is no CPU fallback")
+
+_lib = C.CDLL(LIB_PATH)
+
+# ---------------------------------------------------------------- C types --
+QSM_OK, QSM_EDIM, QSM_EPARAM, QSM_EHIP, QSM_ENOMEM, QSM_ECONFIG, QSM_EUNSUPPORTED = range(7)
+
+PROGRESS_FN = C.CFUNCTYPE(None, C.c_size_t, C.c_size_t, C.c_void_p)
+
+
+class _CGrid(C.Structure):
+    _fields_ = [("nx", C.c_size_t), ("ny", C.c_size_t), ("nz", C.c_size_t),
+                ("vsx", C.c_double), ("vsy", C.c_double), ("vsz", C.c_double)]
+
+
+class _CTvParams(C.Structure):
+    _fields_ = [("lambda_", C.c_double), ("rho", C.c_double), ("tol", C.c_double),
+                ("max_iter", C.c_size_t)]
+
+
+class _CVsharpParams(C.Structure):
+    _fields_ = [("threshold", C.c_double), ("max_radius", C.c_double),
+                ("min_radius", C.c_double)]
+
+
+class _CScanMetadata(C.Structure):
+    _fields_ = [("nx", C.c_size_t), ("ny", C.c_size_t), ("nz", C.c_size_t),
+                ("vsx", C.c_double), ("vsy", C.c_double), ("vsz", C.c_double),
+                ("echo_times", C.POINTER(C.c_double)), ("n_echo_times", C.c_size_t),
+                ("field_strength", C.c_double), ("b0_direction", C.c_double * 3)]
+
+
+class _CTvVolume(C.Structure):
+    _fields_ = [("grid", _CGrid), ("bdir", C.c_double * 3), ("d_field", C.c_void_p),
+                ("d_mask", C.c_void_p), ("d_chi", C.c_void_p)]
+
+
+_D = C.POINTER(C.c_double)
+_U8 = C.POINTER(C.c_uint8)
+_SZ = C.c_size_t
+
+_SIGS = {
+    "qsm_tv_params_default": (None, [C.POINTER(_CTvParams)]),
+    "qsm_vsharp_params_default": (None, [C.POINTER(_CVsharpParams)]),
+    "qsm_last_error_message": (C.c_char_p, []),
+    "qsm_backend_info": (C.c_int, [C.c_char_p, _SZ]),
+    "qsm_fft_length_supported": (C.c_int, [_SZ]),
+    "qsm_tv_admm": (C.c_int, [_D, _U8, C.POINTER(_CGrid), _D, C.POINTER(_CTvParams), PROGRESS_FN,
+                              C.c_void_p, _D]),
+    "qsm_tv_admm_ex": (C.c_int, [_D, _U8, C.POINTER(_CGrid), _D, C.POINTER(_CTvParams),
+                                 PROGRESS_FN, C.c_void_p, _D, C.POINTER(_SZ), _D]),
+    "qsm_vsharp": (C.c_int, [_D, _U8, C.POINTER(_CGrid), C.POINTER(_CVsharpParams), PROGRESS_FN,
+                             C.c_void_p, _D, _U8]),
+    "qsm_vsharp_with_radii": (C.c_int, [_D, _U8, C.POINTER(_CGrid), _D, _SZ, C.c_double,
+                                        PROGRESS_FN, C.c_void_p, _D, _U8]),
+    "qsm_run_dipole_inversion": (C.c_int, [_D, _SZ, _U8, _SZ, C.POINTER(_CScanMetadata), C.c_int,
+                                           C.POINTER(_CTvParams), PROGRESS_FN, C.c_void_p, _D]),
+    "qsm_run_bg_removal": (C.c_int, [_D, _SZ, _U8, _SZ, C.POINTER(_CScanMetadata), C.c_int,
+                                     C.POINTER(_CVsharpParams), PROGRESS_FN, C.c_void_p, _D, _U8]),
+    "qsm_tv_batch_create": (C.c_int, [C.POINTER(_CTvVolume), _SZ, C.POINTER(_CTvParams),
+                                      C.c_void_p, C.POINTER(C.c_void_p)]),
+    "qsm_tv_batch_run": (C.c_int, [C.c_void_p, PROGRESS_FN, C.c_void_p]),
+    "qsm_tv_batch_status": (C.c_int, [C.c_void_p, _SZ, C.POINTER(_SZ), _D]),
+    "qsm_tv_batch_destroy": (None, [C.c_void_p]),
+    "qsm_tv_batch_profile": (C.c_int, [C.c_void_p, _SZ, _D, _SZ]),
+    "qsm_device_alloc": (C.c_int, [_SZ, C.POINTER(C.c_void_p)]),
+    "qsm_device_free": (C.c_int, [C.c_void_p]),
+    "qsm_memcpy_htod": (C.c_int, [C.c_void_p, C.c_void_p, _SZ]),
+    "qsm_memcpy_dtoh": (C.c_int, [C.c_void_p, C.c_void_p, _SZ]),
+    "qsm_device_synchronize": (C.c_int, []),
+}
+for _name, (_res, _args) in _SIGS.items():
+    _fn = getattr(_lib, _name)
+    _fn.restype = _res
+    _fn.argtypes = _args
+
+
+def c_library() -> C.CDLL:
+    """The loaded libqsm_gpu.so."""
+    return _lib
+
+
+class QsmError(RuntimeError):
+    """A non-zero status from the C ABI."""
+
+    def __init__(self, code: int, message: str):
+        super().__init__(f"[{code}] {message}")
+        self.code = code
+        self.message = message
+
+
+def _check(rc: int):
+    if rc != QSM_OK:
+        raise QsmError(rc, _lib.qsm_last_error_message().decode())
+
+
+def backend_info() -> str:
+    buf = C.create_string_buffer(512)
+    _check(_lib.qsm_backend_info(buf, 512))
+    return buf.value.decode()
+
+
+def fft_length_supported(n: int) -> bool:
+    return bool(_lib.qsm_fft_length_supported(n))
+
+
+# ------------------------------------------------------------------ types --
+@dataclass(frozen=True)
+class Grid:
+    """src/grid.rs:9-44"""
+    dims: tuple
+    voxel_size: tuple = (1.0, 1.0, 1.0)
+
+    @staticmethod
+    def new(nx, ny, nz, vsx, vsy, vsz) -> "Grid":
+        return Grid((int(nx), int(ny), int(nz)), (float(vsx), float(vsy), float(vsz)))
+
+    def nx(self): return self.dims[0]
+    def ny(self): return self.dims[1]
+    def nz(self): return self.dims[2]
+    def vsx(self): return self.voxel_size[0]
+    def vsy(self): return self.voxel_size[1]
+    def vsz(self): return self.voxel_size[2]
+
+    def n_total(self) -> int:
+        return self.dims[0] * self.dims[1] * self.dims[2]
+
+    def _c(self) -> _CGrid:
+        return _CGrid(*[int(d) for d in self.dims], *[float(v) for v in self.voxel_size])
+
+
+@dataclass
+class TvParams:
+    """src/inversion/tv.rs:21-43 (``lambda`` is spelled ``lambda_``)."""
+    lambda_: float = 2e-4
+    rho: float = 2e-2
+    tol: float = 1e-3
+    max_iter: int = 250
+
+    @staticmethod
+    def default() -> "TvParams":
+        c = _CTvParams()
+        _lib.qsm_tv_params_default(C.byref(c))
+        return TvParams(c.lambda_, c.rho, c.tol, int(c.max_iter))
+
+    def _c(self) -> _CTvParams:
+        return _CTvParams(float(self.lambda_), float(self.rho), float(self.tol), int(self.max_iter))
+
+
+@dataclass
+class VsharpParams:
+    """src/bgremove/vsharp.rs:20-39"""
+    threshold: float = 0.05
+    max_radius: float = 12.0
+    min_radius: float = 1.0
+
+    @staticmethod
+    def default() -> "VsharpParams":
+        c = _CVsharpParams()
+        _lib.qsm_vsharp_params_default(C.byref(c))
+        return VsharpParams(c.threshold, c.max_radius, c.min_radius)
+
+    def _c(self) -> _CVsharpParams:
+        return _CVsharpParams(float(self.threshold), float(self.max_radius), float(self.min_radius))
+
+
+def _flat64(a) -> np.ndarray:
+    a = np.asarray(a)
+    if a.ndim == 3:
+        a = a.ravel(order="F")
+    return np.ascontiguousarray(a.ravel(), dtype=np.float64)
+
+
+def _flatu8(a) -> np.ndarray:
+    a = np.asarray(a)
+    if a.ndim == 3:
+        a = a.ravel(order="F")
+    return np.ascontiguousarray(a.ravel(), dtype=np.uint8)
+
+
+def _progress(cb: Optional[Callable[[int, int], None]]):
+    if cb is None:
+        return PROGRESS_FN(0)
+    return PROGRESS_FN(lambda cur, tot, _u: cb(int(cur), int(tot)))
+
+
+def _check_len(grid: Grid, *arrays):
+    n = grid.n_total()
+    for a in arrays:
+        if a.size != n:
+            # the reference panics on short slices (index out of bounds)
+            raise ValueError(f"array of length {a.size} does not match grid of {n} voxels")
+
+
+# ------------------------------------------------------------- algorithms --
+def tv_admm(local_field, mask, grid: Grid, bdir, params: TvParams,
+            progress: Optional[Callable[[int, int], None]] = None,
+            return_info: bool = False):
+    """``inversion::tv_admm`` (src/inversion/tv.rs:63-97) on the GPU.
+
+    FP32 FFTs and ADMM state, FP64 dipole/inverse-operator construction and
+    FP64 convergence norms.  Returns chi (float64, N); with ``return_info``
+    also (iterations run, last relative change)."""
+    f = _flat64(local_field)
+    m = _flatu8(mask)
+    _check_len(grid, f, m)
+    out = np.empty(f.size)
+    b = np.ascontiguousarray(np.asarray(bdir, dtype=np.float64))
+    g, p = grid._c(), params._c()
+    it, rel = _SZ(0), C.c_double(0.0)
+    cb = _progress(progress)
+    _check(_lib.qsm_tv_admm_ex(f.ctypes.data_as(_D), m.ctypes.data_as(_U8), C.byref(g),
+                               b.ctypes.data_as(_D), C.byref(p), cb, None,
+                               out.ctypes.data_as(_D), C.byref(it), C.byref(rel)))
+    if return_info:
+        return out, int(it.value), float(rel.value)
+    return out
+
+
+def vsharp(field, mask, grid: Grid, params: VsharpParams,
+           progress: Optional[Callable[[int, int], None]] = None):
+    """``bgremove::vsharp`` (src/bgremove/vsharp.rs:55-64) on the GPU.
+    Returns (local_field float64 N, eroded_mask uint8 N)."""
+    f = _flat64(field)
+    m = _flatu8(mask)
+    _check_len(grid, f, m)
+    local = np.empty(f.size)
+    mout = np.empty(f.size, dtype=np.uint8)
+    g, p = grid._c(), params._c()
+    cb = _progress(progress)
+    _check(_lib.qsm_vsharp(f.ctypes.data_as(_D), m.ctypes.data_as(_U8), C.byref(g), C.byref(p), cb,
+                           None, local.ctypes.data_as(_D), mout.ctypes.data_as(_U8)))
+    return local, mout
+
+
+def vsharp_with_radii(field, mask, grid: Grid, radii: Sequence[float], threshold: float,
+                      progress: Optional[Callable[[int, int], None]] = None):
+    """``vsharp_with_radii`` (src/bgremove/vsharp.rs:88-199) on the GPU."""
+    f = _flat64(field)
+    m = _flatu8(mask)
+    _check_len(grid, f, m)
+    r = np.ascontiguousarray(np.asarray(radii, dtype=np.float64).ravel())
+    local = np.empty(f.size)
+    mout = np.empty(f.size, dtype=np.uint8)
+    g = grid._c()
+    cb = _progress(progress)
+    rp = r.ctypes.data_as(_D) if r.size else None
+    _check(_lib.qsm_vsharp_with_radii(f.ctypes.data_as(_D), m.ctypes.data_as(_U8), C.byref(g), rp,
+                                      r.size, float(threshold), cb, None,
+                                      local.ctypes.data_as(_D), mout.ctypes.data_as(_U8)))
+    return local, mout
+
+
+# --------------------------------------------------------------- pipeline --
+class InversionAlgorithm(enum.IntEnum):
+    """src/pipeline/config.rs:47-58"""
+    Tkd = 0
+    Tsvd = 1
+    Tikhonov = 2
+    Tv = 3
+    Rts = 4
+    Nltv = 5
+    Medi = 6
+    Ilsqr = 7
+    Tgv = 8
+    Qsmart = 9
+
+
+class BgRemovalAlgorithm(enum.IntEnum):
+    """src/pipeline/config.rs:33-43"""
+    Vsharp = 0
+    Pdf = 1
+    Lbv = 2
+    Ismv = 3
+    Sharp = 4
+    Resharp = 5
+    Harperella = 6
+    Iharperella = 7
+
+
+@dataclass
+class ScanMetadata:
+    """src/pipeline/config.rs:300-318"""
+    dims: tuple
+    voxel_size: tuple
+    echo_times: list = field(default_factory=lambda: [0.005])
+    field_strength: float = 3.0
+    b0_direction: tuple = (0.0, 0.0, 1.0)
+
+    def grid(self) -> Grid:
+        return Grid(tuple(self.dims), tuple(self.voxel_size))
+
+    def _c(self):
+        et = np.ascontiguousarray(np.asarray(self.echo_times, dtype=np.float64))
+        c = _CScanMetadata(*[int(d) for d in self.dims], *[float(v) for v in self.voxel_size],
+                           et.ctypes.data_as(_D) if et.size else None, et.size,
+                           float(self.field_strength),
+                           (C.c_double * 3)(*[float(v) for v in self.b0_direction]))
+        return c, et
+
+
+@dataclass
+class InversionConfig:
+    """src/pipeline/config.rs:236-264 (the Tv arm is the one with a GPU backend)."""
+    algorithm: InversionAlgorithm = InversionAlgorithm.Rts
+    tv: TvParams = field(default_factory=TvParams)
+
+
+@dataclass
+class BgRemovalConfig:
+    """src/pipeline/config.rs:202-230 (the Vsharp arm is the one with a GPU backend)."""
+    algorithm: BgRemovalAlgorithm = BgRemovalAlgorithm.Vsharp
+    vsharp: VsharpParams = field(default_factory=VsharpParams)
+
+
+@dataclass
+class BgRemovalResult:
+    """src/pipeline/config.rs:336-342"""
+    local_field_ppm: np.ndarray
+    eroded_mask: np.ndarray
+
+
+class PipelineError(Exception):
+    """src/pipeline/config.rs:344-368: kind is one of InvalidConfig,
+    InvalidInput, AlgorithmError, DimensionMismatch."""
+
+    def __init__(self, kind: str, message: str, expected: int = None, got: int = None):
+        super().__init__(f"{kind}: {message}")
+        self.kind = kind
+        self.expected = expected
+        self.got = got
+
+
+def _pipeline_error(e: QsmError, expected=None, got=None) -> PipelineError:
+    kind = {QSM_EDIM: "DimensionMismatch", QSM_ECONFIG: "InvalidConfig",
+            QSM_EPARAM: "InvalidInput"}.get(e.code, "AlgorithmError")
+    return PipelineError(kind, e.message, expected, got)
+
+
+def run_dipole_inversion(local_field_ppm, mask, metadata: ScanMetadata, config: InversionConfig,
+                         magnitude=None, progress: Optional[Callable[[int, int], None]] = None):
+    """src/pipeline/inversion.rs:28-114 -- length check, then the Tv arm on the
+    GPU; other arms raise PipelineError('InvalidConfig')."""
+    f = _flat64(local_field_ppm)
+    m = _flatu8(mask)
+    md, _keep = metadata._c()
+    p = config.tv._c()
+    out = np.empty(f.size)
+    cb = _progress(progress)
+    rc = _lib.qsm_run_dipole_inversion(f.ctypes.data_as(_D), f.size, m.ctypes.data_as(_U8), m.size,
+                                       C.byref(md), int(config.algorithm), C.byref(p), cb, None,
+                                       out.ctypes.data_as(_D))
+    if rc != QSM_OK:
+        n = int(np.prod(metadata.dims))
+        raise _pipeline_error(QsmError(rc, _lib.qsm_last_error_message().decode()), n, f.size)
+    return out
+
+
+def run_bg_removal(field_ppm, mask, metadata: ScanMetadata, config: BgRemovalConfig,
+                   progress: Optional[Callable[[int, int], None]] = None) -> BgRemovalResult:
+    """src/pipeline/bg_removal.rs:20-92 -- length check, then the Vsharp arm on
+    the GPU; other arms raise PipelineError('InvalidConfig')."""
+    f = _flat64(field_ppm)
+    m = _flatu8(mask)
+    md, _keep = metadata._c()
+    p = config.vsharp._c()
+    local = np.empty(f.size)
+    mout = np.empty(f.size, dtype=np.uint8)
+    cb = _progress(progress)
+    rc = _lib.qsm_run_bg_removal(f.ctypes.data_as(_D), f.size, m.ctypes.data_as(_U8), m.size,
+                                 C.byref(md), int(config.algorithm), C.byref(p), cb, None,
+                                 local.ctypes.data_as(_D), mout.ctypes.data_as(_U8))
+    if rc != QSM_OK:
+        n = int(np.prod(metadata.dims))
+        raise _pipeline_error(QsmError(rc, _lib.qsm_last_error_message().decode()), n, f.size)
+    return BgRemovalResult(local, mout)
+
+
+# ---------------------------------------------------------- device memory --
+class DeviceArray:
+    """A device allocation owned by this library's HIP runtime, holding a
+    1-D numpy dtype.  ``ptr`` is the raw device address."""
+
+    def __init__(self, n: int, dtype):
+        self.dtype = np.dtype(dtype)
+        self.n = int(n)
+        p = C.c_void_p()
+        _check(_lib.qsm_device_alloc(self.n * self.dtype.itemsize, C.byref(p)))
+        self.ptr = p.value
+
+    @staticmethod
+    def from_numpy(a) -> "DeviceArray":
+        a = np.ascontiguousarray(a)
+        d = DeviceArray(a.size, a.dtype)
+        d.upload(a)
+        return d
+
+    def upload(self, a):
+        a = np.ascontiguousarray(a, dtype=self.dtype)
+        assert a.size == self.n
+        _check(_lib.qsm_memcpy_htod(C.c_void_p(self.ptr), a.ctypes.data_as(C.c_void_p), a.nbytes))
+
+    def numpy(self) -> np.ndarray:
+        out = np.empty(self.n, dtype=self.dtype)
+        _check(_lib.qsm_memcpy_dtoh(out.ctypes.data_as(C.c_void_p), C.c_void_p(self.ptr), out.nbytes))
+        return out
+
+    def free(self):
+        if getattr(self, "ptr", None):
+            _lib.qsm_device_free(C.c_void_p(self.ptr))
+            self.ptr = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+def synchronize():
+    _check(_lib.qsm_device_synchronize())
+
+
+# ------------------------------------------------ device-resident batches --
+class TvBatch:
+    """Batched TV-ADMM over device-resident volumes (``qsm_tv_batch_*``).
+
+    ``volumes`` is a list of (grid, bdir, d_field_ptr, d_mask_ptr, d_chi_ptr)
+    with raw device addresses (f64 field, u8 mask, f64 output); the caller
+    keeps the memory alive.  ``stream`` is a raw hipStream_t or None."""
+
+    def __init__(self, volumes, params: TvParams, stream: Optional[int] = None):
+        arr = (_CTvVolume * len(volumes))()
+        for i, (grid, bdir, pf, pm, pc) in enumerate(volumes):
+            arr[i].grid = grid._c()
+            arr[i].bdir = (C.c_double * 3)(*[float(v) for v in bdir])
+            arr[i].d_field, arr[i].d_mask, arr[i].d_chi = int(pf), int(pm), int(pc)
+        self._p = params._c()
+        h = C.c_void_p()
+        _check(_lib.qsm_tv_batch_create(arr, len(volumes), C.byref(self._p),
+                                        C.c_void_p(stream) if stream else None, C.byref(h)))
+        self._h = h
+        self.n = len(volumes)
+
+    def run(self, progress: Optional[Callable[[int, int], None]] = None):
+        cb = _progress(progress)
+        _check(_lib.qsm_tv_batch_run(self._h, cb, None))
+
+    def status(self, v: int):
+        it, rel = _SZ(0), C.c_double(0.0)
+        _check(_lib.qsm_tv_batch_status(self._h, v, C.byref(it), C.byref(rel)))
+        return int(it.value), float(rel.value)
+
+    def profile(self, iters: int = 20):
+        """Mean per-launch milliseconds of the six kernels of one iteration
+        (see qsm_tv_batch_profile in include/qsm_gpu.h)."""
+        out = np.zeros(6)
+        _check(_lib.qsm_tv_batch_profile(self._h, iters, out.ctypes.data_as(_D), 6))
+        return out
+
+    def close(self):
+        if getattr(self, "_h", None):
+            _lib.qsm_tv_batch_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+__all__ = [
+    "Grid", "TvParams", "VsharpParams", "tv_admm", "vsharp", "vsharp_with_radii",
+    "ScanMetadata", "InversionAlgorithm", "BgRemovalAlgorithm", "InversionConfig",
+    "BgRemovalConfig", "BgRemovalResult", "PipelineError", "run_dipole_inversion",
+    "run_bg_removal", "TvBatch", "QsmError", "backend_info", "fft_length_supported",
+    "c_library", "LIB_PATH", "DeviceArray", "synchronize",
+]

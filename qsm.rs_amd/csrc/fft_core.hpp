@@ -1,0 +1,318 @@
+// Mixed-radix Stockham FFT building blocks for gfx950.
+//
+// A workgroup holds `ncols` independent sequences of length n in LDS, element
+// e of sequence c at lds[e*es + c*cs].  Each stage of radix R reads the R
+// inputs of every butterfly into registers, waits at a barrier, then applies
+// twiddles, a radix-R DFT and writes the R outputs back to their Stockham
+// (self-sorting) positions, so the transform ends in natural order with no
+// bit-reversal pass.  For butterfly j of a stage where Ns is the product of
+// the radices already applied:
+//   inputs   e = j + r*(n/R)                          r = 0..R-1
+//   twiddle  w = exp(-+2 pi i * (j mod Ns) * r / (Ns*R))
+//   outputs  e = (j / Ns)*Ns*R + (j mod Ns) + r*Ns
+// (the Stockham autosort formulation of a decimation-in-time FFT).
+//
+// Everything index-related is __host__ __device__ so the same code can be
+// run one simulated thread at a time on the CPU and compared against a
+// reference DFT before it ever runs on the GPU.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace qsmfft {
+
+constexpr int kMaxStages = 10;
+constexpr int kThreads = 256;  // threads per workgroup for every FFT kernel
+
+// Per-length plan.  Stored in device memory; read with uniform loads.
+struct Plan {
+    int n;
+    int nstages;
+    int radix[kMaxStages];
+    int tw_off;  // offset (in float2) of this length's table exp(-2 pi i q/n), q<n
+};
+
+__host__ __device__ inline float2 cmul(float2 a, float2 b) {
+    return make_float2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+__host__ __device__ inline float2 cadd(float2 a, float2 b) { return make_float2(a.x + b.x, a.y + b.y); }
+__host__ __device__ inline float2 csub(float2 a, float2 b) { return make_float2(a.x - b.x, a.y - b.y); }
+__host__ __device__ inline float2 cconj(float2 a) { return make_float2(a.x, -a.y); }
+__host__ __device__ inline float2 cscale(float2 a, float s) { return make_float2(a.x * s, a.y * s); }
+// multiply by -i (forward) or +i (inverse)
+template <bool INV>
+__host__ __device__ inline float2 mul_mi(float2 a) {
+    return INV ? make_float2(-a.y, a.x) : make_float2(a.y, -a.x);
+}
+
+// ---------------------------------------------------------------- DFT(R) --
+// In-register DFT of R points; forward uses exp(-2 pi i/R), inverse exp(+).
+template <int R, bool INV>
+struct Dft;
+
+template <bool INV>
+struct Dft<2, INV> {
+    __host__ __device__ static inline void run(float2* v, const float2*, int) {
+        float2 a = v[0], b = v[1];
+        v[0] = cadd(a, b);
+        v[1] = csub(a, b);
+    }
+};
+
+template <bool INV>
+struct Dft<3, INV> {
+    __host__ __device__ static inline void run(float2* v, const float2*, int) {
+        const float c = -0.5f;
+        const float s = 0.86602540378443864676f;  // sin(2 pi/3)
+        float2 a = v[0], b = v[1], d = v[2];
+        float2 t = cadd(b, d);
+        float2 m = cadd(a, cscale(t, c));
+        float2 q = cscale(csub(b, d), s);  // X1 = m - i q (fwd), X2 = m + i q
+        v[0] = cadd(a, t);
+        v[1] = cadd(m, mul_mi<INV>(q));
+        v[2] = csub(m, mul_mi<INV>(q));
+    }
+};
+
+template <bool INV>
+struct Dft<4, INV> {
+    __host__ __device__ static inline void run(float2* v, const float2*, int) {
+        float2 a = cadd(v[0], v[2]), b = csub(v[0], v[2]);
+        float2 c = cadd(v[1], v[3]), d = mul_mi<INV>(csub(v[1], v[3]));
+        v[0] = cadd(a, c);
+        v[1] = cadd(b, d);
+        v[2] = csub(a, c);
+        v[3] = csub(b, d);
+    }
+};
+
+template <bool INV>
+struct Dft<5, INV> {
+    __host__ __device__ static inline void run(float2* v, const float2*, int) {
+        const float c1 = 0.30901699437494742410f;   // cos(2 pi/5)
+        const float c2 = -0.80901699437494742410f;  // cos(4 pi/5)
+        const float s1 = 0.95105651629515357212f;   // sin(2 pi/5)
+        const float s2 = 0.58778525229247312917f;   // sin(4 pi/5)
+        float2 x0 = v[0];
+        float2 t1 = cadd(v[1], v[4]), t2 = cadd(v[2], v[3]);
+        float2 t3 = csub(v[1], v[4]), t4 = csub(v[2], v[3]);
+        float2 a1 = cadd(x0, cadd(cscale(t1, c1), cscale(t2, c2)));
+        float2 a2 = cadd(x0, cadd(cscale(t1, c2), cscale(t2, c1)));
+        float2 b1 = cadd(cscale(t3, s1), cscale(t4, s2));
+        float2 b2 = csub(cscale(t3, s2), cscale(t4, s1));
+        v[0] = cadd(x0, cadd(t1, t2));
+        v[1] = cadd(a1, mul_mi<INV>(b1));
+        v[4] = csub(a1, mul_mi<INV>(b1));
+        v[2] = cadd(a2, mul_mi<INV>(b2));
+        v[3] = csub(a2, mul_mi<INV>(b2));
+    }
+};
+
+template <bool INV>
+struct Dft<8, INV> {
+    __host__ __device__ static inline void run(float2* v, const float2*, int) {
+        const float h = 0.70710678118654752440f;
+        float2 e[4] = {v[0], v[2], v[4], v[6]};
+        float2 o[4] = {v[1], v[3], v[5], v[7]};
+        Dft<4, INV>::run(e, nullptr, 0);
+        Dft<4, INV>::run(o, nullptr, 0);
+        // W8^1 = h(1 -+ i), W8^2 = -+i, W8^3 = h(-1 -+ i)
+        float2 w1 = INV ? make_float2(h * (o[1].x - o[1].y), h * (o[1].x + o[1].y))
+                        : make_float2(h * (o[1].x + o[1].y), h * (o[1].y - o[1].x));
+        float2 w2 = mul_mi<INV>(o[2]);
+        float2 w3 = INV ? make_float2(-h * (o[3].x + o[3].y), h * (o[3].x - o[3].y))
+                        : make_float2(h * (o[3].y - o[3].x), -h * (o[3].x + o[3].y));
+        v[0] = cadd(e[0], o[0]);
+        v[4] = csub(e[0], o[0]);
+        v[1] = cadd(e[1], w1);
+        v[5] = csub(e[1], w1);
+        v[2] = cadd(e[2], w2);
+        v[6] = csub(e[2], w2);
+        v[3] = cadd(e[3], w3);
+        v[7] = csub(e[3], w3);
+    }
+};
+
+// Generic odd prime radix: direct DFT with roots taken from the length-n
+// twiddle table (R divides n, so exp(-2 pi i m/R) = tw[m*n/R]).
+template <int R, bool INV>
+struct DftGeneric {
+    __host__ __device__ static inline void run(float2* v, const float2* tw, int n) {
+        float2 in[R];
+#pragma unroll
+        for (int r = 0; r < R; ++r) in[r] = v[r];
+        const int st = n / R;
+#pragma unroll
+        for (int q = 0; q < R; ++q) {
+            float2 acc = in[0];
+#pragma unroll
+            for (int r = 1; r < R; ++r) {
+                float2 w = tw[((r * q) % R) * st];
+                if (INV) w = cconj(w);
+                acc = cadd(acc, cmul(in[r], w));
+            }
+            v[q] = acc;
+        }
+    }
+};
+template <bool INV> struct Dft<7, INV> : DftGeneric<7, INV> {};
+template <bool INV> struct Dft<11, INV> : DftGeneric<11, INV> {};
+template <bool INV> struct Dft<13, INV> : DftGeneric<13, INV> {};
+
+// --------------------------------------------------------- index helpers --
+// Exact q = j / d for j, d < 2^16 via m = ceil(2^32 / d).
+__host__ __device__ inline uint64_t magic_for(uint32_t d) {
+    return ((uint64_t(1) << 32) + d - 1) / d;
+}
+__host__ __device__ inline uint32_t fastdiv(uint32_t j, uint64_t m) {
+    return (uint32_t)(((uint64_t)j * m) >> 32);
+}
+
+// Geometry of one stage.  COLFAST: consecutive threads take consecutive
+// sequences (column tiles, LOG2C = log2(ncols)); otherwise consecutive
+// threads take consecutive butterflies of one sequence (rows).
+struct StageGeom {
+    int n, ncols, es, cs;
+    int nb;           // n / R
+    int Ns;           // product of radices already applied
+    int tstep;        // n / (Ns * R)
+    uint64_t m_nb;    // magic for / nb
+    uint64_t m_Ns;    // magic for / Ns
+};
+
+__host__ __device__ inline StageGeom make_geom(int n, int ncols, int es, int cs, int R, int Ns) {
+    StageGeom g;
+    g.n = n; g.ncols = ncols; g.es = es; g.cs = cs;
+    g.nb = n / R; g.Ns = Ns; g.tstep = n / (Ns * R);
+    g.m_nb = magic_for((uint32_t)g.nb);
+    g.m_Ns = magic_for((uint32_t)Ns);
+    return g;
+}
+
+template <bool COLFAST, int LOG2C>
+__host__ __device__ inline void split_b(int b, const StageGeom& g, int& j, int& c) {
+    if (COLFAST) {
+        c = b & ((1 << LOG2C) - 1);
+        j = b >> LOG2C;
+    } else {
+        c = (int)fastdiv((uint32_t)b, g.m_nb);
+        j = b - c * g.nb;
+    }
+}
+
+// Load phase of a stage: thread tid gathers the inputs of its butterflies.
+template <int R, int BMAX, bool COLFAST, int LOG2C>
+__host__ __device__ inline void stage_load(const float2* lds, int tid, const StageGeom& g,
+                                           float2 (&v)[BMAX][R]) {
+    const int total = g.nb * g.ncols;
+#pragma unroll
+    for (int t = 0; t < BMAX; ++t) {
+        int b = tid + t * kThreads;
+        if (b < total) {
+            int j, c;
+            split_b<COLFAST, LOG2C>(b, g, j, c);
+            const float2* p = lds + j * g.es + c * g.cs;
+#pragma unroll
+            for (int r = 0; r < R; ++r) v[t][r] = p[r * g.nb * g.es];
+        }
+    }
+}
+
+// Compute + store phase of a stage.
+template <int R, int BMAX, bool COLFAST, int LOG2C, bool INV>
+__host__ __device__ inline void stage_store(float2* lds, int tid, const StageGeom& g,
+                                            float2 (&v)[BMAX][R], const float2* tw) {
+    const int total = g.nb * g.ncols;
+#pragma unroll
+    for (int t = 0; t < BMAX; ++t) {
+        int b = tid + t * kThreads;
+        if (b < total) {
+            int j, c;
+            split_b<COLFAST, LOG2C>(b, g, j, c);
+            int grp = (int)fastdiv((uint32_t)j, g.m_Ns);
+            int k = j - grp * g.Ns;
+            if (k != 0) {
+                int ts = k * g.tstep;
+#pragma unroll
+                for (int r = 1; r < R; ++r) {
+                    float2 w = tw[r * ts];
+                    if (INV) w = cconj(w);
+                    v[t][r] = cmul(v[t][r], w);
+                }
+            }
+            Dft<R, INV>::run(v[t], tw, g.n);
+            float2* p = lds + (grp * g.Ns * R + k) * g.es + c * g.cs;
+#pragma unroll
+            for (int r = 0; r < R; ++r) p[r * g.Ns * g.es] = v[t][r];
+        }
+    }
+}
+
+// Elements per thread the register staging is sized for.
+template <int R, int EMAX>
+struct Bmax {
+    static constexpr int value = (EMAX + R - 1) / R;
+};
+
+#ifdef __HIPCC__
+// Full in-LDS transform on the device.  Caller must have synchronised after
+// filling LDS; returns after a final barrier.  n*ncols <= kThreads*EMAX.
+template <int EMAX, bool COLFAST, int LOG2C, bool INV>
+__device__ inline void lds_fft(float2* lds, const Plan& P, int ncols, int es, int cs,
+                               const float2* tw) {
+    const int tid = threadIdx.x;
+    int Ns = 1;
+    for (int s = 0; s < P.nstages; ++s) {
+        const int R = P.radix[s];
+        const StageGeom g = make_geom(P.n, ncols, es, cs, R, Ns);
+#define QSM_STAGE(RR)                                                         \
+    case RR: {                                                                \
+        float2 v[Bmax<RR, EMAX>::value][RR];                                  \
+        stage_load<RR, Bmax<RR, EMAX>::value, COLFAST, LOG2C>(lds, tid, g, v); \
+        __syncthreads();                                                      \
+        stage_store<RR, Bmax<RR, EMAX>::value, COLFAST, LOG2C, INV>(lds, tid, g, v, tw); \
+        __syncthreads();                                                      \
+    } break;
+        switch (R) {
+            QSM_STAGE(2)
+            QSM_STAGE(3)
+            QSM_STAGE(4)
+            QSM_STAGE(5)
+            QSM_STAGE(7)
+            QSM_STAGE(8)
+            QSM_STAGE(11)
+            QSM_STAGE(13)
+            default: break;
+        }
+#undef QSM_STAGE
+        Ns *= R;
+    }
+}
+#endif
+
+// --------------------------------------------------------- host planning --
+// Radix sequence for length n: 8s first, then 4, 2, 3, 5, 7, 11, 13.
+// Returns false if n has a prime factor outside {2,3,5,7,11,13} or needs
+// more than kMaxStages stages.
+inline bool factorize(int n, int* radix, int* nstages) {
+    int m = n, s = 0;
+    if (n < 1) return false;
+    auto push = [&](int r) {
+        if (s >= kMaxStages) return false;
+        radix[s++] = r;
+        m /= r;
+        return true;
+    };
+    while (m % 8 == 0) if (!push(8)) return false;
+    while (m % 4 == 0) if (!push(4)) return false;
+    while (m % 2 == 0) if (!push(2)) return false;
+    const int odd[] = {3, 5, 7, 11, 13};
+    for (int p : odd)
+        while (m % p == 0) if (!push(p)) return false;
+    if (m != 1) return false;
+    *nstages = s;
+    return true;
+}
+
+}  // namespace qsmfft

@@ -1,0 +1,86 @@
+// Shared device/host structures of the gfx950 QSM library.
+//
+// Data layout in HBM (per volume, FP32):
+//   real volumes   x, field, ...  : nx*ny*nz floats, Fortran order
+//                                   (idx = i + j*nx + k*nx*ny)
+//   dual variable  u              : 3 planes of N floats (ux | uy | uz)
+//   half spectrum  spec, F, ...   : nxh*ny*nz complex (float2), nxh = nx/2+1,
+//                                   idx = kx + ky*nxh + kz*nxh*ny
+// Every kernel is launched once for a whole batch of volumes; a LaunchTable
+// maps the flat block index to (volume, block-within-volume).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "fft_core.hpp"
+
+namespace qsm {
+
+constexpr int kMaxVolumes = 64;
+constexpr int kTile = 16;      // columns per y/z-pass tile (16 * 8 B = 128 B rows)
+constexpr int kLog2Tile = 4;
+
+struct LaunchTable {
+    int nvol;
+    int start[kMaxVolumes + 1];  // start[v] = first block of volume v; start[nvol] = total
+};
+
+// Convergence state of one TV-ADMM volume, written by the reduce kernel.
+struct VolStatus {
+    int done;      // 1 once relative_change < tol
+    int iters;     // iterations executed
+    double rel;    // last relative change
+};
+
+// Spectral buffers a kernel can address by index.
+enum SpecSel : int { SPEC_WORK = 0, SPEC_A = 1, SPEC_B = 2, SPEC_C = 3, SPEC_COUNT = 4 };
+
+struct VolDesc {
+    int nx, ny, nz, nxh;
+    int rows_x;                  // pair-rows per block in the x passes
+    int ntx;                     // y/z tiles along kx: ceil(nxh / kTile)
+    uint64_t m_nx, m_nxh, m_ny;  // magic divisors
+    qsmfft::Plan px, py, pz;
+    float hx, hy, hz;            // 1/voxel size (gradient / divergence)
+    float thr;                   // lambda / rho
+    double vsx, vsy, vsz;
+    double bx, by, bz;           // normalised B0 direction
+    double rho, tol;
+    // TV-ADMM state
+    float* x[2];                 // ping-pong solution
+    float* u[2];                 // ping-pong dual variable (3N each)
+    float2* spec[SPEC_COUNT];    // half spectra
+    float* q;                    // rho * inv_a / N (half spectrum, real)
+    double* partial;             // per-block (sum d^2, sum x^2) of the C2R pass
+    int npartial;
+    // inputs / outputs
+    const double* field;         // f64 input (device)
+    const uint8_t* mask;         // u8 mask (device)
+    double* out;                 // f64 output (device)
+    uint8_t* mask_out;           // u8 mask output (device, V-SHARP)
+    // V-SHARP state
+    float* vs_local;             // selected high-passed field (N)
+    float* vs_sfirst;            // Re FFT of the largest normalised sphere (half spectrum)
+    uint8_t* vs_final;           // final (eroded) mask (N)
+    const double* vs_radii;      // radii in processing order (device)
+    const double* vs_counts;     // voxels inside each sphere (device)
+    double vs_threshold;
+};
+
+__device__ __host__ inline int find_vol(const LaunchTable& lt, int bid) {
+    int v = 0;
+    while (v + 1 < lt.nvol && bid >= lt.start[v + 1]) ++v;
+    return v;
+}
+
+// Flat block index -> logical block, so that the blocks one XCD runs
+// (hardware dispatches consecutive block ids round-robin over 8 XCDs) are a
+// contiguous range of logical blocks sharing that XCD's L2.
+__device__ __host__ inline int xcd_remap(int bid, int total) {
+    const int q = total / 8, r = total % 8;
+    const int x = bid % 8, t = bid / 8;
+    return x * q + (x < r ? x : r) + t;
+}
+
+}  // namespace qsm

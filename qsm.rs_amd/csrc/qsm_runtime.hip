@@ -1,0 +1,890 @@
+// Host runtime of the gfx950 QSM library: planning, device workspaces, the
+// iteration loop with the reference's progress semantics, and the extern "C"
+// entry points declared in include/qsm_gpu.h.
+#include "qsm_internal.hpp"
+#include "../../include/qsm_gpu.h"
+
+#include <algorithm>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <string>
+#include <vector>
+
+namespace qsm {
+
+void launch_r2c(int emax, int src, hipStream_t st, const VolDesc* d, const LaunchTable& lt,
+                const float2* tw, int out_sel, int parity, const VolStatus* status);
+void launch_ypass(int emax, bool inv, hipStream_t st, const VolDesc* d, const LaunchTable& lt,
+                  const float2* tw, int sel, int sel2, const VolStatus* status);
+void launch_zpass(int emax, int op, hipStream_t st, const VolDesc* d, const LaunchTable& lt,
+                  const float2* tw, const VolStatus* status);
+void launch_c2r_tv(int emax, hipStream_t st, const VolDesc* d, const LaunchTable& lt, const float2* tw,
+                   int parity, const VolStatus* status);
+void launch_tv_reduce(hipStream_t st, const VolDesc* d, int nvol, VolStatus* status, VolStatus* mirror);
+void launch_tv_output(hipStream_t st, const VolDesc* d, const LaunchTable& lt, const VolStatus* status);
+void launch_fill_zero(hipStream_t st, float* p, size_t n);
+void launch_status_reset(hipStream_t st, VolStatus* s, int nvol);
+void launch_vs_zfwd(int emax, hipStream_t st, const VolDesc* d, const LaunchTable& lt, const float2* tw,
+                    int sel);
+void launch_vs_zradius(int emax, hipStream_t st, const VolDesc* d, const LaunchTable& lt, const float2* tw,
+                       int ridx, int first);
+void launch_vs_zdeconv(int emax, hipStream_t st, const VolDesc* d, const LaunchTable& lt, const float2* tw,
+                       int divide);
+void launch_vs_select(hipStream_t st, const VolDesc* d, const LaunchTable& lt, const float2* tw, int ridx,
+                      double delta);
+void launch_vs_out(hipStream_t st, const VolDesc* d, const LaunchTable& lt, const float2* tw);
+void launch_vs_init(hipStream_t st, const VolDesc* d, const LaunchTable& lt);
+
+enum { SRC_F64_ = 0, SRC_U8_ = 1, SRC_F32_ = 2, SRC_SPHERE_ = 3, SRC_TVDIV_ = 4 };
+enum { ZOP_TV_SETUP_ = 0, ZOP_TV_SOLVE_ = 1 };
+
+// ------------------------------------------------------------ error state --
+thread_local std::string g_err;
+
+int fail(int code, const char* fmt, ...) {
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof buf, fmt, ap);
+    va_end(ap);
+    g_err = buf;
+    return code;
+}
+
+#define QSM_HIP(call)                                                                  \
+    do {                                                                               \
+        hipError_t e_ = (call);                                                        \
+        if (e_ != hipSuccess)                                                          \
+            return fail(e_ == hipErrorOutOfMemory ? QSM_ENOMEM : QSM_EHIP, "%s: %s (%s:%d)", \
+                        #call, hipGetErrorString(e_), __FILE__, __LINE__);            \
+    } while (0)
+
+#define QSM_CHECK_LAUNCH()                                                             \
+    do {                                                                               \
+        hipError_t e_ = hipGetLastError();                                             \
+        if (e_ != hipSuccess)                                                          \
+            return fail(QSM_EHIP, "kernel launch: %s (%s:%d)", hipGetErrorString(e_),  \
+                        __FILE__, __LINE__);                                           \
+    } while (0)
+
+// -------------------------------------------------------------- planning --
+constexpr int kMaxLen = 512;
+
+bool length_ok(size_t n) {
+    if (n < 1 || n > (size_t)kMaxLen) return false;
+    int r[qsmfft::kMaxStages], s;
+    return qsmfft::factorize((int)n, r, &s);
+}
+
+// EMAX (elements per thread) needed for a column pass of length n.
+int emax_for_col(int n) { return n <= 256 ? 16 : (n <= 384 ? 24 : 32); }
+
+struct TwiddleBank {
+    std::vector<float2> host;
+    std::map<int, int> off;
+    int add(int n) {
+        auto it = off.find(n);
+        if (it != off.end()) return it->second;
+        int o = (int)host.size();
+        for (int q = 0; q < n; ++q) {
+            // exp(-2 pi i q/n) from an exactly reduced angle, rounded once
+            long double a = -2.0L * 3.14159265358979323846264338327950288L * q / n;
+            host.push_back(make_float2((float)cosl(a), (float)sinl(a)));
+        }
+        off[n] = o;
+        return o;
+    }
+};
+
+qsmfft::Plan make_plan(int n, TwiddleBank& tb) {
+    qsmfft::Plan p{};
+    p.n = n;
+    qsmfft::factorize(n, p.radix, &p.nstages);
+    p.tw_off = tb.add(n);
+    return p;
+}
+
+template <class T>
+int dev_alloc(T** p, size_t count) {
+    if (count == 0) count = 1;
+    hipError_t e = hipMalloc((void**)p, count * sizeof(T));
+    if (e != hipSuccess)
+        return fail(e == hipErrorOutOfMemory ? QSM_ENOMEM : QSM_EHIP, "hipMalloc(%zu bytes): %s",
+                    count * sizeof(T), hipGetErrorString(e));
+    return QSM_OK;
+}
+
+int validate_grid(const qsm_grid* g) {
+    if (!g) return fail(QSM_EPARAM, "grid is null");
+    if (g->nx == 0 || g->ny == 0 || g->nz == 0) return fail(QSM_EPARAM, "grid has a zero dimension");
+    if (!(g->vsx > 0) || !(g->vsy > 0) || !(g->vsz > 0))
+        return fail(QSM_EPARAM, "voxel sizes must be positive");
+    const size_t dims[3] = {g->nx, g->ny, g->nz};
+    for (size_t d : dims)
+        if (!length_ok(d))
+            return fail(QSM_EUNSUPPORTED,
+                        "axis length %zu not supported by the GPU FFT (needs 2^a3^b5^c7^d11^e13^f <= %d)",
+                        d, kMaxLen);
+    if (g->nx * g->ny * g->nz > (size_t)1 << 29)
+        return fail(QSM_EUNSUPPORTED, "volume of %zu voxels exceeds 2^29", g->nx * g->ny * g->nz);
+    return QSM_OK;
+}
+
+// Fill the geometry part of a descriptor and the block counts of each pass.
+struct PassBlocks {
+    int xpass, ypass, zpass, out;
+};
+
+PassBlocks fill_geometry(VolDesc& V, const qsm_grid& g, TwiddleBank& tb) {
+    V.nx = (int)g.nx; V.ny = (int)g.ny; V.nz = (int)g.nz;
+    V.nxh = V.nx / 2 + 1;
+    int rb = (qsmfft::kThreads * 16) / V.nx;
+    V.rows_x = std::max(1, std::min(32, rb));
+    V.ntx = (V.nxh + kTile - 1) / kTile;
+    V.m_nx = qsmfft::magic_for((uint32_t)V.nx);
+    V.m_nxh = qsmfft::magic_for((uint32_t)V.nxh);
+    V.m_ny = qsmfft::magic_for((uint32_t)V.ny);
+    V.px = make_plan(V.nx, tb);
+    V.py = make_plan(V.ny, tb);
+    V.pz = make_plan(V.nz, tb);
+    V.vsx = g.vsx; V.vsy = g.vsy; V.vsz = g.vsz;
+    V.hx = (float)(1.0 / g.vsx); V.hy = (float)(1.0 / g.vsy); V.hz = (float)(1.0 / g.vsz);
+    PassBlocks pb;
+    const int npairs = (V.ny * V.nz + 1) / 2;
+    pb.xpass = (npairs + V.rows_x - 1) / V.rows_x;
+    pb.ypass = V.ntx * V.nz;
+    pb.zpass = V.ntx * V.ny;
+    const size_t N = (size_t)V.nx * V.ny * V.nz;
+    pb.out = (int)((N + 1023) / 1024);
+    return pb;
+}
+
+}  // namespace qsm
+
+using namespace qsm;
+
+// ============================================================ TV batch ====
+struct qsm_tv_batch {
+    int nvol = 0;
+    qsm_tv_params params{};
+    std::vector<VolDesc> h;
+    VolDesc* d_vols = nullptr;
+    VolStatus* d_status = nullptr;
+    VolStatus* h_status = nullptr;   // pinned mirror written by the reduce kernel
+    VolStatus* h_status_dev = nullptr;
+    float2* d_tw = nullptr;
+    LaunchTable lt_x{}, lt_y{}, lt_z{}, lt_out{};
+    int emax_x = 16, emax_y = 16, emax_z = 16;
+    std::vector<void*> allocs;
+    hipStream_t stream = nullptr;
+    bool own_stream = false;
+    static constexpr int kWindow = 4;
+    hipEvent_t ev[kWindow] = {};
+};
+
+static void batch_free(qsm_tv_batch* b) {
+    if (!b) return;
+    if (b->stream) (void)hipStreamSynchronize(b->stream);
+    for (void* p : b->allocs) (void)hipFree(p);
+    if (b->h_status) (void)hipHostFree(b->h_status);
+    for (auto& e : b->ev)
+        if (e) (void)hipEventDestroy(e);
+    if (b->own_stream && b->stream) (void)hipStreamDestroy(b->stream);
+    delete b;
+}
+
+static int tv_batch_create_impl(const qsm_tv_volume* vols, size_t n_vols, const qsm_tv_params* params,
+                                void* stream, qsm_tv_batch** out) {
+    if (!vols || !params || !out) return fail(QSM_EPARAM, "null argument");
+    if (n_vols == 0 || n_vols > (size_t)kMaxVolumes)
+        return fail(QSM_EPARAM, "batch size %zu outside 1..%d", n_vols, kMaxVolumes);
+    if (!(params->rho != 0.0) || std::isnan(params->rho) || std::isnan(params->lambda))
+        return fail(QSM_EPARAM, "rho must be non-zero and lambda/rho not NaN");
+    for (size_t v = 0; v < n_vols; ++v) {
+        int rc = validate_grid(&vols[v].grid);
+        if (rc) return rc;
+        if (!vols[v].d_field || !vols[v].d_mask || !vols[v].d_chi)
+            return fail(QSM_EPARAM, "volume %zu has a null device pointer", v);
+        const double* b = vols[v].bdir;
+        double bn = std::sqrt(b[0] * b[0] + b[1] * b[1] + b[2] * b[2]);
+        if (!(bn > 0)) return fail(QSM_EPARAM, "volume %zu: B0 direction has zero length", v);
+    }
+    qsm_tv_batch* B = new qsm_tv_batch();
+    B->nvol = (int)n_vols;
+    B->params = *params;
+    B->h.resize(n_vols);
+    TwiddleBank tb;
+    B->lt_x.nvol = B->lt_y.nvol = B->lt_z.nvol = B->lt_out.nvol = (int)n_vols;
+    int cx = 0, cy = 0, cz = 0, co = 0;
+    auto cleanup = [&](int rc) { batch_free(B); return rc; };
+    for (size_t v = 0; v < n_vols; ++v) {
+        VolDesc& V = B->h[v];
+        std::memset(&V, 0, sizeof V);
+        PassBlocks pb = fill_geometry(V, vols[v].grid, tb);
+        const double* b = vols[v].bdir;
+        double bn = std::sqrt(b[0] * b[0] + b[1] * b[1] + b[2] * b[2]);
+        V.bx = b[0] / bn; V.by = b[1] / bn; V.bz = b[2] / bn;  // src/kernels/dipole.rs:39-43
+        V.rho = params->rho;
+        V.tol = params->tol;
+        V.thr = (float)(params->lambda / params->rho);          // src/inversion/tv.rs:78
+        V.field = vols[v].d_field;
+        V.mask = vols[v].d_mask;
+        V.out = vols[v].d_chi;
+        B->lt_x.start[v] = cx; cx += pb.xpass;
+        B->lt_y.start[v] = cy; cy += pb.ypass;
+        B->lt_z.start[v] = cz; cz += pb.zpass;
+        B->lt_out.start[v] = co; co += pb.out;
+        V.npartial = pb.xpass;
+        B->emax_y = std::max(B->emax_y, emax_for_col(V.ny));
+        B->emax_z = std::max(B->emax_z, emax_for_col(V.nz));
+        const size_t N = (size_t)V.nx * V.ny * V.nz;
+        const size_t Nh = (size_t)V.nxh * V.ny * V.nz;
+        int rc;
+        for (int p = 0; p < 2; ++p) {
+            if ((rc = dev_alloc(&V.x[p], N))) return cleanup(rc);
+            B->allocs.push_back(V.x[p]);
+            if ((rc = dev_alloc(&V.u[p], 3 * N))) return cleanup(rc);
+            B->allocs.push_back(V.u[p]);
+        }
+        for (int s : {SPEC_WORK, SPEC_A}) {
+            if ((rc = dev_alloc(&V.spec[s], Nh))) return cleanup(rc);
+            B->allocs.push_back(V.spec[s]);
+        }
+        if ((rc = dev_alloc(&V.q, Nh))) return cleanup(rc);
+        B->allocs.push_back(V.q);
+        if ((rc = dev_alloc(&V.partial, 2 * (size_t)pb.xpass))) return cleanup(rc);
+        B->allocs.push_back(V.partial);
+    }
+    B->lt_x.start[n_vols] = cx;
+    B->lt_y.start[n_vols] = cy;
+    B->lt_z.start[n_vols] = cz;
+    B->lt_out.start[n_vols] = co;
+    int rc;
+    if ((rc = dev_alloc(&B->d_tw, tb.host.size()))) return cleanup(rc);
+    B->allocs.push_back(B->d_tw);
+    if ((rc = dev_alloc(&B->d_vols, n_vols))) return cleanup(rc);
+    B->allocs.push_back(B->d_vols);
+    if ((rc = dev_alloc(&B->d_status, n_vols))) return cleanup(rc);
+    B->allocs.push_back(B->d_status);
+    {
+        hipError_t e = hipHostMalloc((void**)&B->h_status, n_vols * sizeof(VolStatus),
+                                     hipHostMallocMapped | hipHostMallocCoherent);
+        if (e != hipSuccess) { B->h_status = nullptr; return cleanup(fail(QSM_EHIP, "hipHostMalloc: %s", hipGetErrorString(e))); }
+        e = hipHostGetDevicePointer((void**)&B->h_status_dev, B->h_status, 0);
+        if (e != hipSuccess) return cleanup(fail(QSM_EHIP, "hipHostGetDevicePointer: %s", hipGetErrorString(e)));
+        std::memset(B->h_status, 0, n_vols * sizeof(VolStatus));
+    }
+    if (stream) {
+        B->stream = (hipStream_t)stream;
+    } else {
+        hipError_t e = hipStreamCreateWithFlags(&B->stream, hipStreamNonBlocking);
+        if (e != hipSuccess) { B->stream = nullptr; return cleanup(fail(QSM_EHIP, "hipStreamCreate: %s", hipGetErrorString(e))); }
+        B->own_stream = true;
+    }
+    for (auto& e : B->ev) {
+        hipError_t he = hipEventCreateWithFlags(&e, hipEventDisableTiming);
+        if (he != hipSuccess) { e = nullptr; return cleanup(fail(QSM_EHIP, "hipEventCreate: %s", hipGetErrorString(he))); }
+    }
+    hipError_t e1 = hipMemcpy(B->d_tw, tb.host.data(), tb.host.size() * sizeof(float2), hipMemcpyHostToDevice);
+    hipError_t e2 = hipMemcpy(B->d_vols, B->h.data(), n_vols * sizeof(VolDesc), hipMemcpyHostToDevice);
+    if (e1 != hipSuccess || e2 != hipSuccess)
+        return cleanup(fail(QSM_EHIP, "hipMemcpy of plan tables: %s",
+                            hipGetErrorString(e1 != hipSuccess ? e1 : e2)));
+    *out = B;
+    return QSM_OK;
+}
+
+// One TV-ADMM iteration (src/inversion/admm.rs:76-145) for every volume.
+static int enqueue_iteration(qsm_tv_batch* B, size_t it) {
+    const int parity = (int)((it - 1) & 1);
+    hipStream_t st = B->stream;
+    launch_r2c(16, SRC_TVDIV_, st, B->d_vols, B->lt_x, B->d_tw, SPEC_WORK, parity, B->d_status);
+    launch_ypass(B->emax_y, false, st, B->d_vols, B->lt_y, B->d_tw, SPEC_WORK, -1, B->d_status);
+    launch_zpass(B->emax_z, ZOP_TV_SOLVE_, st, B->d_vols, B->lt_z, B->d_tw, B->d_status);
+    launch_ypass(B->emax_y, true, st, B->d_vols, B->lt_y, B->d_tw, SPEC_WORK, -1, B->d_status);
+    launch_c2r_tv(16, st, B->d_vols, B->lt_x, B->d_tw, parity, B->d_status);
+    launch_tv_reduce(st, B->d_vols, B->nvol, B->d_status, B->h_status_dev);
+    QSM_CHECK_LAUNCH();
+    return QSM_OK;
+}
+
+static int tv_batch_run_impl(qsm_tv_batch* B, qsm_progress_fn progress, void* user) {
+    if (!B) return fail(QSM_EPARAM, "batch is null");
+    hipStream_t st = B->stream;
+    // Setup: x_0 = 0, u_{-1} = 0 (AdmmBuffers::new, src/inversion/admm.rs:39-52),
+    // then f^ and inv_a (prepare_admm_spectral, src/inversion/admm.rs:150-178).
+    for (auto& V : B->h) {
+        const size_t N = (size_t)V.nx * V.ny * V.nz;
+        launch_fill_zero(st, V.x[0], N);
+        launch_fill_zero(st, V.u[1], 3 * N);
+    }
+    launch_status_reset(st, B->d_status, B->nvol);
+    std::memset(B->h_status, 0, B->nvol * sizeof(VolStatus));
+    launch_r2c(16, SRC_F64_, st, B->d_vols, B->lt_x, B->d_tw, SPEC_WORK, 0, nullptr);
+    launch_ypass(B->emax_y, false, st, B->d_vols, B->lt_y, B->d_tw, SPEC_WORK, -1, nullptr);
+    launch_zpass(B->emax_z, ZOP_TV_SETUP_, st, B->d_vols, B->lt_z, B->d_tw, nullptr);
+    QSM_CHECK_LAUNCH();
+
+    const size_t max_iter = B->params.max_iter;
+    // relative_change >= 0 (or NaN), so with tol <= 0 no iteration can stop
+    // early and the host never needs to look at the device state.
+    const bool can_converge = B->params.tol > 0.0;
+    if (!progress && !can_converge) {
+        for (size_t it = 1; it <= max_iter; ++it) {
+            int rc = enqueue_iteration(B, it);
+            if (rc) return rc;
+        }
+    } else {
+        // Keep up to kWindow iterations in flight; report progress for
+        // iteration it+1 only once iteration it is known not to have
+        // converged, so the callback sequence equals the reference's.
+        size_t next = 1;
+        if (max_iter > 0 && progress) progress(1, max_iter, user);
+        for (size_t it = 1; it <= max_iter; ++it) {
+            while (next <= max_iter && next < it + qsm_tv_batch::kWindow) {
+                int rc = enqueue_iteration(B, next);
+                if (rc) return rc;
+                QSM_HIP(hipEventRecord(B->ev[next % qsm_tv_batch::kWindow], st));
+                ++next;
+            }
+            QSM_HIP(hipEventSynchronize(B->ev[it % qsm_tv_batch::kWindow]));
+            bool all_done = true;
+            for (int v = 0; v < B->nvol; ++v) {
+                const volatile VolStatus* hs = B->h_status + v;
+                const int done = hs->done, iters = hs->iters;
+                if (!(done && (size_t)iters <= it)) all_done = false;
+            }
+            if (all_done) {
+                if (progress) progress(it, it, user);
+                break;
+            }
+            if (it < max_iter && progress) progress(it + 1, max_iter, user);
+        }
+    }
+    launch_tv_output(st, B->d_vols, B->lt_out, B->d_status);
+    QSM_CHECK_LAUNCH();
+    QSM_HIP(hipStreamSynchronize(st));
+    return QSM_OK;
+}
+
+
+
+// Event-timed copy of the iteration loop: every kernel of `iters` iterations
+// is bracketed by HIP events on the batch's stream; ms_out[k] receives the
+// mean duration of kernel slot k (0 r2c+div, 1 y fwd, 2 z solve, 3 y inv,
+// 4 c2r, 5 reduce).  Runs the spectral setup first so the data are realistic.
+static int tv_batch_profile_impl(qsm_tv_batch* B, size_t iters, double* ms_out, size_t n_out) {
+    if (!B || !ms_out || n_out < 6 || iters == 0) return fail(QSM_EPARAM, "bad profile arguments");
+    hipStream_t st = B->stream;
+    for (auto& V : B->h) {
+        const size_t N = (size_t)V.nx * V.ny * V.nz;
+        launch_fill_zero(st, V.x[0], N);
+        launch_fill_zero(st, V.u[1], 3 * N);
+    }
+    launch_status_reset(st, B->d_status, B->nvol);
+    launch_r2c(16, SRC_F64_, st, B->d_vols, B->lt_x, B->d_tw, SPEC_WORK, 0, nullptr);
+    launch_ypass(B->emax_y, false, st, B->d_vols, B->lt_y, B->d_tw, SPEC_WORK, -1, nullptr);
+    launch_zpass(B->emax_z, ZOP_TV_SETUP_, st, B->d_vols, B->lt_z, B->d_tw, nullptr);
+    QSM_CHECK_LAUNCH();
+    hipEvent_t ev[7];
+    for (auto& e : ev) QSM_HIP(hipEventCreate(&e));
+    double acc[6] = {0, 0, 0, 0, 0, 0};
+    for (size_t it = 1; it <= iters; ++it) {
+        const int parity = (int)((it - 1) & 1);
+        QSM_HIP(hipEventRecord(ev[0], st));
+        launch_r2c(16, SRC_TVDIV_, st, B->d_vols, B->lt_x, B->d_tw, SPEC_WORK, parity, nullptr);
+        QSM_HIP(hipEventRecord(ev[1], st));
+        launch_ypass(B->emax_y, false, st, B->d_vols, B->lt_y, B->d_tw, SPEC_WORK, -1, nullptr);
+        QSM_HIP(hipEventRecord(ev[2], st));
+        launch_zpass(B->emax_z, ZOP_TV_SOLVE_, st, B->d_vols, B->lt_z, B->d_tw, nullptr);
+        QSM_HIP(hipEventRecord(ev[3], st));
+        launch_ypass(B->emax_y, true, st, B->d_vols, B->lt_y, B->d_tw, SPEC_WORK, -1, nullptr);
+        QSM_HIP(hipEventRecord(ev[4], st));
+        launch_c2r_tv(16, st, B->d_vols, B->lt_x, B->d_tw, parity, nullptr);
+        QSM_HIP(hipEventRecord(ev[5], st));
+        launch_tv_reduce(st, B->d_vols, B->nvol, B->d_status, nullptr);
+        QSM_HIP(hipEventRecord(ev[6], st));
+        QSM_CHECK_LAUNCH();
+        QSM_HIP(hipEventSynchronize(ev[6]));
+        for (int k = 0; k < 6; ++k) {
+            float ms = 0.f;
+            QSM_HIP(hipEventElapsedTime(&ms, ev[k], ev[k + 1]));
+            acc[k] += ms;
+        }
+    }
+    for (auto& e : ev) (void)hipEventDestroy(e);
+    for (int k = 0; k < 6; ++k) ms_out[k] = acc[k] / (double)iters;
+    return QSM_OK;
+}
+
+// ========================================================= V-SHARP batch ==
+// Voxels of the wrapped sphere of src/kernels/smv.rs:19-67, counted over the
+// sphere's bounding box only (same coordinates and the same f64 operation
+// order as the kernel, so the count equals the reference's `count`).
+static double sphere_count(const qsm_grid& g, double radius) {
+    const double r2 = radius * radius;
+    auto coords = [&](size_t n, double vs, std::vector<double>& out) {
+        const double lim = std::fabs(radius) / vs + 1.0;
+        for (size_t i = 0; i < n; ++i) {
+            double c = (i <= n / 2) ? (double)i : (double)((long long)i - (long long)n);
+            if (std::fabs(c) <= lim) out.push_back(c * vs);
+        }
+    };
+    std::vector<double> xs, ys, zs;
+    coords(g.nx, g.vsx, xs);
+    coords(g.ny, g.vsy, ys);
+    coords(g.nz, g.vsz, zs);
+    double count = 0.0;
+    for (double dz : zs)
+        for (double dy : ys)
+            for (double dx : xs) {
+                volatile double d2 = dx * dx;
+                d2 = d2 + dy * dy;
+                d2 = d2 + dz * dz;
+                if (d2 <= r2) count += 1.0;
+            }
+    return count;
+}
+
+struct qsm_vsharp_batch {
+    int nvol = 0;
+    std::vector<VolDesc> h;
+    VolDesc* d_vols = nullptr;
+    float2* d_tw = nullptr;
+    double* d_radii = nullptr;
+    std::vector<double> radii;  // processing order (descending)
+    double threshold = 0.05;
+    bool sharp = false;         // one radius: sharp_core semantics
+    LaunchTable lt_x{}, lt_rows{}, lt_y{}, lt_y2{}, lt_z{}, lt_init{};
+    int emax_y = 16, emax_z = 16;
+    std::vector<void*> allocs;
+    hipStream_t stream = nullptr;
+    bool own_stream = false;
+};
+
+static void vs_free(qsm_vsharp_batch* b) {
+    if (!b) return;
+    if (b->stream) (void)hipStreamSynchronize(b->stream);
+    for (void* p : b->allocs) (void)hipFree(p);
+    if (b->own_stream && b->stream) (void)hipStreamDestroy(b->stream);
+    delete b;
+}
+
+static int vs_create_impl(const qsm_vsharp_volume* vols, size_t n_vols, const double* radii,
+                          size_t n_radii, double threshold, void* stream, qsm_vsharp_batch** out) {
+    if (!vols || !out || (n_radii && !radii)) return fail(QSM_EPARAM, "null argument");
+    if (n_vols == 0 || n_vols > (size_t)kMaxVolumes)
+        return fail(QSM_EPARAM, "batch size %zu outside 1..%d", n_vols, kMaxVolumes);
+    if (n_radii == 0) return fail(QSM_EPARAM, "at least one radius is required for a batch");
+    for (size_t r = 0; r < n_radii; ++r)
+        if (std::isnan(radii[r])) return fail(QSM_EPARAM, "radius %zu is NaN", r);
+    for (size_t v = 0; v < n_vols; ++v) {
+        int rc = validate_grid(&vols[v].grid);
+        if (rc) return rc;
+        if (!vols[v].d_field || !vols[v].d_mask || !vols[v].d_local || !vols[v].d_mask_out)
+            return fail(QSM_EPARAM, "volume %zu has a null device pointer", v);
+    }
+    qsm_vsharp_batch* B = new qsm_vsharp_batch();
+    auto cleanup = [&](int rc) { vs_free(B); return rc; };
+    B->nvol = (int)n_vols;
+    B->threshold = threshold;
+    // src/bgremove/vsharp.rs:114-116: sort descending (stable)
+    B->radii.assign(radii, radii + n_radii);
+    std::stable_sort(B->radii.begin(), B->radii.end(), [](double a, double b) { return b < a; });
+    B->sharp = (n_radii == 1);
+    const int R = (int)n_radii;
+    B->h.resize(n_vols);
+    TwiddleBank tb;
+    for (LaunchTable* lt : {&B->lt_x, &B->lt_rows, &B->lt_y, &B->lt_y2, &B->lt_z, &B->lt_init})
+        lt->nvol = (int)n_vols;
+    int cx = 0, cr = 0, cy = 0, cy2 = 0, cz = 0, ci = 0;
+    int rc;
+    if ((rc = dev_alloc(&B->d_radii, (size_t)R * (1 + n_vols)))) return cleanup(rc);
+    B->allocs.push_back(B->d_radii);
+    std::vector<double> host_tab((size_t)R * (1 + n_vols));
+    for (int r = 0; r < R; ++r) host_tab[r] = B->radii[r];
+    for (size_t v = 0; v < n_vols; ++v) {
+        VolDesc& V = B->h[v];
+        std::memset(&V, 0, sizeof V);
+        PassBlocks pb = fill_geometry(V, vols[v].grid, tb);
+        V.field = vols[v].d_field;
+        V.mask = vols[v].d_mask;
+        V.out = vols[v].d_local;
+        V.mask_out = vols[v].d_mask_out;
+        V.vs_threshold = threshold;
+        V.vs_radii = B->d_radii;
+        V.vs_counts = B->d_radii + (size_t)R * (1 + v);
+        for (int r = 0; r < R; ++r) host_tab[(size_t)R * (1 + v) + r] = sphere_count(vols[v].grid, B->radii[r]);
+        const int nrows = V.ny * V.nz;
+        B->lt_x.start[v] = cx; cx += pb.xpass;
+        B->lt_rows.start[v] = cr; cr += (nrows + V.rows_x - 1) / V.rows_x;
+        B->lt_y.start[v] = cy; cy += pb.ypass;
+        B->lt_y2.start[v] = cy2; cy2 += 2 * pb.ypass;
+        B->lt_z.start[v] = cz; cz += pb.zpass;
+        B->lt_init.start[v] = ci; ci += pb.out;
+        B->emax_y = std::max(B->emax_y, emax_for_col(V.ny));
+        B->emax_z = std::max(B->emax_z, emax_for_col(V.nz));
+        const size_t N = (size_t)V.nx * V.ny * V.nz;
+        const size_t Nh = (size_t)V.nxh * V.ny * V.nz;
+        for (int sidx = 0; sidx < SPEC_COUNT; ++sidx) {
+            if ((rc = dev_alloc(&V.spec[sidx], Nh))) return cleanup(rc);
+            B->allocs.push_back(V.spec[sidx]);
+        }
+        if ((rc = dev_alloc(&V.vs_sfirst, Nh))) return cleanup(rc);
+        B->allocs.push_back(V.vs_sfirst);
+        if ((rc = dev_alloc(&V.vs_local, N))) return cleanup(rc);
+        B->allocs.push_back(V.vs_local);
+        if ((rc = dev_alloc(&V.vs_final, N))) return cleanup(rc);
+        B->allocs.push_back(V.vs_final);
+    }
+    B->lt_x.start[n_vols] = cx;
+    B->lt_rows.start[n_vols] = cr;
+    B->lt_y.start[n_vols] = cy;
+    B->lt_y2.start[n_vols] = cy2;
+    B->lt_z.start[n_vols] = cz;
+    B->lt_init.start[n_vols] = ci;
+    if ((rc = dev_alloc(&B->d_tw, tb.host.size()))) return cleanup(rc);
+    B->allocs.push_back(B->d_tw);
+    if ((rc = dev_alloc(&B->d_vols, n_vols))) return cleanup(rc);
+    B->allocs.push_back(B->d_vols);
+    if (stream) {
+        B->stream = (hipStream_t)stream;
+    } else {
+        hipError_t e = hipStreamCreateWithFlags(&B->stream, hipStreamNonBlocking);
+        if (e != hipSuccess) { B->stream = nullptr; return cleanup(fail(QSM_EHIP, "hipStreamCreate: %s", hipGetErrorString(e))); }
+        B->own_stream = true;
+    }
+    hipError_t e1 = hipMemcpy(B->d_tw, tb.host.data(), tb.host.size() * sizeof(float2), hipMemcpyHostToDevice);
+    hipError_t e2 = hipMemcpy(B->d_vols, B->h.data(), n_vols * sizeof(VolDesc), hipMemcpyHostToDevice);
+    hipError_t e3 = hipMemcpy(B->d_radii, host_tab.data(), host_tab.size() * sizeof(double), hipMemcpyHostToDevice);
+    if (e1 != hipSuccess || e2 != hipSuccess || e3 != hipSuccess)
+        return cleanup(fail(QSM_EHIP, "hipMemcpy of plan tables failed"));
+    *out = B;
+    return QSM_OK;
+}
+
+static int vs_run_impl(qsm_vsharp_batch* B, qsm_progress_fn progress, void* user) {
+    if (!B) return fail(QSM_EPARAM, "batch is null");
+    hipStream_t st = B->stream;
+    const int R = (int)B->radii.size();
+    const double delta = 1.0 - std::sqrt(1e-7);  // src/bgremove/vsharp.rs:129
+    // F^ = FFT(field) (vsharp.rs:117-122), M^ = FFT(mask) once for all radii
+    launch_r2c(16, SRC_F64_, st, B->d_vols, B->lt_x, B->d_tw, SPEC_A, 0, nullptr);
+    launch_ypass(B->emax_y, false, st, B->d_vols, B->lt_y, B->d_tw, SPEC_A, -1, nullptr);
+    launch_vs_zfwd(B->emax_z, st, B->d_vols, B->lt_z, B->d_tw, SPEC_A);
+    launch_r2c(16, SRC_U8_, st, B->d_vols, B->lt_x, B->d_tw, SPEC_B, 0, nullptr);
+    launch_ypass(B->emax_y, false, st, B->d_vols, B->lt_y, B->d_tw, SPEC_B, -1, nullptr);
+    launch_vs_zfwd(B->emax_z, st, B->d_vols, B->lt_z, B->d_tw, SPEC_B);
+    launch_vs_init(st, B->d_vols, B->lt_init);
+    QSM_CHECK_LAUNCH();
+    for (int r = 0; r < R; ++r) {
+        if (progress) {
+            // report radius r+1 once radius r has finished on the device
+            if (r > 0) QSM_HIP(hipStreamSynchronize(st));
+            if (B->sharp) progress(1, 1, user);           // vsharp.rs:103-108
+            else progress((size_t)r + 1, (size_t)R, user); // vsharp.rs:134
+        }
+        launch_r2c(16, SRC_SPHERE_, st, B->d_vols, B->lt_x, B->d_tw, SPEC_WORK, r, nullptr);
+        launch_ypass(B->emax_y, false, st, B->d_vols, B->lt_y, B->d_tw, SPEC_WORK, -1, nullptr);
+        launch_vs_zradius(B->emax_z, st, B->d_vols, B->lt_z, B->d_tw, r, r == 0 ? 1 : 0);
+        launch_ypass(B->emax_y, true, st, B->d_vols, B->lt_y2, B->d_tw, SPEC_WORK, SPEC_C, nullptr);
+        launch_vs_select(st, B->d_vols, B->lt_rows, B->d_tw, r, delta);
+        QSM_CHECK_LAUNCH();
+    }
+    // deconvolution with the largest kernel (vsharp.rs:180-196 / sharp.rs:105-131)
+    launch_r2c(16, SRC_F32_, st, B->d_vols, B->lt_x, B->d_tw, SPEC_WORK, 0, nullptr);
+    launch_ypass(B->emax_y, false, st, B->d_vols, B->lt_y, B->d_tw, SPEC_WORK, -1, nullptr);
+    launch_vs_zdeconv(B->emax_z, st, B->d_vols, B->lt_z, B->d_tw, B->sharp ? 1 : 0);
+    launch_ypass(B->emax_y, true, st, B->d_vols, B->lt_y, B->d_tw, SPEC_WORK, -1, nullptr);
+    launch_vs_out(st, B->d_vols, B->lt_x, B->d_tw);
+    QSM_CHECK_LAUNCH();
+    QSM_HIP(hipStreamSynchronize(st));
+    return QSM_OK;
+}
+
+// Host-pointer V-SHARP with explicit radii (src/bgremove/vsharp.rs:88-199).
+static int vsharp_host(const double* field, const uint8_t* mask, const qsm_grid* grid,
+                       const double* radii, size_t n_radii, double threshold,
+                       qsm_progress_fn progress, void* user, double* local_out, uint8_t* mask_out) {
+    if (!field || !mask || !grid || !local_out || !mask_out || (n_radii && !radii))
+        return fail(QSM_EPARAM, "null argument");
+    int rc = validate_grid(grid);
+    if (rc) return rc;
+    const size_t N = grid->nx * grid->ny * grid->nz;
+    if (n_radii == 0) {  // vsharp.rs:98-100
+        std::fill(local_out, local_out + N, 0.0);
+        std::memcpy(mask_out, mask, N);
+        return QSM_OK;
+    }
+    double *d_f = nullptr, *d_l = nullptr;
+    uint8_t *d_m = nullptr, *d_mo = nullptr;
+    qsm_vsharp_batch* B = nullptr;
+    auto done = [&](int code) {
+        if (B) vs_free(B);
+        if (d_f) (void)hipFree(d_f);
+        if (d_l) (void)hipFree(d_l);
+        if (d_m) (void)hipFree(d_m);
+        if (d_mo) (void)hipFree(d_mo);
+        return code;
+    };
+    if ((rc = dev_alloc(&d_f, N)) || (rc = dev_alloc(&d_l, N)) || (rc = dev_alloc(&d_m, N)) ||
+        (rc = dev_alloc(&d_mo, N)))
+        return done(rc);
+    hipError_t e = hipMemcpy(d_f, field, N * sizeof(double), hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(d_m, mask, N, hipMemcpyHostToDevice);
+    if (e != hipSuccess) return done(fail(QSM_EHIP, "hipMemcpy H2D: %s", hipGetErrorString(e)));
+    qsm_vsharp_volume vol;
+    vol.grid = *grid;
+    vol.d_field = d_f; vol.d_mask = d_m; vol.d_local = d_l; vol.d_mask_out = d_mo;
+    if ((rc = vs_create_impl(&vol, 1, radii, n_radii, threshold, nullptr, &B))) return done(rc);
+    if ((rc = vs_run_impl(B, progress, user))) return done(rc);
+    e = hipMemcpy(local_out, d_l, N * sizeof(double), hipMemcpyDeviceToHost);
+    if (e == hipSuccess) e = hipMemcpy(mask_out, d_mo, N, hipMemcpyDeviceToHost);
+    if (e != hipSuccess) return done(fail(QSM_EHIP, "hipMemcpy D2H: %s", hipGetErrorString(e)));
+    return done(QSM_OK);
+}
+
+// src/bgremove/vsharp.rs:70-82.  A non-positive step would never terminate
+// in the reference; it is rejected instead.
+static int vsharp_radii(const qsm_vsharp_params* p, std::vector<double>& radii) {
+    if (!(p->min_radius > 0.0))
+        return fail(QSM_EPARAM, "min_radius must be > 0 (it is the step between radii)");
+    if (std::isnan(p->max_radius)) return fail(QSM_EPARAM, "max_radius is NaN");
+    double r = p->max_radius;
+    const double step = p->min_radius;
+    while (r >= step) {
+        radii.push_back(r);
+        r -= step;
+        if (radii.size() > 100000) return fail(QSM_EPARAM, "too many radii");
+    }
+    if (radii.empty()) radii.push_back(p->max_radius);
+    return QSM_OK;
+}
+
+// ====================================================== extern "C" API ====
+extern "C" {
+
+void qsm_tv_params_default(qsm_tv_params* p) {
+    if (!p) return;
+    p->lambda = 2e-4;  // src/inversion/tv.rs:34-43
+    p->rho = 2e-2;
+    p->tol = 1e-3;
+    p->max_iter = 250;
+}
+
+void qsm_vsharp_params_default(qsm_vsharp_params* p) {
+    if (!p) return;
+    p->threshold = 0.05;  // src/bgremove/vsharp.rs:31-39
+    p->max_radius = 12.0;
+    p->min_radius = 1.0;
+}
+
+const char* qsm_last_error_message(void) { return g_err.c_str(); }
+
+int qsm_fft_length_supported(size_t n) { return length_ok(n) ? 1 : 0; }
+
+int qsm_backend_info(char* buf, size_t len) {
+    if (!buf || len == 0) return fail(QSM_EPARAM, "null buffer");
+    int dev = 0;
+    hipDeviceProp_t prop;
+    hipError_t e = hipGetDevice(&dev);
+    if (e == hipSuccess) e = hipGetDeviceProperties(&prop, dev);
+    if (e != hipSuccess) {
+        snprintf(buf, len, "qsm_gpu (gfx950 build); no HIP device: %s", hipGetErrorString(e));
+        return QSM_OK;
+    }
+    snprintf(buf, len, "qsm_gpu gfx950 build on %s (%s), %d CUs, %.1f GiB HBM", prop.name,
+             prop.gcnArchName, prop.multiProcessorCount, prop.totalGlobalMem / 1073741824.0);
+    return QSM_OK;
+}
+
+int qsm_tv_batch_create(const qsm_tv_volume* vols, size_t n_vols, const qsm_tv_params* params,
+                        void* stream, qsm_tv_batch** out) {
+    g_err.clear();
+    return tv_batch_create_impl(vols, n_vols, params, stream, out);
+}
+
+int qsm_tv_batch_run(qsm_tv_batch* batch, qsm_progress_fn progress, void* user) {
+    g_err.clear();
+    return tv_batch_run_impl(batch, progress, user);
+}
+
+int qsm_tv_batch_status(const qsm_tv_batch* B, size_t v, size_t* iters, double* rel) {
+    if (!B || v >= (size_t)B->nvol) return fail(QSM_EPARAM, "bad batch or volume index");
+    VolStatus s;
+    hipError_t e = hipMemcpy(&s, B->d_status + v, sizeof s, hipMemcpyDeviceToHost);
+    if (e != hipSuccess) return fail(QSM_EHIP, "hipMemcpy status: %s", hipGetErrorString(e));
+    if (iters) *iters = (size_t)s.iters;
+    if (rel) *rel = s.rel;
+    return QSM_OK;
+}
+
+void qsm_tv_batch_destroy(qsm_tv_batch* batch) { batch_free(batch); }
+
+int qsm_tv_batch_profile(qsm_tv_batch* batch, size_t iters, double* ms_out, size_t n_out) {
+    g_err.clear();
+    return tv_batch_profile_impl(batch, iters, ms_out, n_out);
+}
+
+int qsm_tv_admm_ex(const double* local_field, const uint8_t* mask, const qsm_grid* grid,
+                   const double bdir[3], const qsm_tv_params* params, qsm_progress_fn progress,
+                   void* user, double* chi_out, size_t* iters_out, double* rel_out) {
+    g_err.clear();
+    if (!local_field || !mask || !grid || !bdir || !params || !chi_out)
+        return fail(QSM_EPARAM, "null argument");
+    int rc = validate_grid(grid);
+    if (rc) return rc;
+    const size_t N = grid->nx * grid->ny * grid->nz;
+    double *d_f = nullptr, *d_chi = nullptr;
+    uint8_t* d_m = nullptr;
+    qsm_tv_batch* B = nullptr;
+    auto done = [&](int code) {
+        if (B) batch_free(B);
+        if (d_f) (void)hipFree(d_f);
+        if (d_chi) (void)hipFree(d_chi);
+        if (d_m) (void)hipFree(d_m);
+        return code;
+    };
+    if ((rc = dev_alloc(&d_f, N)) || (rc = dev_alloc(&d_chi, N)) || (rc = dev_alloc(&d_m, N)))
+        return done(rc);
+    hipError_t e = hipMemcpy(d_f, local_field, N * sizeof(double), hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(d_m, mask, N, hipMemcpyHostToDevice);
+    if (e != hipSuccess) return done(fail(QSM_EHIP, "hipMemcpy H2D: %s", hipGetErrorString(e)));
+    qsm_tv_volume vol;
+    vol.grid = *grid;
+    vol.bdir[0] = bdir[0]; vol.bdir[1] = bdir[1]; vol.bdir[2] = bdir[2];
+    vol.d_field = d_f; vol.d_mask = d_m; vol.d_chi = d_chi;
+    if ((rc = tv_batch_create_impl(&vol, 1, params, nullptr, &B))) return done(rc);
+    if ((rc = tv_batch_run_impl(B, progress, user))) return done(rc);
+    e = hipMemcpy(chi_out, d_chi, N * sizeof(double), hipMemcpyDeviceToHost);
+    if (e != hipSuccess) return done(fail(QSM_EHIP, "hipMemcpy D2H: %s", hipGetErrorString(e)));
+    if (iters_out || rel_out) {
+        size_t it = 0;
+        double rel = 0;
+        if ((rc = qsm_tv_batch_status(B, 0, &it, &rel))) return done(rc);
+        if (iters_out) *iters_out = it;
+        if (rel_out) *rel_out = rel;
+    }
+    return done(QSM_OK);
+}
+
+int qsm_tv_admm(const double* local_field, const uint8_t* mask, const qsm_grid* grid,
+                const double bdir[3], const qsm_tv_params* params, qsm_progress_fn progress,
+                void* user, double* chi_out) {
+    return qsm_tv_admm_ex(local_field, mask, grid, bdir, params, progress, user, chi_out, nullptr,
+                          nullptr);
+}
+
+int qsm_run_dipole_inversion(const double* local_field_ppm, size_t field_len, const uint8_t* mask,
+                             size_t mask_len, const qsm_scan_metadata* md,
+                             qsm_inversion_algorithm algorithm, const qsm_tv_params* tv_params,
+                             qsm_progress_fn progress, void* user, double* chi_out) {
+    g_err.clear();
+    if (!md) return fail(QSM_EPARAM, "metadata is null");
+    const size_t n = md->nx * md->ny * md->nz;
+    // src/pipeline/inversion.rs:36-45
+    if (field_len != n) return fail(QSM_EDIM, "dimension mismatch: expected %zu, got %zu", n, field_len);
+    if (mask_len != n) return fail(QSM_EDIM, "dimension mismatch: expected %zu, got %zu", n, mask_len);
+    if (algorithm == QSM_INV_TGV || algorithm == QSM_INV_QSMART)
+        return fail(QSM_ECONFIG, "invalid config: %s should use run_tgv or run_qsmart",
+                    algorithm == QSM_INV_TGV ? "Tgv" : "Qsmart");
+    if (algorithm != QSM_INV_TV)
+        return fail(QSM_ECONFIG, "invalid config: algorithm %d has no GPU backend here (Tv only)",
+                    (int)algorithm);
+    qsm_grid g{md->nx, md->ny, md->nz, md->vsx, md->vsy, md->vsz};
+    qsm_tv_params p;
+    if (tv_params) p = *tv_params; else qsm_tv_params_default(&p);
+    return qsm_tv_admm(local_field_ppm, mask, &g, md->b0_direction, &p, progress, user, chi_out);
+}
+
+int qsm_vsharp_batch_create(const qsm_vsharp_volume* vols, size_t n_vols, const double* radii,
+                            size_t n_radii, double threshold, void* stream, qsm_vsharp_batch** out) {
+    g_err.clear();
+    return vs_create_impl(vols, n_vols, radii, n_radii, threshold, stream, out);
+}
+
+int qsm_vsharp_batch_run(qsm_vsharp_batch* batch, qsm_progress_fn progress, void* user) {
+    g_err.clear();
+    return vs_run_impl(batch, progress, user);
+}
+
+void qsm_vsharp_batch_destroy(qsm_vsharp_batch* batch) { vs_free(batch); }
+
+int qsm_vsharp_with_radii(const double* field, const uint8_t* mask, const qsm_grid* grid,
+                          const double* radii, size_t n_radii, double threshold,
+                          qsm_progress_fn progress, void* user, double* local_out,
+                          uint8_t* mask_out) {
+    g_err.clear();
+    return vsharp_host(field, mask, grid, radii, n_radii, threshold, progress, user, local_out,
+                       mask_out);
+}
+
+int qsm_vsharp(const double* field, const uint8_t* mask, const qsm_grid* grid,
+               const qsm_vsharp_params* params, qsm_progress_fn progress, void* user,
+               double* local_out, uint8_t* mask_out) {
+    g_err.clear();
+    if (!params) return fail(QSM_EPARAM, "params is null");
+    std::vector<double> radii;
+    int rc = vsharp_radii(params, radii);
+    if (rc) return rc;
+    return vsharp_host(field, mask, grid, radii.data(), radii.size(), params->threshold, progress,
+                       user, local_out, mask_out);
+}
+
+int qsm_run_bg_removal(const double* field_ppm, size_t field_len, const uint8_t* mask,
+                       size_t mask_len, const qsm_scan_metadata* md,
+                       qsm_bg_removal_algorithm algorithm, const qsm_vsharp_params* vsharp_params,
+                       qsm_progress_fn progress, void* user, double* local_out, uint8_t* mask_out) {
+    g_err.clear();
+    if (!md) return fail(QSM_EPARAM, "metadata is null");
+    const size_t n = md->nx * md->ny * md->nz;
+    // src/pipeline/bg_removal.rs:30-35
+    if (field_len != n) return fail(QSM_EDIM, "dimension mismatch: expected %zu, got %zu", n, field_len);
+    if (mask_len != n) return fail(QSM_EDIM, "dimension mismatch: expected %zu, got %zu", n, mask_len);
+    if (algorithm != QSM_BG_VSHARP)
+        return fail(QSM_ECONFIG, "invalid config: algorithm %d has no GPU backend here (Vsharp only)",
+                    (int)algorithm);
+    qsm_grid g{md->nx, md->ny, md->nz, md->vsx, md->vsy, md->vsz};
+    qsm_vsharp_params p;
+    if (vsharp_params) p = *vsharp_params; else qsm_vsharp_params_default(&p);
+    return qsm_vsharp(field_ppm, mask, &g, &p, progress, user, local_out, mask_out);
+}
+
+int qsm_device_alloc(size_t bytes, void** out) {
+    g_err.clear();
+    if (!out) return fail(QSM_EPARAM, "null output pointer");
+    *out = nullptr;
+    QSM_HIP(hipMalloc(out, bytes ? bytes : 1));
+    return QSM_OK;
+}
+
+int qsm_device_free(void* ptr) {
+    g_err.clear();
+    if (ptr) QSM_HIP(hipFree(ptr));
+    return QSM_OK;
+}
+
+int qsm_memcpy_htod(void* dst, const void* src, size_t bytes) {
+    g_err.clear();
+    if ((!dst || !src) && bytes) return fail(QSM_EPARAM, "null pointer");
+    if (bytes) QSM_HIP(hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice));
+    return QSM_OK;
+}
+
+int qsm_memcpy_dtoh(void* dst, const void* src, size_t bytes) {
+    g_err.clear();
+    if ((!dst || !src) && bytes) return fail(QSM_EPARAM, "null pointer");
+    if (bytes) QSM_HIP(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost));
+    return QSM_OK;
+}
+
+int qsm_device_synchronize(void) {
+    g_err.clear();
+    QSM_HIP(hipDeviceSynchronize());
+    return QSM_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,116 @@
+// CPU emulation of the LDS Stockham FFT in fft_core.hpp.
+//
+// Runs the same stage_load / stage_store code the GPU kernels use, one
+// simulated thread at a time with a "barrier" between the phases, for every
+// length the workloads need, both layouts (row and column tiles) and both
+// directions, and compares against a direct O(n^2) DFT in double precision.
+// Built with hipcc as host code; prints one line per case and exits non-zero
+// on any mismatch.
+#include "../../qsm.rs_amd/csrc/fft_core.hpp"
+
+#include <cmath>
+#include <complex>
+#include <cstdio>
+#include <cstdlib>
+#include <vector>
+
+using namespace qsmfft;
+
+template <int R, int EMAX, bool COLFAST, int LOG2C, bool INV>
+static void emu_stage(std::vector<float2>& lds, const StageGeom& g, const float2* tw) {
+    constexpr int B = Bmax<R, EMAX>::value;
+    std::vector<std::array<std::array<float2, R>, B>> regs(kThreads);
+    for (int tid = 0; tid < kThreads; ++tid) {
+        float2 v[B][R];
+        stage_load<R, B, COLFAST, LOG2C>(lds.data(), tid, g, v);
+        for (int t = 0; t < B; ++t)
+            for (int r = 0; r < R; ++r) regs[tid][t][r] = v[t][r];
+    }
+    for (int tid = 0; tid < kThreads; ++tid) {
+        float2 v[B][R];
+        for (int t = 0; t < B; ++t)
+            for (int r = 0; r < R; ++r) v[t][r] = regs[tid][t][r];
+        stage_store<R, B, COLFAST, LOG2C, INV>(lds.data(), tid, g, v, tw);
+    }
+}
+
+template <int EMAX, bool COLFAST, int LOG2C, bool INV>
+static void emu_fft(std::vector<float2>& lds, const Plan& P, int ncols, int es, int cs,
+                    const float2* tw) {
+    int Ns = 1;
+    for (int s = 0; s < P.nstages; ++s) {
+        int R = P.radix[s];
+        StageGeom g = make_geom(P.n, ncols, es, cs, R, Ns);
+        switch (R) {
+            case 2: emu_stage<2, EMAX, COLFAST, LOG2C, INV>(lds, g, tw); break;
+            case 3: emu_stage<3, EMAX, COLFAST, LOG2C, INV>(lds, g, tw); break;
+            case 4: emu_stage<4, EMAX, COLFAST, LOG2C, INV>(lds, g, tw); break;
+            case 5: emu_stage<5, EMAX, COLFAST, LOG2C, INV>(lds, g, tw); break;
+            case 7: emu_stage<7, EMAX, COLFAST, LOG2C, INV>(lds, g, tw); break;
+            case 8: emu_stage<8, EMAX, COLFAST, LOG2C, INV>(lds, g, tw); break;
+            case 11: emu_stage<11, EMAX, COLFAST, LOG2C, INV>(lds, g, tw); break;
+            case 13: emu_stage<13, EMAX, COLFAST, LOG2C, INV>(lds, g, tw); break;
+            default: std::abort();
+        }
+        Ns *= R;
+    }
+}
+
+template <int EMAX, bool COLFAST, int LOG2C, bool INV>
+static double run_case(int n, int ncols) {
+    Plan P{};
+    P.n = n;
+    if (!factorize(n, P.radix, &P.nstages)) return -1.0;
+    std::vector<float2> tw(n);
+    for (int q = 0; q < n; ++q) {
+        double a = -2.0 * M_PI * q / n;
+        tw[q] = make_float2((float)std::cos(a), (float)std::sin(a));
+    }
+    int es = COLFAST ? ncols : 1;
+    int cs = COLFAST ? 1 : n;
+    std::vector<float2> lds(n * ncols);
+    std::vector<std::complex<double>> in(n * ncols);
+    srand(n * 7919 + ncols);
+    for (int c = 0; c < ncols; ++c)
+        for (int e = 0; e < n; ++e) {
+            double re = rand() / (double)RAND_MAX - 0.5, im = rand() / (double)RAND_MAX - 0.5;
+            in[c * n + e] = {re, im};
+            lds[e * es + c * cs] = make_float2((float)re, (float)im);
+        }
+    emu_fft<EMAX, COLFAST, LOG2C, INV>(lds, P, ncols, es, cs, tw.data());
+    double maxerr = 0.0, maxref = 0.0;
+    const double sgn = INV ? 1.0 : -1.0;
+    for (int c = 0; c < ncols; ++c)
+        for (int k = 0; k < n; ++k) {
+            std::complex<double> acc = 0.0;
+            for (int e = 0; e < n; ++e)
+                acc += in[c * n + e] * std::polar(1.0, sgn * 2.0 * M_PI * ((long)k * e % n) / n);
+            float2 g = lds[k * es + c * cs];
+            maxerr = std::fmax(maxerr, std::abs(acc - std::complex<double>(g.x, g.y)));
+            maxref = std::fmax(maxref, std::abs(acc));
+        }
+    return maxerr / maxref;
+}
+
+int main() {
+    const int lens[] = {1, 2, 3, 4, 5, 6, 7, 8, 10, 11, 12, 13, 14, 16, 22, 26, 32, 64,
+                        72, 80, 88, 96, 104, 112, 120, 128, 144, 160, 176, 192, 208, 224,
+                        240, 256, 384};
+    int bad = 0;
+    for (int n : lens) {
+        // column tiles of 16 (EMAX 24 covers n <= 384)
+        double e1 = (n * 16 <= kThreads * 24) ? run_case<24, true, 4, false>(n, 16) : 0.0;
+        double e2 = (n * 16 <= kThreads * 24) ? run_case<24, true, 4, true>(n, 16) : 0.0;
+        // row tiles: as many rows as fit in EMAX 16
+        int rows = (kThreads * 16) / n; if (rows < 1) rows = 1; if (rows > 32) rows = 32;
+        double e3 = (n <= kThreads * 16) ? run_case<16, false, 0, false>(n, rows) : 0.0;
+        double e4 = (n <= kThreads * 16) ? run_case<16, false, 0, true>(n, rows) : 0.0;
+        double worst = std::fmax(std::fmax(e1, e2), std::fmax(e3, e4));
+        bool ok = e1 >= 0 && e2 >= 0 && e3 >= 0 && e4 >= 0 && worst < 2e-6;
+        printf("n=%4d colfwd=%.2e colinv=%.2e rowfwd=%.2e rowinv=%.2e %s\n", n, e1, e2, e3, e4,
+               ok ? "ok" : "FAIL");
+        if (!ok) ++bad;
+    }
+    printf("%s\n", bad ? "FAILED" : "ALL OK");
+    return bad ? 1 : 0;
+}

@@ -1,0 +1,102 @@
+"""GPU parity of V-SHARP / SHARP (src/bgremove/vsharp.rs:88-199,
+src/bgremove/sharp.rs:64-131) against the f64 oracle.  Eroded masks must be
+identical; the local field is compared at FP32 tolerance."""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+def sphere_mask(n, radius):
+    g = np.arange(n) - n // 2
+    X, Y, Z = np.meshgrid(g, g, g, indexing="ij")
+    return ((X * X + Y * Y + Z * Z) <= radius * radius).astype(np.uint8).ravel(order="F")
+
+
+def _rel(a, b, m=None):
+    if m is None:
+        m = np.ones(a.size, bool)
+    m = m.astype(bool)
+    d = np.max(np.abs(a[m] - b[m])) if m.any() else 0.0
+    s = np.max(np.abs(b[m])) if m.any() else 1.0
+    return d / (s if s > 0 else 1.0)
+
+
+@pytest.mark.parametrize("n,vs,radii", [
+    (16, (1.0, 1.0, 1.0), [6.0, 4.0, 3.0, 2.0]),
+    (16, (0.8, 1.0, 1.2), [4.0, 3.0, 2.0]),
+    (24, (1.0, 1.0, 1.0), [5.0, 4.0, 3.0, 2.0, 1.0]),
+    (20, (0.6, 0.6, 0.6), [3.0, 2.4, 1.8]),
+])
+def test_vsharp_matches_oracle(qsm, oracle, n, vs, radii):
+    rng = np.random.default_rng(n)
+    dims = (n, n, n)
+    g = np.arange(n)
+    Z = np.broadcast_to(g[None, None, :], dims).ravel(order="F")
+    field = Z * 0.1 + rng.standard_normal(n ** 3) * 0.01
+    mask = sphere_mask(n, n / 3)
+    grid = qsm.Grid(dims, vs)
+    local, m = qsm.vsharp_with_radii(field, mask, grid, radii, 0.05)
+    lo, mo = oracle.vsharp_with_radii(field, mask, dims, vs, radii, 0.05)
+    assert np.array_equal(m, mo)
+    err = _rel(local, lo, mo)
+    print(n, vs, radii, "mask voxels", int(mo.sum()), "max rel err", err)
+    assert err < 1e-4
+    assert np.all(local[m == 0] == 0.0)
+
+
+def test_single_radius_is_sharp(qsm, oracle):
+    # src/bgremove/vsharp.rs:341-371
+    n = 8
+    dims = (n, n, n)
+    field = np.arange(n ** 3) * 0.001
+    mask = np.ones(n ** 3, np.uint8)
+    local, m = qsm.vsharp_with_radii(field, mask, qsm.Grid(dims), [3.0], 0.05)
+    lo, mo = oracle.sharp_core(field, mask, dims, (1, 1, 1), 0.05, 3.0)
+    assert np.array_equal(m, mo)
+    assert np.max(np.abs(local - lo)) < 1e-5 * max(1.0, np.max(np.abs(lo)))
+
+
+def test_empty_radii(qsm):
+    n = 8
+    field = np.arange(n ** 3) * 0.001
+    mask = np.ones(n ** 3, np.uint8)
+    calls = []
+    local, m = qsm.vsharp_with_radii(field, mask, qsm.Grid((n, n, n)), [], 0.05,
+                                     progress=lambda a, b: calls.append((a, b)))
+    assert np.all(local == 0.0) and np.array_equal(m, mask) and calls == []
+
+
+def test_unsorted_radii(qsm):
+    n = 8
+    field = np.arange(n ** 3) * 0.001
+    mask = np.ones(n ** 3, np.uint8)
+    g = qsm.Grid((n, n, n))
+    a = qsm.vsharp_with_radii(field, mask, g, [4.0, 3.0, 2.0], 0.05)
+    b = qsm.vsharp_with_radii(field, mask, g, [2.0, 4.0, 3.0], 0.05)
+    assert np.array_equal(a[1], b[1]) and np.array_equal(a[0], b[0])
+
+
+def test_progress_sequence(qsm, oracle):
+    n = 8
+    field = np.arange(n ** 3) * 0.001
+    mask = np.ones(n ** 3, np.uint8)
+    calls, ocalls = [], []
+    qsm.vsharp_with_radii(field, mask, qsm.Grid((n, n, n)), [4.0, 3.0, 2.0], 0.05,
+                          progress=lambda a, b: calls.append((a, b)))
+    oracle.vsharp_with_radii(field, mask, (n, n, n), (1, 1, 1), [4.0, 3.0, 2.0], 0.05,
+                             progress=lambda a, b: ocalls.append((a, b)))
+    assert calls == ocalls == [(1, 3), (2, 3), (3, 3)]
+
+
+def test_vsharp_defaults_phantom(qsm, oracle, phantom):
+    # default radii 12..1 mm on a 64^3 phantom with harmonic background
+    ph = phantom.make_phantom((64, 64, 64), n_ellipsoids=8, seed=1004)
+    ph = phantom.add_background(ph, seed=1004)
+    grid = qsm.Grid(ph.dims, ph.voxel_size)
+    local, m = qsm.vsharp(ph.total_field, ph.mask, grid, qsm.VsharpParams.default())
+    lo, mo = oracle.vsharp(ph.total_field, ph.mask, ph.dims, ph.voxel_size)
+    assert np.array_equal(m, mo)
+    e = oracle.nrmse(local, lo, mo)
+    print("vsharp 64^3 nrmse", e, "maxabs", np.max(np.abs(local - lo)))
+    assert e < 1e-4
