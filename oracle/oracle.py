@@ -69,6 +69,7 @@ def lib():
             "orc_rmse": (_F, [_D, _D, _U8, _SZ]),
             "orc_nrmse": (_F, [_D, _D, _U8, _SZ]),
             "orc_num_threads": (C.c_int, []),
+            "orc_set_num_threads": (None, [C.c_int]),
         }
         for name, (res, args) in sig.items():
             fn = getattr(L, name)
@@ -104,6 +105,10 @@ def _cb(progress):
 
 def num_threads() -> int:
     return lib().orc_num_threads()
+
+
+def set_num_threads(n: int) -> None:
+    lib().orc_set_num_threads(int(n))
 
 
 def fftfreq(n: int, d: float) -> np.ndarray:

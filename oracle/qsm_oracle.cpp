@@ -200,6 +200,14 @@ std::vector<double> fft_real_kernel(const double* kern, size_t nx, size_t ny, si
 
 extern "C" {
 
+void orc_set_num_threads(int n) {
+#ifdef _OPENMP
+    omp_set_num_threads(n > 0 ? n : 1);
+#else
+    (void)n;
+#endif
+}
+
 int orc_num_threads(void) {
 #ifdef _OPENMP
     return omp_get_max_threads();

@@ -110,8 +110,11 @@ void orc_dipole_forward(const double* chi, size_t nx, size_t ny, size_t nz,
 double orc_rmse(const double* a, const double* b, const uint8_t* mask, size_t n);
 double orc_nrmse(const double* a, const double* b, const uint8_t* mask, size_t n);
 
-/* Number of OpenMP threads the FFT passes use. */
+/* Number of OpenMP threads the FFT passes use, and a setter (mirrors the
+ * reference's rayon pool size; src/inversion/tv.rs:181-184 runs a 1-thread
+ * pool to check thread-count invariance). */
 int orc_num_threads(void);
+void orc_set_num_threads(int n);
 
 #ifdef __cplusplus
 }

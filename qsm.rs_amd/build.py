@@ -16,7 +16,12 @@ CSRC = os.path.join(HERE, "csrc")
 BUILD = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libqsm_gpu.so")
 
-SOURCES = ["qsm_kernels.hip", "qsm_vsharp.hip", "qsm_runtime.hip"]
+# qsm_pass.hip is compiled once per specialised axis length (and once with
+# length 0 for the runtime-plan fallback); keep in step with
+# QSM_FIXED_LENGTHS in qsm_misc.hip.
+FIXED_LENGTHS = [64, 128, 144, 160, 176, 192, 208, 224, 240, 256, 384]
+UNITS = ([("qsm_pass.hip", f"qsm_pass_{n}", [f"-DQSM_PASS_N={n}"]) for n in [0] + FIXED_LENGTHS]
+         + [("qsm_misc.hip", "qsm_misc", []), ("qsm_runtime.hip", "qsm_runtime", [])])
 
 HIPCC_FLAGS = [
     "-O3", "-std=c++17", "--offload-arch=gfx950", "-fPIC",
@@ -33,9 +38,10 @@ def hipcc() -> str:
     return "hipcc"
 
 
-def _compile(src: str) -> str:
-    obj = os.path.join(BUILD, os.path.splitext(src)[0] + ".o")
-    cmd = [hipcc(), *HIPCC_FLAGS, "-c", os.path.join(CSRC, src), "-o", obj]
+def _compile(unit) -> str:
+    src, name, defs = unit
+    obj = os.path.join(BUILD, name + ".o")
+    cmd = [hipcc(), *HIPCC_FLAGS, *defs, "-c", os.path.join(CSRC, src), "-o", obj]
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         sys.stderr.write(r.stdout + r.stderr)
@@ -43,11 +49,10 @@ def _compile(src: str) -> str:
     return obj
 
 
-def build(jobs: int = 4, verbose: bool = True) -> str:
+def build(jobs: int = 8, verbose: bool = True) -> str:
     os.makedirs(BUILD, exist_ok=True)
-    srcs = [s for s in SOURCES if os.path.exists(os.path.join(CSRC, s))]
     with ThreadPoolExecutor(max_workers=max(1, jobs)) as ex:
-        objs = list(ex.map(_compile, srcs))
+        objs = list(ex.map(_compile, UNITS))
     tmp = LIB + ".tmp"
     cmd = [hipcc(), "--offload-arch=gfx950", "-shared", "-fPIC", "-o", tmp, *objs]
     r = subprocess.run(cmd, capture_output=True, text=True)
@@ -62,6 +67,6 @@ def build(jobs: int = 4, verbose: bool = True) -> str:
 
 if __name__ == "__main__":
     ap = argparse.ArgumentParser()
-    ap.add_argument("--jobs", type=int, default=4)
+    ap.add_argument("--jobs", type=int, default=8)
     a = ap.parse_args()
     build(a.jobs)

@@ -30,19 +30,22 @@ struct Plan {
     int n;
     int nstages;
     int radix[kMaxStages];
+    int ns[kMaxStages];          // product of the radices before stage s
+    uint64_t m_nb[kMaxStages];   // magic for division by n / radix[s]
+    uint64_t m_ns[kMaxStages];   // magic for division by ns[s]
     int tw_off;  // offset (in float2) of this length's table exp(-2 pi i q/n), q<n
 };
 
-__host__ __device__ inline float2 cmul(float2 a, float2 b) {
+__host__ __device__ __forceinline__ float2 cmul(float2 a, float2 b) {
     return make_float2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
 }
-__host__ __device__ inline float2 cadd(float2 a, float2 b) { return make_float2(a.x + b.x, a.y + b.y); }
-__host__ __device__ inline float2 csub(float2 a, float2 b) { return make_float2(a.x - b.x, a.y - b.y); }
-__host__ __device__ inline float2 cconj(float2 a) { return make_float2(a.x, -a.y); }
-__host__ __device__ inline float2 cscale(float2 a, float s) { return make_float2(a.x * s, a.y * s); }
+__host__ __device__ __forceinline__ float2 cadd(float2 a, float2 b) { return make_float2(a.x + b.x, a.y + b.y); }
+__host__ __device__ __forceinline__ float2 csub(float2 a, float2 b) { return make_float2(a.x - b.x, a.y - b.y); }
+__host__ __device__ __forceinline__ float2 cconj(float2 a) { return make_float2(a.x, -a.y); }
+__host__ __device__ __forceinline__ float2 cscale(float2 a, float s) { return make_float2(a.x * s, a.y * s); }
 // multiply by -i (forward) or +i (inverse)
 template <bool INV>
-__host__ __device__ inline float2 mul_mi(float2 a) {
+__host__ __device__ __forceinline__ float2 mul_mi(float2 a) {
     return INV ? make_float2(-a.y, a.x) : make_float2(a.y, -a.x);
 }
 
@@ -53,7 +56,7 @@ struct Dft;
 
 template <bool INV>
 struct Dft<2, INV> {
-    __host__ __device__ static inline void run(float2* v, const float2*, int) {
+    __host__ __device__ static __forceinline__ void run(float2* v, const float2*, int) {
         float2 a = v[0], b = v[1];
         v[0] = cadd(a, b);
         v[1] = csub(a, b);
@@ -62,7 +65,7 @@ struct Dft<2, INV> {
 
 template <bool INV>
 struct Dft<3, INV> {
-    __host__ __device__ static inline void run(float2* v, const float2*, int) {
+    __host__ __device__ static __forceinline__ void run(float2* v, const float2*, int) {
         const float c = -0.5f;
         const float s = 0.86602540378443864676f;  // sin(2 pi/3)
         float2 a = v[0], b = v[1], d = v[2];
@@ -77,7 +80,7 @@ struct Dft<3, INV> {
 
 template <bool INV>
 struct Dft<4, INV> {
-    __host__ __device__ static inline void run(float2* v, const float2*, int) {
+    __host__ __device__ static __forceinline__ void run(float2* v, const float2*, int) {
         float2 a = cadd(v[0], v[2]), b = csub(v[0], v[2]);
         float2 c = cadd(v[1], v[3]), d = mul_mi<INV>(csub(v[1], v[3]));
         v[0] = cadd(a, c);
@@ -89,7 +92,7 @@ struct Dft<4, INV> {
 
 template <bool INV>
 struct Dft<5, INV> {
-    __host__ __device__ static inline void run(float2* v, const float2*, int) {
+    __host__ __device__ static __forceinline__ void run(float2* v, const float2*, int) {
         const float c1 = 0.30901699437494742410f;   // cos(2 pi/5)
         const float c2 = -0.80901699437494742410f;  // cos(4 pi/5)
         const float s1 = 0.95105651629515357212f;   // sin(2 pi/5)
@@ -111,7 +114,7 @@ struct Dft<5, INV> {
 
 template <bool INV>
 struct Dft<8, INV> {
-    __host__ __device__ static inline void run(float2* v, const float2*, int) {
+    __host__ __device__ static __forceinline__ void run(float2* v, const float2*, int) {
         const float h = 0.70710678118654752440f;
         float2 e[4] = {v[0], v[2], v[4], v[6]};
         float2 o[4] = {v[1], v[3], v[5], v[7]};
@@ -134,38 +137,93 @@ struct Dft<8, INV> {
     }
 };
 
-// Generic odd prime radix: direct DFT with roots taken from the length-n
-// twiddle table (R divides n, so exp(-2 pi i m/R) = tw[m*n/R]).
+// Odd prime radix R: pairs x_r and x_{R-r} share cos and (negated) sin, so
+//   X_q     = A_q -+ i B_q,   X_{R-q} = A_q +- i B_q   (forward / inverse)
+//   A_q = x0 + sum_r (x_r + x_{R-r}) cos(2 pi r q/R)
+//   B_q =      sum_r (x_r - x_{R-r}) sin(2 pi r q/R),   r, q = 1..(R-1)/2,
+// with the cosines and sines as compile-time constants.
+template <int R>
+struct PrimeTable;
+template <>
+struct PrimeTable<7> {
+    static constexpr float c[7] = {1.0f, 0.62348980185873359439f, -0.22252093395631433737f,
+                                   -0.90096886790241903498f, -0.90096886790241914600f,
+                                   -0.22252093395631458717f, 0.62348980185873337234f};
+    static constexpr float s[7] = {0.0f, 0.78183148246802980363f, 0.97492791218182361934f,
+                                   0.43388373911755823142f, -0.43388373911755800938f,
+                                   -0.97492791218182361934f, -0.78183148246802991466f};
+};
+template <>
+struct PrimeTable<11> {
+    static constexpr float c[11] = {1.0f, 0.84125353283118120551f, 0.41541501300188643508f,
+                                    -0.14231483827328500480f, -0.65486073394528498959f,
+                                    -0.95949297361449736865f, -0.95949297361449747967f,
+                                    -0.65486073394528521163f, -0.14231483827328522684f,
+                                    0.41541501300188604651f, 0.84125353283118120551f};
+    static constexpr float s[11] = {0.0f, 0.54064081745559755543f, 0.90963199535451833011f,
+                                    0.98982144188093279524f, 0.75574957435425826890f,
+                                    0.28173255684142967104f, -0.28173255684142939348f,
+                                    -0.75574957435425815788f, -0.98982144188093268422f,
+                                    -0.90963199535451855215f, -0.54064081745559744441f};
+};
+template <>
+struct PrimeTable<13> {
+    static constexpr float c[13] = {1.0f, 0.88545602565320991051f, 0.56806474673115592289f,
+                                    0.12053668025532300601f, -0.35460488704253545489f,
+                                    -0.74851074817110119231f, -0.97094181742605201180f,
+                                    -0.97094181742605212282f, -0.74851074817110130333f,
+                                    -0.35460488704253589898f, 0.12053668025532320029f,
+                                    0.56806474673115481266f, 0.88545602565321002153f};
+    static constexpr float s[13] = {0.0f, 0.46472317204376850652f, 0.82298386589365635224f,
+                                    0.99270887409805397272f, 0.93501624268541483342f,
+                                    0.66312265824079519305f, 0.23931566428755768339f,
+                                    -0.23931566428755743359f, -0.66312265824079497101f,
+                                    -0.93501624268541472240f, -0.99270887409805397272f,
+                                    -0.82298386589365701838f, -0.46472317204376839550f};
+};
+
 template <int R, bool INV>
-struct DftGeneric {
-    __host__ __device__ static inline void run(float2* v, const float2* tw, int n) {
-        float2 in[R];
+struct DftOddPrime {
+    __host__ __device__ static __forceinline__ void run(float2* v, const float2*, int) {
+        constexpr int M = (R - 1) / 2;
+        float2 sp[M], sm[M];
 #pragma unroll
-        for (int r = 0; r < R; ++r) in[r] = v[r];
-        const int st = n / R;
-#pragma unroll
-        for (int q = 0; q < R; ++q) {
-            float2 acc = in[0];
-#pragma unroll
-            for (int r = 1; r < R; ++r) {
-                float2 w = tw[((r * q) % R) * st];
-                if (INV) w = cconj(w);
-                acc = cadd(acc, cmul(in[r], w));
-            }
-            v[q] = acc;
+        for (int r = 1; r <= M; ++r) {
+            sp[r - 1] = cadd(v[r], v[R - r]);
+            sm[r - 1] = csub(v[r], v[R - r]);
         }
+        const float2 x0 = v[0];
+        float2 tot = x0;
+#pragma unroll
+        for (int r = 0; r < M; ++r) tot = cadd(tot, sp[r]);
+#pragma unroll
+        for (int q = 1; q <= M; ++q) {
+            float2 A = x0, B = make_float2(0.0f, 0.0f);
+#pragma unroll
+            for (int r = 1; r <= M; ++r) {
+                const float c = PrimeTable<R>::c[(r * q) % R];
+                const float s = PrimeTable<R>::s[(r * q) % R];
+                A = make_float2(A.x + sp[r - 1].x * c, A.y + sp[r - 1].y * c);
+                B = make_float2(B.x + sm[r - 1].x * s, B.y + sm[r - 1].y * s);
+            }
+            // -i*B for the forward transform, +i*B for the inverse
+            const float2 jb = mul_mi<INV>(B);
+            v[q] = cadd(A, jb);
+            v[R - q] = csub(A, jb);
+        }
+        v[0] = tot;
     }
 };
-template <bool INV> struct Dft<7, INV> : DftGeneric<7, INV> {};
-template <bool INV> struct Dft<11, INV> : DftGeneric<11, INV> {};
-template <bool INV> struct Dft<13, INV> : DftGeneric<13, INV> {};
+template <bool INV> struct Dft<7, INV> : DftOddPrime<7, INV> {};
+template <bool INV> struct Dft<11, INV> : DftOddPrime<11, INV> {};
+template <bool INV> struct Dft<13, INV> : DftOddPrime<13, INV> {};
 
 // --------------------------------------------------------- index helpers --
 // Exact q = j / d for j, d < 2^16 via m = ceil(2^32 / d).
 __host__ __device__ inline uint64_t magic_for(uint32_t d) {
     return ((uint64_t(1) << 32) + d - 1) / d;
 }
-__host__ __device__ inline uint32_t fastdiv(uint32_t j, uint64_t m) {
+__host__ __device__ __forceinline__ uint32_t fastdiv(uint32_t j, uint64_t m) {
     return (uint32_t)(((uint64_t)j * m) >> 32);
 }
 
@@ -190,8 +248,21 @@ __host__ __device__ inline StageGeom make_geom(int n, int ncols, int es, int cs,
     return g;
 }
 
+// Stage geometry from a plan's precomputed per-stage values (no divisions).
+__host__ __device__ __forceinline__ StageGeom plan_geom(const Plan& P, int s, int ncols, int es, int cs) {
+    StageGeom g;
+    const int R = P.radix[s];
+    g.n = P.n; g.ncols = ncols; g.es = es; g.cs = cs;
+    g.Ns = P.ns[s];
+    g.nb = P.n / R;
+    g.tstep = g.nb / g.Ns;
+    g.m_nb = P.m_nb[s];
+    g.m_Ns = P.m_ns[s];
+    return g;
+}
+
 template <bool COLFAST, int LOG2C>
-__host__ __device__ inline void split_b(int b, const StageGeom& g, int& j, int& c) {
+__host__ __device__ __forceinline__ void split_b(int b, const StageGeom& g, int& j, int& c) {
     if (COLFAST) {
         c = b & ((1 << LOG2C) - 1);
         j = b >> LOG2C;
@@ -203,7 +274,7 @@ __host__ __device__ inline void split_b(int b, const StageGeom& g, int& j, int& 
 
 // Load phase of a stage: thread tid gathers the inputs of its butterflies.
 template <int R, int BMAX, bool COLFAST, int LOG2C>
-__host__ __device__ inline void stage_load(const float2* lds, int tid, const StageGeom& g,
+__host__ __device__ __forceinline__ void stage_load(const float2* lds, int tid, const StageGeom& g,
                                            float2 (&v)[BMAX][R]) {
     const int total = g.nb * g.ncols;
 #pragma unroll
@@ -221,7 +292,7 @@ __host__ __device__ inline void stage_load(const float2* lds, int tid, const Sta
 
 // Compute + store phase of a stage.
 template <int R, int BMAX, bool COLFAST, int LOG2C, bool INV>
-__host__ __device__ inline void stage_store(float2* lds, int tid, const StageGeom& g,
+__host__ __device__ __forceinline__ void stage_store(float2* lds, int tid, const StageGeom& g,
                                             float2 (&v)[BMAX][R], const float2* tw) {
     const int total = g.nb * g.ncols;
 #pragma unroll
@@ -259,13 +330,13 @@ struct Bmax {
 // Full in-LDS transform on the device.  Caller must have synchronised after
 // filling LDS; returns after a final barrier.  n*ncols <= kThreads*EMAX.
 template <int EMAX, bool COLFAST, int LOG2C, bool INV>
-__device__ inline void lds_fft(float2* lds, const Plan& P, int ncols, int es, int cs,
+__device__ __forceinline__ void lds_fft(float2* lds, const Plan& P, int ncols, int es, int cs,
                                const float2* tw) {
     const int tid = threadIdx.x;
-    int Ns = 1;
-    for (int s = 0; s < P.nstages; ++s) {
+    const int nst = P.nstages;
+    for (int s = 0; s < nst; ++s) {
         const int R = P.radix[s];
-        const StageGeom g = make_geom(P.n, ncols, es, cs, R, Ns);
+        const StageGeom g = plan_geom(P, s, ncols, es, cs);
 #define QSM_STAGE(RR)                                                         \
     case RR: {                                                                \
         float2 v[Bmax<RR, EMAX>::value][RR];                                  \
@@ -286,7 +357,6 @@ __device__ inline void lds_fft(float2* lds, const Plan& P, int ncols, int es, in
             default: break;
         }
 #undef QSM_STAGE
-        Ns *= R;
     }
 }
 #endif
@@ -312,6 +382,20 @@ inline bool factorize(int n, int* radix, int* nstages) {
         while (m % p == 0) if (!push(p)) return false;
     if (m != 1) return false;
     *nstages = s;
+    return true;
+}
+
+// Fill a plan's radix sequence and per-stage geometry for length n.
+inline bool make_plan_stages(int n, Plan& p) {
+    p.n = n;
+    if (!factorize(n, p.radix, &p.nstages)) return false;
+    int ns = 1;
+    for (int s = 0; s < p.nstages; ++s) {
+        p.ns[s] = ns;
+        p.m_nb[s] = magic_for((uint32_t)(n / p.radix[s]));
+        p.m_ns[s] = magic_for((uint32_t)ns);
+        ns *= p.radix[s];
+    }
     return true;
 }
 

@@ -23,7 +23,8 @@ constexpr int kLog2Tile = 4;
 
 struct LaunchTable {
     int nvol;
-    int start[kMaxVolumes + 1];  // start[v] = first block of volume v; start[nvol] = total
+    int start[kMaxVolumes + 1];  // start[v] = first block of entry v; start[nvol] = total
+    int vol[kMaxVolumes];        // global volume index of entry v
 };
 
 // Convergence state of one TV-ADMM volume, written by the reduce kernel.
@@ -74,6 +75,14 @@ __device__ __host__ inline int find_vol(const LaunchTable& lt, int bid) {
     return v;
 }
 
+// Global volume index of flat block `bid`; lb receives the block index
+// within that volume.
+__device__ __host__ inline int locate(const LaunchTable& lt, int bid, int& lb) {
+    const int v = find_vol(lt, bid);
+    lb = bid - lt.start[v];
+    return lt.vol[v];
+}
+
 // Flat block index -> logical block, so that the blocks one XCD runs
 // (hardware dispatches consecutive block ids round-robin over 8 XCDs) are a
 // contiguous range of logical blocks sharing that XCD's L2.
@@ -82,5 +91,30 @@ __device__ __host__ inline int xcd_remap(int bid, int total) {
     const int x = bid % 8, t = bid / 8;
     return x * q + (x < r ? x : r) + t;
 }
+
+// Everything one pass launch needs; kind selects the kernel.
+enum PassKind : int {
+    P_R2C_TVDIV = 0, P_R2C_F64, P_R2C_U8, P_R2C_F32, P_R2C_SPHERE,
+    P_Y_FWD, P_Y_INV,
+    P_Z_TV_SETUP, P_Z_TV_SOLVE, P_Z_VS_FWD, P_Z_VS_RADIUS, P_Z_VS_DECONV,
+    P_C2R_TV, P_C2R_VS_SELECT, P_C2R_VS_OUT,
+    P_KIND_COUNT
+};
+
+struct PassArgs {
+    hipStream_t stream;
+    const VolDesc* vols;
+    const float2* tw;
+    const VolStatus* status;  // convergence flags checked at block entry (may be null)
+    int sel, sel2;            // spectrum selectors (y / z passes, R2C output)
+    int parity;               // TV iteration parity
+    int ridx, first, divide;  // V-SHARP radius index / first radius / SHARP division
+    double delta;             // erosion threshold
+};
+
+// Launch `kind` over the volumes of `lt`, all of which share the length of
+// the pass axis `n`: a compile-time-length kernel when n is one of the
+// specialised lengths, else the runtime-plan kernel sized by emax.
+void launch_pass(int kind, int n, int emax, const LaunchTable& lt, const PassArgs& a);
 
 }  // namespace qsm

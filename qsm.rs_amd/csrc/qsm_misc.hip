@@ -1,0 +1,123 @@
+// Small kernels outside the FFT passes: the FP64 relative-change reduction,
+// masked f64 output, fills, and the pass dispatcher over specialised lengths.
+#include "qsm_kernels.hpp"
+
+namespace qsm {
+
+// relative_change = ||x - x_prev|| / (||x|| + 1e-20); converged when < tol
+// (src/inversion/admm.rs:117-119, src/utils/mod.rs:69-77).  One block per
+// volume, fixed summation order, so repeated runs give identical results.
+__global__ __launch_bounds__(kThreads) void k_tv_reduce(const VolDesc* __restrict__ vols, int nvol,
+                                                        VolStatus* __restrict__ status,
+                                                        VolStatus* __restrict__ mirror) {
+    __shared__ double sh[kThreads / 64];
+    const int v = blockIdx.x;
+    if (v >= nvol) return;
+    const VolDesc& V = vols[v];
+    if (status[v].done) return;
+    double sd = 0.0, sx = 0.0;
+    for (int b = threadIdx.x; b < V.npartial; b += kThreads) {
+        sd += V.partial[2 * b];
+        sx += V.partial[2 * b + 1];
+    }
+    const double tsd = block_sum(sd, sh);
+    const double tsx = block_sum(sx, sh);
+    if (threadIdx.x == 0) {
+        const double rel = sqrt(tsd) / (sqrt(tsx) + 1e-20);
+        VolStatus st = status[v];
+        st.iters += 1;
+        st.rel = rel;
+        if (rel < V.tol) st.done = 1;
+        status[v] = st;
+        if (mirror) {
+            // host-visible copy, read by the host loop after an event wait
+            mirror[v] = st;
+            __threadfence_system();
+        }
+    }
+}
+
+// chi = x where mask != 0, else 0 (src/utils/mask.rs:47-53), widened to f64.
+__global__ __launch_bounds__(kThreads) void k_tv_output(const VolDesc* __restrict__ vols, LaunchTable lt,
+                                                        const VolStatus* __restrict__ status) {
+    int lb;
+    const int gv = locate(lt, blockIdx.x, lb);
+    const VolDesc& V = vols[gv];
+    const size_t N = (size_t)V.nx * V.ny * V.nz;
+    const float* x = V.x[status[gv].iters & 1];
+    const size_t end = (size_t)(lb + 1) * kThreads * 4;
+    for (size_t i = (size_t)lb * kThreads * 4 + threadIdx.x; i < N && i < end; i += kThreads)
+        V.out[i] = V.mask[i] ? (double)x[i] : 0.0;
+}
+
+__global__ __launch_bounds__(kThreads) void k_vs_init(const VolDesc* __restrict__ vols, LaunchTable lt) {
+    int lb;
+    const int gv = locate(lt, blockIdx.x, lb);
+    const VolDesc& V = vols[gv];
+    const size_t N = (size_t)V.nx * V.ny * V.nz;
+    const size_t end = (size_t)(lb + 1) * kThreads * 4;
+    for (size_t i = (size_t)lb * kThreads * 4 + threadIdx.x; i < N && i < end; i += kThreads) {
+        V.vs_local[i] = 0.0f;
+        V.vs_final[i] = 0;
+    }
+}
+
+// Zero fill (a kernel rather than a memset, so the sequence stays capturable).
+__global__ __launch_bounds__(kThreads) void k_fill_zero(float* __restrict__ p, size_t n) {
+    size_t i = (size_t)blockIdx.x * kThreads + threadIdx.x;
+    const size_t stride = (size_t)gridDim.x * kThreads;
+    for (; i < n; i += stride) p[i] = 0.0f;
+}
+
+__global__ void k_status_reset(VolStatus* status, int nvol) {
+    const int v = blockIdx.x * blockDim.x + threadIdx.x;
+    if (v < nvol) {
+        VolStatus s;
+        s.done = 0;
+        s.iters = 0;
+        s.rel = 0.0;
+        status[v] = s;
+    }
+}
+
+void launch_tv_reduce(hipStream_t st, const VolDesc* d, int nvol, VolStatus* status, VolStatus* mirror) {
+    k_tv_reduce<<<dim3(nvol), dim3(kThreads), 0, st>>>(d, nvol, status, mirror);
+}
+void launch_tv_output(hipStream_t st, const VolDesc* d, const LaunchTable& lt, const VolStatus* status) {
+    k_tv_output<<<dim3(lt.start[lt.nvol]), dim3(kThreads), 0, st>>>(d, lt, status);
+}
+void launch_vs_init(hipStream_t st, const VolDesc* d, const LaunchTable& lt) {
+    k_vs_init<<<dim3(lt.start[lt.nvol]), dim3(kThreads), 0, st>>>(d, lt);
+}
+void launch_fill_zero(hipStream_t st, float* p, size_t n) {
+    if (n == 0) return;
+    size_t blocks = (n + kThreads - 1) / kThreads;
+    if (blocks > 4096) blocks = 4096;
+    k_fill_zero<<<dim3((unsigned)blocks), dim3(kThreads), 0, st>>>(p, n);
+}
+void launch_status_reset(hipStream_t st, VolStatus* s, int nvol) {
+    k_status_reset<<<dim3((nvol + 63) / 64), dim3(64), 0, st>>>(s, nvol);
+}
+
+// ------------------------------------------------------------ dispatcher --
+#define QSM_FIXED_LENGTHS(X) X(64) X(128) X(144) X(160) X(176) X(192) X(208) X(224) X(240) X(256) X(384)
+#define QSM_DECLARE(n) void launch_fixed_##n(int kind, const LaunchTable& lt, const PassArgs& a);
+QSM_FIXED_LENGTHS(QSM_DECLARE)
+void launch_generic(int kind, int emax, const LaunchTable& lt, const PassArgs& a);
+
+bool fixed_length(int n) {
+#define QSM_IS(m) if (n == m) return true;
+    QSM_FIXED_LENGTHS(QSM_IS)
+    return false;
+}
+
+void launch_pass(int kind, int n, int emax, const LaunchTable& lt, const PassArgs& a) {
+    if (lt.nvol == 0 || lt.start[lt.nvol] == 0) return;
+    switch (n) {
+#define QSM_CASE(m) case m: launch_fixed_##m(kind, lt, a); return;
+        QSM_FIXED_LENGTHS(QSM_CASE)
+        default: launch_generic(kind, emax, lt, a); return;
+    }
+}
+
+}  // namespace qsm

@@ -1,0 +1,66 @@
+// One translation unit per specialised axis length: compiled with
+// -DQSM_PASS_N=<length> it defines launch_fixed_<length>(), with
+// -DQSM_PASS_N=0 the runtime-plan fallback launch_generic().
+#include "qsm_kernels.hpp"
+
+#ifndef QSM_PASS_N
+#error "compile with -DQSM_PASS_N=<axis length> (0 for the runtime-plan kernels)"
+#endif
+
+namespace qsm {
+
+static bool is_row_kind(int kind) {
+    return kind == P_R2C_TVDIV || kind == P_R2C_F64 || kind == P_R2C_U8 || kind == P_R2C_F32 ||
+           kind == P_R2C_SPHERE || kind == P_C2R_TV || kind == P_C2R_VS_SELECT || kind == P_C2R_VS_OUT;
+}
+
+template <int N, int EG>
+static void launch_rows(int kind, const LaunchTable& lt, const PassArgs& a) {
+    const dim3 g(lt.start[lt.nvol]), b(kThreads);
+    hipStream_t st = a.stream;
+    switch (kind) {
+        case P_R2C_TVDIV: k_r2c_x<N, EG, SRC_TVDIV><<<g, b, 0, st>>>(a.vols, lt, a.tw, a.sel, a.parity, a.status); break;
+        case P_R2C_F64: k_r2c_x<N, EG, SRC_F64><<<g, b, 0, st>>>(a.vols, lt, a.tw, a.sel, 0, a.status); break;
+        case P_R2C_U8: k_r2c_x<N, EG, SRC_U8><<<g, b, 0, st>>>(a.vols, lt, a.tw, a.sel, 0, a.status); break;
+        case P_R2C_F32: k_r2c_x<N, EG, SRC_F32><<<g, b, 0, st>>>(a.vols, lt, a.tw, a.sel, 0, a.status); break;
+        case P_R2C_SPHERE: k_r2c_x<N, EG, SRC_SPHERE><<<g, b, 0, st>>>(a.vols, lt, a.tw, a.sel, a.ridx, a.status); break;
+        case P_C2R_TV: k_c2r_tv<N, EG><<<g, b, 0, st>>>(a.vols, lt, a.tw, a.parity, a.status); break;
+        case P_C2R_VS_SELECT: k_c2r_vs_select<N, EG><<<g, b, 0, st>>>(a.vols, lt, a.tw, a.ridx, a.delta); break;
+        case P_C2R_VS_OUT: k_c2r_vs_out<N, EG><<<g, b, 0, st>>>(a.vols, lt, a.tw); break;
+        default: break;
+    }
+}
+
+template <int N, int EG>
+static void launch_cols(int kind, const LaunchTable& lt, const PassArgs& a) {
+    const dim3 g(lt.start[lt.nvol]), b(kThreads);
+    hipStream_t st = a.stream;
+    switch (kind) {
+        case P_Y_FWD: k_y_pass<N, EG, false><<<g, b, 0, st>>>(a.vols, lt, a.tw, a.sel, a.sel2, a.status); break;
+        case P_Y_INV: k_y_pass<N, EG, true><<<g, b, 0, st>>>(a.vols, lt, a.tw, a.sel, a.sel2, a.status); break;
+        case P_Z_TV_SETUP: k_z_tv<N, EG, ZOP_TV_SETUP><<<g, b, 0, st>>>(a.vols, lt, a.tw, a.status); break;
+        case P_Z_TV_SOLVE: k_z_tv<N, EG, ZOP_TV_SOLVE><<<g, b, 0, st>>>(a.vols, lt, a.tw, a.status); break;
+        case P_Z_VS_FWD: k_z_vs<N, EG, ZOP_VS_FWD><<<g, b, 0, st>>>(a.vols, lt, a.tw, a.sel, 0, 0); break;
+        case P_Z_VS_RADIUS: k_z_vs<N, EG, ZOP_VS_RADIUS><<<g, b, 0, st>>>(a.vols, lt, a.tw, 0, a.ridx, a.first); break;
+        case P_Z_VS_DECONV: k_z_vs<N, EG, ZOP_VS_DECONV><<<g, b, 0, st>>>(a.vols, lt, a.tw, 0, 0, a.divide); break;
+        default: break;
+    }
+}
+
+#if QSM_PASS_N > 0
+#define QSM_CAT2(a, b) a##b
+#define QSM_CAT(a, b) QSM_CAT2(a, b)
+void QSM_CAT(launch_fixed_, QSM_PASS_N)(int kind, const LaunchTable& lt, const PassArgs& a) {
+    if (is_row_kind(kind)) launch_rows<QSM_PASS_N, 0>(kind, lt, a);
+    else launch_cols<QSM_PASS_N, 0>(kind, lt, a);
+}
+#else
+void launch_generic(int kind, int emax, const LaunchTable& lt, const PassArgs& a) {
+    if (is_row_kind(kind)) launch_rows<0, 16>(kind, lt, a);
+    else if (emax <= 16) launch_cols<0, 16>(kind, lt, a);
+    else if (emax <= 24) launch_cols<0, 24>(kind, lt, a);
+    else launch_cols<0, 32>(kind, lt, a);
+}
+#endif
+
+}  // namespace qsm

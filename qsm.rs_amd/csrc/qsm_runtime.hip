@@ -15,31 +15,12 @@
 
 namespace qsm {
 
-void launch_r2c(int emax, int src, hipStream_t st, const VolDesc* d, const LaunchTable& lt,
-                const float2* tw, int out_sel, int parity, const VolStatus* status);
-void launch_ypass(int emax, bool inv, hipStream_t st, const VolDesc* d, const LaunchTable& lt,
-                  const float2* tw, int sel, int sel2, const VolStatus* status);
-void launch_zpass(int emax, int op, hipStream_t st, const VolDesc* d, const LaunchTable& lt,
-                  const float2* tw, const VolStatus* status);
-void launch_c2r_tv(int emax, hipStream_t st, const VolDesc* d, const LaunchTable& lt, const float2* tw,
-                   int parity, const VolStatus* status);
 void launch_tv_reduce(hipStream_t st, const VolDesc* d, int nvol, VolStatus* status, VolStatus* mirror);
 void launch_tv_output(hipStream_t st, const VolDesc* d, const LaunchTable& lt, const VolStatus* status);
 void launch_fill_zero(hipStream_t st, float* p, size_t n);
 void launch_status_reset(hipStream_t st, VolStatus* s, int nvol);
-void launch_vs_zfwd(int emax, hipStream_t st, const VolDesc* d, const LaunchTable& lt, const float2* tw,
-                    int sel);
-void launch_vs_zradius(int emax, hipStream_t st, const VolDesc* d, const LaunchTable& lt, const float2* tw,
-                       int ridx, int first);
-void launch_vs_zdeconv(int emax, hipStream_t st, const VolDesc* d, const LaunchTable& lt, const float2* tw,
-                       int divide);
-void launch_vs_select(hipStream_t st, const VolDesc* d, const LaunchTable& lt, const float2* tw, int ridx,
-                      double delta);
-void launch_vs_out(hipStream_t st, const VolDesc* d, const LaunchTable& lt, const float2* tw);
 void launch_vs_init(hipStream_t st, const VolDesc* d, const LaunchTable& lt);
-
-enum { SRC_F64_ = 0, SRC_U8_ = 1, SRC_F32_ = 2, SRC_SPHERE_ = 3, SRC_TVDIV_ = 4 };
-enum { ZOP_TV_SETUP_ = 0, ZOP_TV_SOLVE_ = 1 };
+bool fixed_length(int n);
 
 // ------------------------------------------------------------ error state --
 thread_local std::string g_err;
@@ -101,8 +82,7 @@ struct TwiddleBank {
 
 qsmfft::Plan make_plan(int n, TwiddleBank& tb) {
     qsmfft::Plan p{};
-    p.n = n;
-    qsmfft::factorize(n, p.radix, &p.nstages);
+    qsmfft::make_plan_stages(n, p);
     p.tw_off = tb.add(n);
     return p;
 }
@@ -162,6 +142,52 @@ PassBlocks fill_geometry(VolDesc& V, const qsm_grid& g, TwiddleBank& tb) {
     return pb;
 }
 
+// Volumes of a batch grouped by the length of one pass axis: one launch per
+// group, so each group runs the kernel compiled for its length.
+struct PassGroup {
+    int n;
+    int emax;
+    LaunchTable lt;
+};
+using Groups = std::vector<PassGroup>;
+
+Groups make_groups(const std::vector<int>& len, const std::vector<int>& blocks, bool column) {
+    Groups gs;
+    for (size_t v = 0; v < len.size(); ++v) {
+        PassGroup* g = nullptr;
+        for (auto& x : gs)
+            if (x.n == len[v]) g = &x;
+        if (!g) {
+            gs.push_back(PassGroup{});
+            g = &gs.back();
+            g->n = len[v];
+            g->emax = column ? emax_for_col(len[v]) : 16;
+            g->lt.nvol = 0;
+            g->lt.start[0] = 0;
+        }
+        LaunchTable& lt = g->lt;
+        lt.vol[lt.nvol] = (int)v;
+        lt.start[lt.nvol + 1] = lt.start[lt.nvol] + blocks[v];
+        ++lt.nvol;
+    }
+    return gs;
+}
+
+void run_groups(int kind, const Groups& gs, const PassArgs& a) {
+    for (const auto& g : gs) launch_pass(kind, g.n, g.emax, g.lt, a);
+}
+
+LaunchTable identity_table(const std::vector<int>& blocks) {
+    LaunchTable lt{};
+    lt.nvol = (int)blocks.size();
+    lt.start[0] = 0;
+    for (size_t v = 0; v < blocks.size(); ++v) {
+        lt.vol[v] = (int)v;
+        lt.start[v + 1] = lt.start[v] + blocks[v];
+    }
+    return lt;
+}
+
 }  // namespace qsm
 
 using namespace qsm;
@@ -176,12 +202,22 @@ struct qsm_tv_batch {
     VolStatus* h_status = nullptr;   // pinned mirror written by the reduce kernel
     VolStatus* h_status_dev = nullptr;
     float2* d_tw = nullptr;
-    LaunchTable lt_x{}, lt_y{}, lt_z{}, lt_out{};
-    int emax_x = 16, emax_y = 16, emax_z = 16;
+    Groups gx, gy, gz;              // x / y / z passes grouped by axis length
+    LaunchTable lt_out{};
     std::vector<void*> allocs;
     hipStream_t stream = nullptr;
     bool own_stream = false;
     static constexpr int kWindow = 4;
+    PassArgs args(const VolStatus* status) const {
+        PassArgs a{};
+        a.stream = stream;
+        a.vols = d_vols;
+        a.tw = d_tw;
+        a.status = status;
+        a.sel = SPEC_WORK;
+        a.sel2 = -1;
+        return a;
+    }
     hipEvent_t ev[kWindow] = {};
 };
 
@@ -217,8 +253,7 @@ static int tv_batch_create_impl(const qsm_tv_volume* vols, size_t n_vols, const 
     B->params = *params;
     B->h.resize(n_vols);
     TwiddleBank tb;
-    B->lt_x.nvol = B->lt_y.nvol = B->lt_z.nvol = B->lt_out.nvol = (int)n_vols;
-    int cx = 0, cy = 0, cz = 0, co = 0;
+    std::vector<int> lx, ly, lz, bx, by, bz, bo;
     auto cleanup = [&](int rc) { batch_free(B); return rc; };
     for (size_t v = 0; v < n_vols; ++v) {
         VolDesc& V = B->h[v];
@@ -233,13 +268,11 @@ static int tv_batch_create_impl(const qsm_tv_volume* vols, size_t n_vols, const 
         V.field = vols[v].d_field;
         V.mask = vols[v].d_mask;
         V.out = vols[v].d_chi;
-        B->lt_x.start[v] = cx; cx += pb.xpass;
-        B->lt_y.start[v] = cy; cy += pb.ypass;
-        B->lt_z.start[v] = cz; cz += pb.zpass;
-        B->lt_out.start[v] = co; co += pb.out;
+        lx.push_back(V.nx); bx.push_back(pb.xpass);
+        ly.push_back(V.ny); by.push_back(pb.ypass);
+        lz.push_back(V.nz); bz.push_back(pb.zpass);
+        bo.push_back(pb.out);
         V.npartial = pb.xpass;
-        B->emax_y = std::max(B->emax_y, emax_for_col(V.ny));
-        B->emax_z = std::max(B->emax_z, emax_for_col(V.nz));
         const size_t N = (size_t)V.nx * V.ny * V.nz;
         const size_t Nh = (size_t)V.nxh * V.ny * V.nz;
         int rc;
@@ -258,10 +291,10 @@ static int tv_batch_create_impl(const qsm_tv_volume* vols, size_t n_vols, const 
         if ((rc = dev_alloc(&V.partial, 2 * (size_t)pb.xpass))) return cleanup(rc);
         B->allocs.push_back(V.partial);
     }
-    B->lt_x.start[n_vols] = cx;
-    B->lt_y.start[n_vols] = cy;
-    B->lt_z.start[n_vols] = cz;
-    B->lt_out.start[n_vols] = co;
+    B->gx = make_groups(lx, bx, false);
+    B->gy = make_groups(ly, by, true);
+    B->gz = make_groups(lz, bz, true);
+    B->lt_out = identity_table(bo);
     int rc;
     if ((rc = dev_alloc(&B->d_tw, tb.host.size()))) return cleanup(rc);
     B->allocs.push_back(B->d_tw);
@@ -299,14 +332,32 @@ static int tv_batch_create_impl(const qsm_tv_volume* vols, size_t n_vols, const 
 
 // One TV-ADMM iteration (src/inversion/admm.rs:76-145) for every volume.
 static int enqueue_iteration(qsm_tv_batch* B, size_t it) {
-    const int parity = (int)((it - 1) & 1);
+    PassArgs a = B->args(B->d_status);
+    a.parity = (int)((it - 1) & 1);
+    run_groups(P_R2C_TVDIV, B->gx, a);
+    run_groups(P_Y_FWD, B->gy, a);
+    run_groups(P_Z_TV_SOLVE, B->gz, a);
+    run_groups(P_Y_INV, B->gy, a);
+    run_groups(P_C2R_TV, B->gx, a);
+    launch_tv_reduce(B->stream, B->d_vols, B->nvol, B->d_status, B->h_status_dev);
+    QSM_CHECK_LAUNCH();
+    return QSM_OK;
+}
+
+// x_0 = 0, u_{-1} = 0 (AdmmBuffers::new, src/inversion/admm.rs:39-52), then
+// f^ and inv_a (prepare_admm_spectral, src/inversion/admm.rs:150-178).
+static int enqueue_setup(qsm_tv_batch* B) {
     hipStream_t st = B->stream;
-    launch_r2c(16, SRC_TVDIV_, st, B->d_vols, B->lt_x, B->d_tw, SPEC_WORK, parity, B->d_status);
-    launch_ypass(B->emax_y, false, st, B->d_vols, B->lt_y, B->d_tw, SPEC_WORK, -1, B->d_status);
-    launch_zpass(B->emax_z, ZOP_TV_SOLVE_, st, B->d_vols, B->lt_z, B->d_tw, B->d_status);
-    launch_ypass(B->emax_y, true, st, B->d_vols, B->lt_y, B->d_tw, SPEC_WORK, -1, B->d_status);
-    launch_c2r_tv(16, st, B->d_vols, B->lt_x, B->d_tw, parity, B->d_status);
-    launch_tv_reduce(st, B->d_vols, B->nvol, B->d_status, B->h_status_dev);
+    for (auto& V : B->h) {
+        const size_t N = (size_t)V.nx * V.ny * V.nz;
+        launch_fill_zero(st, V.x[0], N);
+        launch_fill_zero(st, V.u[1], 3 * N);
+    }
+    launch_status_reset(st, B->d_status, B->nvol);
+    PassArgs a = B->args(nullptr);
+    run_groups(P_R2C_F64, B->gx, a);
+    run_groups(P_Y_FWD, B->gy, a);
+    run_groups(P_Z_TV_SETUP, B->gz, a);
     QSM_CHECK_LAUNCH();
     return QSM_OK;
 }
@@ -314,19 +365,11 @@ static int enqueue_iteration(qsm_tv_batch* B, size_t it) {
 static int tv_batch_run_impl(qsm_tv_batch* B, qsm_progress_fn progress, void* user) {
     if (!B) return fail(QSM_EPARAM, "batch is null");
     hipStream_t st = B->stream;
-    // Setup: x_0 = 0, u_{-1} = 0 (AdmmBuffers::new, src/inversion/admm.rs:39-52),
-    // then f^ and inv_a (prepare_admm_spectral, src/inversion/admm.rs:150-178).
-    for (auto& V : B->h) {
-        const size_t N = (size_t)V.nx * V.ny * V.nz;
-        launch_fill_zero(st, V.x[0], N);
-        launch_fill_zero(st, V.u[1], 3 * N);
-    }
-    launch_status_reset(st, B->d_status, B->nvol);
     std::memset(B->h_status, 0, B->nvol * sizeof(VolStatus));
-    launch_r2c(16, SRC_F64_, st, B->d_vols, B->lt_x, B->d_tw, SPEC_WORK, 0, nullptr);
-    launch_ypass(B->emax_y, false, st, B->d_vols, B->lt_y, B->d_tw, SPEC_WORK, -1, nullptr);
-    launch_zpass(B->emax_z, ZOP_TV_SETUP_, st, B->d_vols, B->lt_z, B->d_tw, nullptr);
-    QSM_CHECK_LAUNCH();
+    {
+        int rc = enqueue_setup(B);
+        if (rc) return rc;
+    }
 
     const size_t max_iter = B->params.max_iter;
     // relative_change >= 0 (or NaN), so with tol <= 0 no iteration can stop
@@ -379,31 +422,26 @@ static int tv_batch_run_impl(qsm_tv_batch* B, qsm_progress_fn progress, void* us
 static int tv_batch_profile_impl(qsm_tv_batch* B, size_t iters, double* ms_out, size_t n_out) {
     if (!B || !ms_out || n_out < 6 || iters == 0) return fail(QSM_EPARAM, "bad profile arguments");
     hipStream_t st = B->stream;
-    for (auto& V : B->h) {
-        const size_t N = (size_t)V.nx * V.ny * V.nz;
-        launch_fill_zero(st, V.x[0], N);
-        launch_fill_zero(st, V.u[1], 3 * N);
+    {
+        int rc = enqueue_setup(B);
+        if (rc) return rc;
     }
-    launch_status_reset(st, B->d_status, B->nvol);
-    launch_r2c(16, SRC_F64_, st, B->d_vols, B->lt_x, B->d_tw, SPEC_WORK, 0, nullptr);
-    launch_ypass(B->emax_y, false, st, B->d_vols, B->lt_y, B->d_tw, SPEC_WORK, -1, nullptr);
-    launch_zpass(B->emax_z, ZOP_TV_SETUP_, st, B->d_vols, B->lt_z, B->d_tw, nullptr);
-    QSM_CHECK_LAUNCH();
     hipEvent_t ev[7];
     for (auto& e : ev) QSM_HIP(hipEventCreate(&e));
     double acc[6] = {0, 0, 0, 0, 0, 0};
+    PassArgs a = B->args(nullptr);
     for (size_t it = 1; it <= iters; ++it) {
-        const int parity = (int)((it - 1) & 1);
+        a.parity = (int)((it - 1) & 1);
         QSM_HIP(hipEventRecord(ev[0], st));
-        launch_r2c(16, SRC_TVDIV_, st, B->d_vols, B->lt_x, B->d_tw, SPEC_WORK, parity, nullptr);
+        run_groups(P_R2C_TVDIV, B->gx, a);
         QSM_HIP(hipEventRecord(ev[1], st));
-        launch_ypass(B->emax_y, false, st, B->d_vols, B->lt_y, B->d_tw, SPEC_WORK, -1, nullptr);
+        run_groups(P_Y_FWD, B->gy, a);
         QSM_HIP(hipEventRecord(ev[2], st));
-        launch_zpass(B->emax_z, ZOP_TV_SOLVE_, st, B->d_vols, B->lt_z, B->d_tw, nullptr);
+        run_groups(P_Z_TV_SOLVE, B->gz, a);
         QSM_HIP(hipEventRecord(ev[3], st));
-        launch_ypass(B->emax_y, true, st, B->d_vols, B->lt_y, B->d_tw, SPEC_WORK, -1, nullptr);
+        run_groups(P_Y_INV, B->gy, a);
         QSM_HIP(hipEventRecord(ev[4], st));
-        launch_c2r_tv(16, st, B->d_vols, B->lt_x, B->d_tw, parity, nullptr);
+        run_groups(P_C2R_TV, B->gx, a);
         QSM_HIP(hipEventRecord(ev[5], st));
         launch_tv_reduce(st, B->d_vols, B->nvol, B->d_status, nullptr);
         QSM_HIP(hipEventRecord(ev[6], st));
@@ -458,11 +496,21 @@ struct qsm_vsharp_batch {
     std::vector<double> radii;  // processing order (descending)
     double threshold = 0.05;
     bool sharp = false;         // one radius: sharp_core semantics
-    LaunchTable lt_x{}, lt_rows{}, lt_y{}, lt_y2{}, lt_z{}, lt_init{};
-    int emax_y = 16, emax_z = 16;
+    Groups gx, grows, gy, gy2, gz;   // passes grouped by axis length
+    LaunchTable lt_init{};
     std::vector<void*> allocs;
     hipStream_t stream = nullptr;
     bool own_stream = false;
+    PassArgs args() const {
+        PassArgs a{};
+        a.stream = stream;
+        a.vols = d_vols;
+        a.tw = d_tw;
+        a.status = nullptr;
+        a.sel = SPEC_WORK;
+        a.sel2 = -1;
+        return a;
+    }
 };
 
 static void vs_free(qsm_vsharp_batch* b) {
@@ -498,9 +546,7 @@ static int vs_create_impl(const qsm_vsharp_volume* vols, size_t n_vols, const do
     const int R = (int)n_radii;
     B->h.resize(n_vols);
     TwiddleBank tb;
-    for (LaunchTable* lt : {&B->lt_x, &B->lt_rows, &B->lt_y, &B->lt_y2, &B->lt_z, &B->lt_init})
-        lt->nvol = (int)n_vols;
-    int cx = 0, cr = 0, cy = 0, cy2 = 0, cz = 0, ci = 0;
+    std::vector<int> lx, ly, lz, bx, brows, by, by2, bz, bi;
     int rc;
     if ((rc = dev_alloc(&B->d_radii, (size_t)R * (1 + n_vols)))) return cleanup(rc);
     B->allocs.push_back(B->d_radii);
@@ -519,14 +565,11 @@ static int vs_create_impl(const qsm_vsharp_volume* vols, size_t n_vols, const do
         V.vs_counts = B->d_radii + (size_t)R * (1 + v);
         for (int r = 0; r < R; ++r) host_tab[(size_t)R * (1 + v) + r] = sphere_count(vols[v].grid, B->radii[r]);
         const int nrows = V.ny * V.nz;
-        B->lt_x.start[v] = cx; cx += pb.xpass;
-        B->lt_rows.start[v] = cr; cr += (nrows + V.rows_x - 1) / V.rows_x;
-        B->lt_y.start[v] = cy; cy += pb.ypass;
-        B->lt_y2.start[v] = cy2; cy2 += 2 * pb.ypass;
-        B->lt_z.start[v] = cz; cz += pb.zpass;
-        B->lt_init.start[v] = ci; ci += pb.out;
-        B->emax_y = std::max(B->emax_y, emax_for_col(V.ny));
-        B->emax_z = std::max(B->emax_z, emax_for_col(V.nz));
+        lx.push_back(V.nx); bx.push_back(pb.xpass);
+        brows.push_back((nrows + V.rows_x - 1) / V.rows_x);
+        ly.push_back(V.ny); by.push_back(pb.ypass); by2.push_back(2 * pb.ypass);
+        lz.push_back(V.nz); bz.push_back(pb.zpass);
+        bi.push_back(pb.out);
         const size_t N = (size_t)V.nx * V.ny * V.nz;
         const size_t Nh = (size_t)V.nxh * V.ny * V.nz;
         for (int sidx = 0; sidx < SPEC_COUNT; ++sidx) {
@@ -540,12 +583,12 @@ static int vs_create_impl(const qsm_vsharp_volume* vols, size_t n_vols, const do
         if ((rc = dev_alloc(&V.vs_final, N))) return cleanup(rc);
         B->allocs.push_back(V.vs_final);
     }
-    B->lt_x.start[n_vols] = cx;
-    B->lt_rows.start[n_vols] = cr;
-    B->lt_y.start[n_vols] = cy;
-    B->lt_y2.start[n_vols] = cy2;
-    B->lt_z.start[n_vols] = cz;
-    B->lt_init.start[n_vols] = ci;
+    B->gx = make_groups(lx, bx, false);
+    B->grows = make_groups(lx, brows, false);
+    B->gy = make_groups(ly, by, true);
+    B->gy2 = make_groups(ly, by2, true);
+    B->gz = make_groups(lz, bz, true);
+    B->lt_init = identity_table(bi);
     if ((rc = dev_alloc(&B->d_tw, tb.host.size()))) return cleanup(rc);
     B->allocs.push_back(B->d_tw);
     if ((rc = dev_alloc(&B->d_vols, n_vols))) return cleanup(rc);
@@ -572,14 +615,19 @@ static int vs_run_impl(qsm_vsharp_batch* B, qsm_progress_fn progress, void* user
     const int R = (int)B->radii.size();
     const double delta = 1.0 - std::sqrt(1e-7);  // src/bgremove/vsharp.rs:129
     // F^ = FFT(field) (vsharp.rs:117-122), M^ = FFT(mask) once for all radii
-    launch_r2c(16, SRC_F64_, st, B->d_vols, B->lt_x, B->d_tw, SPEC_A, 0, nullptr);
-    launch_ypass(B->emax_y, false, st, B->d_vols, B->lt_y, B->d_tw, SPEC_A, -1, nullptr);
-    launch_vs_zfwd(B->emax_z, st, B->d_vols, B->lt_z, B->d_tw, SPEC_A);
-    launch_r2c(16, SRC_U8_, st, B->d_vols, B->lt_x, B->d_tw, SPEC_B, 0, nullptr);
-    launch_ypass(B->emax_y, false, st, B->d_vols, B->lt_y, B->d_tw, SPEC_B, -1, nullptr);
-    launch_vs_zfwd(B->emax_z, st, B->d_vols, B->lt_z, B->d_tw, SPEC_B);
+    PassArgs a = B->args();
+    a.sel = SPEC_A;
+    run_groups(P_R2C_F64, B->gx, a);
+    run_groups(P_Y_FWD, B->gy, a);
+    run_groups(P_Z_VS_FWD, B->gz, a);
+    a.sel = SPEC_B;
+    run_groups(P_R2C_U8, B->gx, a);
+    run_groups(P_Y_FWD, B->gy, a);
+    run_groups(P_Z_VS_FWD, B->gz, a);
     launch_vs_init(st, B->d_vols, B->lt_init);
     QSM_CHECK_LAUNCH();
+    a.sel = SPEC_WORK;
+    a.delta = delta;
     for (int r = 0; r < R; ++r) {
         if (progress) {
             // report radius r+1 once radius r has finished on the device
@@ -587,19 +635,25 @@ static int vs_run_impl(qsm_vsharp_batch* B, qsm_progress_fn progress, void* user
             if (B->sharp) progress(1, 1, user);           // vsharp.rs:103-108
             else progress((size_t)r + 1, (size_t)R, user); // vsharp.rs:134
         }
-        launch_r2c(16, SRC_SPHERE_, st, B->d_vols, B->lt_x, B->d_tw, SPEC_WORK, r, nullptr);
-        launch_ypass(B->emax_y, false, st, B->d_vols, B->lt_y, B->d_tw, SPEC_WORK, -1, nullptr);
-        launch_vs_zradius(B->emax_z, st, B->d_vols, B->lt_z, B->d_tw, r, r == 0 ? 1 : 0);
-        launch_ypass(B->emax_y, true, st, B->d_vols, B->lt_y2, B->d_tw, SPEC_WORK, SPEC_C, nullptr);
-        launch_vs_select(st, B->d_vols, B->lt_rows, B->d_tw, r, delta);
+        a.ridx = r;
+        a.first = (r == 0) ? 1 : 0;
+        a.sel2 = -1;
+        run_groups(P_R2C_SPHERE, B->gx, a);
+        run_groups(P_Y_FWD, B->gy, a);
+        run_groups(P_Z_VS_RADIUS, B->gz, a);
+        a.sel2 = SPEC_C;
+        run_groups(P_Y_INV, B->gy2, a);
+        a.sel2 = -1;
+        run_groups(P_C2R_VS_SELECT, B->grows, a);
         QSM_CHECK_LAUNCH();
     }
     // deconvolution with the largest kernel (vsharp.rs:180-196 / sharp.rs:105-131)
-    launch_r2c(16, SRC_F32_, st, B->d_vols, B->lt_x, B->d_tw, SPEC_WORK, 0, nullptr);
-    launch_ypass(B->emax_y, false, st, B->d_vols, B->lt_y, B->d_tw, SPEC_WORK, -1, nullptr);
-    launch_vs_zdeconv(B->emax_z, st, B->d_vols, B->lt_z, B->d_tw, B->sharp ? 1 : 0);
-    launch_ypass(B->emax_y, true, st, B->d_vols, B->lt_y, B->d_tw, SPEC_WORK, -1, nullptr);
-    launch_vs_out(st, B->d_vols, B->lt_x, B->d_tw);
+    a.divide = B->sharp ? 1 : 0;
+    run_groups(P_R2C_F32, B->gx, a);
+    run_groups(P_Y_FWD, B->gy, a);
+    run_groups(P_Z_VS_DECONV, B->gz, a);
+    run_groups(P_Y_INV, B->gy, a);
+    run_groups(P_C2R_VS_OUT, B->gx, a);
     QSM_CHECK_LAUNCH();
     QSM_HIP(hipStreamSynchronize(st));
     return QSM_OK;

@@ -1,0 +1,232 @@
+// C++ mirror of qsm-core's TV-ADMM / V-SHARP API over the gfx950 C ABI
+// (include/qsm_gpu.h).  Names, argument order and meaning follow the Rust
+// crate so that call sites read the same:
+//
+//   qsm::Grid                       Grid                  src/grid.rs:8-44
+//   qsm::inversion::TvParams        inversion::TvParams   src/inversion/tv.rs:21-43
+//   qsm::inversion::tv_admm         inversion::tv_admm    src/inversion/tv.rs:63-97
+//   qsm::bgremove::VsharpParams     bgremove::VsharpParams src/bgremove/vsharp.rs:20-39
+//   qsm::bgremove::vsharp           bgremove::vsharp      src/bgremove/vsharp.rs:55-64
+//   qsm::pipeline::run_dipole_inversion / run_bg_removal  src/pipeline/*.rs
+//
+// Algorithm functions throw qsm::Error on a non-zero status (the Rust
+// functions panic on bad slices); pipeline functions return a Result-like
+// struct carrying a PipelineError, as the Rust pipeline does.
+#pragma once
+
+#include <cstddef>
+#include <cstdint>
+#include <functional>
+#include <stdexcept>
+#include <string>
+#include <tuple>
+#include <utility>
+#include <vector>
+
+#include "../../include/qsm_gpu.h"
+
+namespace qsm {
+
+struct Error : std::runtime_error {
+    int code;
+    Error(int c, const std::string& m) : std::runtime_error(m), code(c) {}
+};
+
+inline void check(int rc) {
+    if (rc != QSM_OK) throw Error(rc, qsm_last_error_message());
+}
+
+using Progress = std::function<void(size_t, size_t)>;
+
+namespace detail {
+inline void trampoline(size_t cur, size_t total, void* user) {
+    (*static_cast<Progress*>(user))(cur, total);
+}
+}  // namespace detail
+
+struct Grid {
+    std::tuple<size_t, size_t, size_t> dims;
+    std::tuple<double, double, double> voxel_size;
+    static Grid make(size_t nx, size_t ny, size_t nz, double vsx, double vsy, double vsz) {
+        return Grid{{nx, ny, nz}, {vsx, vsy, vsz}};
+    }
+    size_t nx() const { return std::get<0>(dims); }
+    size_t ny() const { return std::get<1>(dims); }
+    size_t nz() const { return std::get<2>(dims); }
+    double vsx() const { return std::get<0>(voxel_size); }
+    double vsy() const { return std::get<1>(voxel_size); }
+    double vsz() const { return std::get<2>(voxel_size); }
+    size_t n_total() const { return nx() * ny() * nz(); }
+    qsm_grid c() const { return qsm_grid{nx(), ny(), nz(), vsx(), vsy(), vsz()}; }
+};
+
+namespace inversion {
+
+struct TvParams {
+    double lambda, rho, tol;
+    size_t max_iter;
+    static TvParams default_params() {
+        qsm_tv_params p;
+        qsm_tv_params_default(&p);
+        return TvParams{p.lambda, p.rho, p.tol, p.max_iter};
+    }
+    qsm_tv_params c() const { return qsm_tv_params{lambda, rho, tol, max_iter}; }
+};
+
+inline std::vector<double> tv_admm(const std::vector<double>& local_field,
+                                   const std::vector<uint8_t>& mask, const Grid& grid,
+                                   std::tuple<double, double, double> bdir, const TvParams& params,
+                                   Progress progress = nullptr) {
+    if (local_field.size() != grid.n_total() || mask.size() != grid.n_total())
+        throw Error(QSM_EDIM, "array length does not match the grid");
+    std::vector<double> chi(grid.n_total());
+    const qsm_grid g = grid.c();
+    const double b[3] = {std::get<0>(bdir), std::get<1>(bdir), std::get<2>(bdir)};
+    const qsm_tv_params p = params.c();
+    check(qsm_tv_admm(local_field.data(), mask.data(), &g, b, &p,
+                      progress ? detail::trampoline : nullptr, progress ? &progress : nullptr,
+                      chi.data()));
+    return chi;
+}
+
+}  // namespace inversion
+
+namespace bgremove {
+
+struct VsharpParams {
+    double threshold, max_radius, min_radius;
+    static VsharpParams default_params() {
+        qsm_vsharp_params p;
+        qsm_vsharp_params_default(&p);
+        return VsharpParams{p.threshold, p.max_radius, p.min_radius};
+    }
+    qsm_vsharp_params c() const { return qsm_vsharp_params{threshold, max_radius, min_radius}; }
+};
+
+inline std::pair<std::vector<double>, std::vector<uint8_t>> vsharp(
+    const std::vector<double>& field, const std::vector<uint8_t>& mask, const Grid& grid,
+    const VsharpParams& params, Progress progress = nullptr) {
+    if (field.size() != grid.n_total() || mask.size() != grid.n_total())
+        throw Error(QSM_EDIM, "array length does not match the grid");
+    std::vector<double> local(grid.n_total());
+    std::vector<uint8_t> eroded(grid.n_total());
+    const qsm_grid g = grid.c();
+    const qsm_vsharp_params p = params.c();
+    check(qsm_vsharp(field.data(), mask.data(), &g, &p, progress ? detail::trampoline : nullptr,
+                     progress ? &progress : nullptr, local.data(), eroded.data()));
+    return {std::move(local), std::move(eroded)};
+}
+
+inline std::pair<std::vector<double>, std::vector<uint8_t>> vsharp_with_radii(
+    const std::vector<double>& field, const std::vector<uint8_t>& mask, const Grid& grid,
+    const std::vector<double>& radii, double threshold, Progress progress = nullptr) {
+    if (field.size() != grid.n_total() || mask.size() != grid.n_total())
+        throw Error(QSM_EDIM, "array length does not match the grid");
+    std::vector<double> local(grid.n_total());
+    std::vector<uint8_t> eroded(grid.n_total());
+    const qsm_grid g = grid.c();
+    check(qsm_vsharp_with_radii(field.data(), mask.data(), &g, radii.data(), radii.size(),
+                                threshold, progress ? detail::trampoline : nullptr,
+                                progress ? &progress : nullptr, local.data(), eroded.data()));
+    return {std::move(local), std::move(eroded)};
+}
+
+}  // namespace bgremove
+
+namespace pipeline {
+
+enum class PipelineErrorKind { InvalidConfig, InvalidInput, AlgorithmError, DimensionMismatch };
+
+struct PipelineError {
+    PipelineErrorKind kind;
+    std::string message;
+};
+
+template <class T>
+struct Result {
+    bool ok;
+    T value;
+    PipelineError error;
+};
+
+struct ScanMetadata {
+    std::tuple<size_t, size_t, size_t> dims;
+    std::tuple<double, double, double> voxel_size;
+    std::vector<double> echo_times;
+    double field_strength;
+    std::tuple<double, double, double> b0_direction;
+    Grid grid() const { return Grid{dims, voxel_size}; }
+    qsm_scan_metadata c() const {
+        qsm_scan_metadata m;
+        m.nx = std::get<0>(dims); m.ny = std::get<1>(dims); m.nz = std::get<2>(dims);
+        m.vsx = std::get<0>(voxel_size); m.vsy = std::get<1>(voxel_size); m.vsz = std::get<2>(voxel_size);
+        m.echo_times = echo_times.data();
+        m.n_echo_times = echo_times.size();
+        m.field_strength = field_strength;
+        m.b0_direction[0] = std::get<0>(b0_direction);
+        m.b0_direction[1] = std::get<1>(b0_direction);
+        m.b0_direction[2] = std::get<2>(b0_direction);
+        return m;
+    }
+};
+
+struct InversionConfig {
+    qsm_inversion_algorithm algorithm = QSM_INV_RTS;
+    inversion::TvParams tv = inversion::TvParams::default_params();
+};
+
+struct BgRemovalConfig {
+    qsm_bg_removal_algorithm algorithm = QSM_BG_VSHARP;
+    bgremove::VsharpParams vsharp = bgremove::VsharpParams::default_params();
+};
+
+struct BgRemovalResult {
+    std::vector<double> local_field_ppm;
+    std::vector<uint8_t> eroded_mask;
+};
+
+inline PipelineError to_error(int rc) {
+    PipelineErrorKind k = rc == QSM_EDIM ? PipelineErrorKind::DimensionMismatch
+                        : rc == QSM_ECONFIG ? PipelineErrorKind::InvalidConfig
+                        : rc == QSM_EPARAM ? PipelineErrorKind::InvalidInput
+                                           : PipelineErrorKind::AlgorithmError;
+    return PipelineError{k, qsm_last_error_message()};
+}
+
+inline Result<std::vector<double>> run_dipole_inversion(const std::vector<double>& local_field_ppm,
+                                                        const std::vector<uint8_t>& mask,
+                                                        const ScanMetadata& md,
+                                                        const InversionConfig& config,
+                                                        Progress progress = nullptr) {
+    Result<std::vector<double>> r{false, std::vector<double>(local_field_ppm.size()), {}};
+    const qsm_scan_metadata m = md.c();
+    const qsm_tv_params p = config.tv.c();
+    const int rc = qsm_run_dipole_inversion(
+        local_field_ppm.data(), local_field_ppm.size(), mask.data(), mask.size(), &m,
+        config.algorithm, &p, progress ? detail::trampoline : nullptr,
+        progress ? &progress : nullptr, r.value.data());
+    if (rc != QSM_OK) { r.error = to_error(rc); r.value.clear(); return r; }
+    r.ok = true;
+    return r;
+}
+
+inline Result<BgRemovalResult> run_bg_removal(const std::vector<double>& field_ppm,
+                                              const std::vector<uint8_t>& mask,
+                                              const ScanMetadata& md, const BgRemovalConfig& config,
+                                              Progress progress = nullptr) {
+    Result<BgRemovalResult> r{false, {std::vector<double>(field_ppm.size()),
+                                      std::vector<uint8_t>(field_ppm.size())}, {}};
+    const qsm_scan_metadata m = md.c();
+    const qsm_vsharp_params p = config.vsharp.c();
+    const int rc = qsm_run_bg_removal(field_ppm.data(), field_ppm.size(), mask.data(), mask.size(),
+                                      &m, config.algorithm, &p,
+                                      progress ? detail::trampoline : nullptr,
+                                      progress ? &progress : nullptr,
+                                      r.value.local_field_ppm.data(), r.value.eroded_mask.data());
+    if (rc != QSM_OK) { r.error = to_error(rc); r.value = {}; return r; }
+    r.ok = true;
+    return r;
+}
+
+}  // namespace pipeline
+}  // namespace qsm

@@ -6,12 +6,13 @@
 // directions, and compares against a direct O(n^2) DFT in double precision.
 // Built with hipcc as host code; prints one line per case and exits non-zero
 // on any mismatch.
-#include "../../qsm.rs_amd/csrc/fft_core.hpp"
+#include "../../qsm.rs_amd/csrc/fft_fixed.hpp"
 
 #include <cmath>
 #include <complex>
 #include <cstdio>
 #include <cstdlib>
+#include <array>
 #include <vector>
 
 using namespace qsmfft;
@@ -37,10 +38,9 @@ static void emu_stage(std::vector<float2>& lds, const StageGeom& g, const float2
 template <int EMAX, bool COLFAST, int LOG2C, bool INV>
 static void emu_fft(std::vector<float2>& lds, const Plan& P, int ncols, int es, int cs,
                     const float2* tw) {
-    int Ns = 1;
     for (int s = 0; s < P.nstages; ++s) {
         int R = P.radix[s];
-        StageGeom g = make_geom(P.n, ncols, es, cs, R, Ns);
+        StageGeom g = plan_geom(P, s, ncols, es, cs);
         switch (R) {
             case 2: emu_stage<2, EMAX, COLFAST, LOG2C, INV>(lds, g, tw); break;
             case 3: emu_stage<3, EMAX, COLFAST, LOG2C, INV>(lds, g, tw); break;
@@ -52,15 +52,13 @@ static void emu_fft(std::vector<float2>& lds, const Plan& P, int ncols, int es, 
             case 13: emu_stage<13, EMAX, COLFAST, LOG2C, INV>(lds, g, tw); break;
             default: std::abort();
         }
-        Ns *= R;
     }
 }
 
 template <int EMAX, bool COLFAST, int LOG2C, bool INV>
 static double run_case(int n, int ncols) {
     Plan P{};
-    P.n = n;
-    if (!factorize(n, P.radix, &P.nstages)) return -1.0;
+    if (!make_plan_stages(n, P)) return -1.0;
     std::vector<float2> tw(n);
     for (int q = 0; q < n; ++q) {
         double a = -2.0 * M_PI * q / n;
@@ -92,7 +90,78 @@ static double run_case(int n, int ncols) {
     return maxerr / maxref;
 }
 
+// ---- compile-time-length path (fft_fixed.hpp) ----
+template <int N, int S, int NCOLS, bool COLFAST, bool INV>
+static void emu_fixed_stages(std::vector<float2>& lds, const float2* tw) {
+    if constexpr (S < FixedSeq<N>::count) {
+        using G = FixedStage<N, S, NCOLS, COLFAST>;
+        std::vector<std::array<std::array<float2, G::R>, G::B>> regs(kThreads);
+        for (int tid = 0; tid < kThreads; ++tid) {
+            float2 v[G::B][G::R];
+            fixed_stage_load<N, S, NCOLS, COLFAST>(lds.data(), tid, v);
+            for (int t = 0; t < G::B; ++t)
+                for (int r = 0; r < G::R; ++r) regs[tid][t][r] = v[t][r];
+        }
+        for (int tid = 0; tid < kThreads; ++tid) {
+            float2 v[G::B][G::R];
+            for (int t = 0; t < G::B; ++t)
+                for (int r = 0; r < G::R; ++r) v[t][r] = regs[tid][t][r];
+            fixed_stage_store<N, S, NCOLS, COLFAST, INV>(lds.data(), tid, v, tw);
+        }
+        emu_fixed_stages<N, S + 1, NCOLS, COLFAST, INV>(lds, tw);
+    }
+}
+
+template <int N, int NCOLS, bool COLFAST, bool INV>
+static double run_fixed(void) {
+    std::vector<float2> tw(N);
+    for (int q = 0; q < N; ++q) {
+        double a = -2.0 * M_PI * q / N;
+        tw[q] = make_float2((float)std::cos(a), (float)std::sin(a));
+    }
+    const int es = COLFAST ? NCOLS : 1, cs = COLFAST ? 1 : N;
+    std::vector<float2> lds(N * NCOLS);
+    std::vector<std::complex<double>> in(N * NCOLS);
+    srand(N * 31 + NCOLS);
+    for (int c = 0; c < NCOLS; ++c)
+        for (int e = 0; e < N; ++e) {
+            double re = rand() / (double)RAND_MAX - 0.5, im = rand() / (double)RAND_MAX - 0.5;
+            in[c * N + e] = {re, im};
+            lds[e * es + c * cs] = make_float2((float)re, (float)im);
+        }
+    emu_fixed_stages<N, 0, NCOLS, COLFAST, INV>(lds, tw.data());
+    double maxerr = 0.0, maxref = 0.0;
+    const double sgn = INV ? 1.0 : -1.0;
+    for (int c = 0; c < NCOLS; ++c)
+        for (int k = 0; k < N; ++k) {
+            std::complex<double> acc = 0.0;
+            for (int e = 0; e < N; ++e)
+                acc += in[c * N + e] * std::polar(1.0, sgn * 2.0 * M_PI * ((long)k * e % N) / N);
+            float2 g = lds[k * es + c * cs];
+            maxerr = std::fmax(maxerr, std::abs(acc - std::complex<double>(g.x, g.y)));
+            maxref = std::fmax(maxref, std::abs(acc));
+        }
+    return maxerr / maxref;
+}
+
+template <int N>
+static int check_fixed() {
+    double e1 = run_fixed<N, 16, true, false>();
+    double e2 = run_fixed<N, 16, true, true>();
+    double e3 = run_fixed<N, fixed_rows(N), false, false>();
+    double e4 = run_fixed<N, fixed_rows(N), false, true>();
+    double worst = std::fmax(std::fmax(e1, e2), std::fmax(e3, e4));
+    bool ok = worst < 2e-6;
+    printf("fixed n=%4d stages=%d colfwd=%.2e colinv=%.2e rowfwd=%.2e rowinv=%.2e %s\n", N,
+           FixedSeq<N>::count, e1, e2, e3, e4, ok ? "ok" : "FAIL");
+    return ok ? 0 : 1;
+}
+
 int main() {
+    int bad_fixed = check_fixed<16>() + check_fixed<64>() + check_fixed<128>() + check_fixed<144>() +
+                    check_fixed<160>() + check_fixed<176>() + check_fixed<192>() +
+                    check_fixed<208>() + check_fixed<224>() + check_fixed<240>() +
+                    check_fixed<256>() + check_fixed<384>();
     const int lens[] = {1, 2, 3, 4, 5, 6, 7, 8, 10, 11, 12, 13, 14, 16, 22, 26, 32, 64,
                         72, 80, 88, 96, 104, 112, 120, 128, 144, 160, 176, 192, 208, 224,
                         240, 256, 384};
@@ -111,6 +180,7 @@ int main() {
                ok ? "ok" : "FAIL");
         if (!ok) ++bad;
     }
+    bad += bad_fixed;
     printf("%s\n", bad ? "FAILED" : "ALL OK");
     return bad ? 1 : 0;
 }

@@ -1,0 +1,211 @@
+// The reference's unit tests for the TV-ADMM / V-SHARP path, restated in C++
+// against the GPU backend through the host mirror qsm.rs_amd/host/qsm.hpp.
+// Each test names the Rust test it follows.  Built by __graft_entry__.build()
+// with g++ against qsm.rs_amd/libqsm_gpu.so; exit status 0 when every test
+// passes.
+#include "../../qsm.rs_amd/host/qsm.hpp"
+
+#include <cmath>
+#include <cstdio>
+#include <vector>
+
+using namespace qsm;
+
+static int g_fail = 0;
+#define EXPECT(cond, msg)                                  \
+    do {                                                   \
+        if (!(cond)) {                                     \
+            std::printf("  FAIL %s: %s\n", __func__, msg); \
+            ++g_fail;                                      \
+            return;                                        \
+        }                                                  \
+    } while (0)
+
+static std::vector<double> ramp(size_t n, double s) {
+    std::vector<double> v(n);
+    for (size_t i = 0; i < n; ++i) v[i] = (double)i * s;
+    return v;
+}
+
+// src/inversion/tv.rs:113-126 test_tv_admm_zero_field
+static void test_tv_admm_zero_field() {
+    const size_t n = 8;
+    Grid g = Grid::make(n, n, n, 1, 1, 1);
+    inversion::TvParams p{1e-3, 0.1, 1e-2, 10};
+    auto chi = inversion::tv_admm(std::vector<double>(n * n * n, 0.0),
+                                  std::vector<uint8_t>(n * n * n, 1), g, {0, 0, 1}, p);
+    for (double v : chi) EXPECT(std::fabs(v) < 1e-8, "zero field should give zero chi");
+}
+
+// src/inversion/tv.rs:129-142 test_tv_admm_finite
+static void test_tv_admm_finite() {
+    const size_t n = 8;
+    Grid g = Grid::make(n, n, n, 1, 1, 1);
+    inversion::TvParams p{1e-3, 0.1, 1e-2, 10};
+    auto chi = inversion::tv_admm(ramp(n * n * n, 0.001), std::vector<uint8_t>(n * n * n, 1), g,
+                                  {0, 0, 1}, p);
+    for (double v : chi) EXPECT(std::isfinite(v), "chi should be finite");
+}
+
+// src/inversion/tv.rs:144-168 test_tv_admm_smoother_than_tkd (TV finite)
+static void test_tv_admm_smoother_than_tkd() {
+    const size_t n = 8;
+    std::vector<double> f(n * n * n);
+    for (size_t i = 0; i < f.size(); ++i) f[i] = (i % 2 == 0) ? 0.01 : -0.01;
+    Grid g = Grid::make(n, n, n, 1, 1, 1);
+    inversion::TvParams p{1e-2, 1.0, 1e-2, 50};
+    auto chi = inversion::tv_admm(f, std::vector<uint8_t>(n * n * n, 1), g, {0, 0, 1}, p);
+    double tv = 0;
+    for (size_t k = 0; k < n; ++k)
+        for (size_t j = 0; j < n; ++j)
+            for (size_t i = 0; i < n; ++i) {
+                size_t idx = i + j * n + k * n * n;
+                tv += std::fabs(chi[(i + 1) % n + j * n + k * n * n] - chi[idx]) +
+                      std::fabs(chi[i + ((j + 1) % n) * n + k * n * n] - chi[idx]) +
+                      std::fabs(chi[i + j * n + ((k + 1) % n) * n * n] - chi[idx]);
+            }
+    EXPECT(std::isfinite(tv), "TV should be finite");
+}
+
+// src/inversion/tv.rs:34-43 TvParams::default
+static void test_tv_params_default() {
+    auto p = inversion::TvParams::default_params();
+    EXPECT(p.lambda == 2e-4 && p.rho == 2e-2 && p.tol == 1e-3 && p.max_iter == 250, "defaults");
+}
+
+// src/bgremove/vsharp.rs:206-219 test_vsharp_zero_field
+static void test_vsharp_zero_field() {
+    const size_t n = 8;
+    Grid g = Grid::make(n, n, n, 1, 1, 1);
+    auto r = bgremove::vsharp_with_radii(std::vector<double>(n * n * n, 0.0),
+                                         std::vector<uint8_t>(n * n * n, 1), g, {4, 3, 2}, 0.05);
+    for (double v : r.first) EXPECT(std::fabs(v) < 1e-10, "zero field -> zero local field");
+}
+
+// src/bgremove/vsharp.rs:247-265 test_vsharp_nonuniform_voxels
+static void test_vsharp_nonuniform_voxels() {
+    const size_t n = 8;
+    Grid g = Grid::make(n, n, n, 0.5, 1.0, 2.0);
+    auto r = bgremove::vsharp_with_radii(ramp(n * n * n, 0.001), std::vector<uint8_t>(n * n * n, 1),
+                                         g, {4, 3, 2}, 0.05);
+    size_t cnt = 0;
+    for (size_t i = 0; i < r.first.size(); ++i) {
+        EXPECT(std::isfinite(r.first[i]), "finite");
+        cnt += r.second[i];
+    }
+    EXPECT(cnt > 0, "final mask should have some voxels");
+}
+
+// src/bgremove/vsharp.rs:302-318 test_vsharp_empty_radii
+static void test_vsharp_empty_radii() {
+    const size_t n = 8;
+    Grid g = Grid::make(n, n, n, 1, 1, 1);
+    std::vector<uint8_t> mask(n * n * n, 1);
+    auto r = bgremove::vsharp_with_radii(ramp(n * n * n, 0.001), mask, g, {}, 0.05);
+    for (double v : r.first) EXPECT(v == 0.0, "empty radii -> zeros");
+    EXPECT(r.second == mask, "empty radii -> original mask");
+}
+
+// src/bgremove/vsharp.rs:321-366 test_vsharp_larger_volume
+static void test_vsharp_larger_volume() {
+    const size_t n = 16;
+    std::vector<double> field(n * n * n);
+    std::vector<uint8_t> mask(n * n * n, 0);
+    const int c = (int)n / 2, rad = (int)n / 3;
+    for (size_t z = 0; z < n; ++z)
+        for (size_t y = 0; y < n; ++y)
+            for (size_t x = 0; x < n; ++x) {
+                field[x + y * n + z * n * n] = (double)z * 0.1;
+                int dx = (int)x - c, dy = (int)y - c, dz = (int)z - c;
+                if (dx * dx + dy * dy + dz * dz <= rad * rad) mask[x + y * n + z * n * n] = 1;
+            }
+    Grid g = Grid::make(n, n, n, 1, 1, 1);
+    auto r = bgremove::vsharp_with_radii(field, mask, g, {6, 4, 3, 2}, 0.05);
+    size_t cnt = 0;
+    for (size_t i = 0; i < field.size(); ++i) {
+        EXPECT(std::isfinite(r.first[i]), "finite");
+        if (r.second[i] == 0) EXPECT(r.first[i] == 0.0, "outside the final mask is zero");
+        cnt += r.second[i];
+    }
+    EXPECT(cnt > 0, "final mask non-empty");
+}
+
+// src/bgremove/vsharp.rs:369-391 test_vsharp_with_progress
+static void test_vsharp_with_progress() {
+    const size_t n = 8;
+    Grid g = Grid::make(n, n, n, 1, 1, 1);
+    std::vector<std::pair<size_t, size_t>> calls;
+    bgremove::vsharp_with_radii(ramp(n * n * n, 0.001), std::vector<uint8_t>(n * n * n, 1), g,
+                                {4, 3, 2}, 0.05,
+                                [&](size_t a, size_t b) { calls.emplace_back(a, b); });
+    const bool ok = calls.size() == 3 && calls[0].first == 1 && calls[0].second == 3 &&
+                    calls[1].first == 2 && calls[2].first == 3 && calls[2].second == 3;
+    EXPECT(ok, "progress (1,3),(2,3),(3,3)");
+}
+
+// src/bgremove/vsharp.rs:431-459 test_vsharp_unsorted_radii
+static void test_vsharp_unsorted_radii() {
+    const size_t n = 8;
+    Grid g = Grid::make(n, n, n, 1, 1, 1);
+    auto f = ramp(n * n * n, 0.001);
+    std::vector<uint8_t> m(n * n * n, 1);
+    auto a = bgremove::vsharp_with_radii(f, m, g, {4, 3, 2}, 0.05);
+    auto b = bgremove::vsharp_with_radii(f, m, g, {2, 4, 3}, 0.05);
+    EXPECT(a.second == b.second, "same mask");
+    for (size_t i = 0; i < f.size(); ++i) EXPECT(std::fabs(a.first[i] - b.first[i]) < 1e-10, "same field");
+}
+
+// src/pipeline/bg_removal.rs:99-120 test_bg_removal_dispatches_vsharp
+static void test_bg_removal_dispatches_vsharp() {
+    pipeline::ScanMetadata md{{8, 8, 8}, {1, 1, 1}, {0.005}, 3.0, {0, 0, 1}};
+    auto r = pipeline::run_bg_removal(std::vector<double>(512, 0.1), std::vector<uint8_t>(512, 1), md,
+                                      pipeline::BgRemovalConfig{});
+    EXPECT(r.ok && r.value.local_field_ppm.size() == 512 && r.value.eroded_mask.size() == 512,
+           "dispatch to vsharp");
+}
+
+// src/pipeline/bg_removal.rs:198-214 test_bg_removal_validates_dims
+static void test_bg_removal_validates_dims() {
+    pipeline::ScanMetadata md{{4, 4, 4}, {1, 1, 1}, {0.005}, 3.0, {0, 0, 1}};
+    auto r = pipeline::run_bg_removal(std::vector<double>(32, 0.0), std::vector<uint8_t>(64, 1), md,
+                                      pipeline::BgRemovalConfig{});
+    EXPECT(!r.ok && r.error.kind == pipeline::PipelineErrorKind::DimensionMismatch,
+           "wrong length -> DimensionMismatch");
+}
+
+// src/pipeline/inversion.rs:214-217 test_inversion_tv, 254-272 test_inversion_rejects_tgv
+static void test_inversion_tv_and_rejects_tgv() {
+    pipeline::ScanMetadata md{{8, 8, 8}, {1, 1, 1}, {0.005}, 3.0, {0, 0, 1}};
+    pipeline::InversionConfig cfg;
+    cfg.algorithm = QSM_INV_TV;
+    auto r = pipeline::run_dipole_inversion(std::vector<double>(512, 0.01),
+                                            std::vector<uint8_t>(512, 1), md, cfg);
+    EXPECT(r.ok && r.value.size() == 512, "Tv arm runs");
+    pipeline::ScanMetadata md4{{4, 4, 4}, {1, 1, 1}, {0.005}, 3.0, {0, 0, 1}};
+    cfg.algorithm = QSM_INV_TGV;
+    auto t = pipeline::run_dipole_inversion(std::vector<double>(64, 0.0), std::vector<uint8_t>(64, 1),
+                                            md4, cfg);
+    EXPECT(!t.ok && t.error.kind == pipeline::PipelineErrorKind::InvalidConfig, "Tgv rejected");
+}
+
+int main() {
+    void (*tests[])() = {test_tv_admm_zero_field, test_tv_admm_finite, test_tv_admm_smoother_than_tkd,
+                         test_tv_params_default, test_vsharp_zero_field, test_vsharp_nonuniform_voxels,
+                         test_vsharp_empty_radii, test_vsharp_larger_volume, test_vsharp_with_progress,
+                         test_vsharp_unsorted_radii, test_bg_removal_dispatches_vsharp,
+                         test_bg_removal_validates_dims, test_inversion_tv_and_rejects_tgv};
+    int i = 0;
+    for (auto t : tests) {
+        const int before = g_fail;
+        try {
+            t();
+        } catch (const std::exception& e) {
+            std::printf("  EXCEPTION in test %d: %s\n", i, e.what());
+            ++g_fail;
+        }
+        std::printf("test %2d %s\n", i, g_fail == before ? "ok" : "FAILED");
+        ++i;
+    }
+    std::printf("%s (%d failures)\n", g_fail ? "FAILED" : "ALL OK", g_fail);
+    return g_fail ? 1 : 0;
+}
