@@ -34,8 +34,8 @@ PKG_DIR = os.path.join(ROOT, "qsm.rs_amd")
 METRIC = "TV-ADMM solves/s at 256²×160: FP32+batched vs plain, NRMSE≤1e-3 vs f64"
 HBM_PEAK_GBS = 8000.0
 
-KERNELS = ["r2c_x+div (z/u update, bdiv, R2C x)", "c2c_y fwd", "c2c_z + x-update + inverse z",
-           "c2c_y inv", "c2r_x + relchange sums", "relchange reduce"]
+KERNELS = ["r2c_x+div (previous relchange, z/u update, bdiv, R2C x)", "c2c_y fwd",
+           "c2c_z + x-update + inverse z", "c2c_y inv", "c2r_x + relchange sums"]
 
 
 def _load(name, path, search=None):
@@ -60,8 +60,24 @@ def alg_bytes(dims):
         28 * Nh,           # spectrum r/w (16Nh) + f^ (8Nh) + rho*inv_a/N (4Nh)
         16 * Nh,
         8 * Nh + 8 * N,    # read spectrum + x_prev, write x
-        0,
     ]
+
+
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "pmc_traffic_tv_256x256x160.txt")
+
+
+def traffic_for(kernel_index, nvol, dims):
+    """Measured HBM bytes per launch of one iteration kernel, from the
+    committed rocprofv3 PMC summary (256x256x160, one volume; scaled by the
+    batch)."""
+    names = ["k_r2c_x (TVDIV)", "k_y_pass", "k_z_tv (SOLVE)", "k_y_pass", "k_c2r_tv"]
+    if tuple(dims) != (256, 256, 160) or kernel_index >= len(names) or not os.path.exists(TRAFFIC_FILE):
+        return None
+    for line in open(TRAFFIC_FILE):
+        if line.startswith(names[kernel_index]):
+            hbm_mb = float(line[len(names[kernel_index]):].split()[3])
+            return int(hbm_mb * 1e6 * nvol)
+    return None
 
 
 def cpu_baseline(ph, params, iters_sample, threads):
@@ -97,12 +113,18 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--batch", type=int, default=1, help="volumes of the workload per launch")
+    ap.add_argument("--batch", type=int, default=4,
+                    help="volumes per launch on the batched path (value); 1 = plain only")
     ap.add_argument("--iters", type=int, default=200)
     ap.add_argument("--profile-iters", type=int, default=20)
     ap.add_argument("--cpu-iters", type=int, default=4)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--skip-plain", action="store_true",
+                    help="time only the batched path (so a profiler sees batched launches only)")
     ap.add_argument("--dims", type=str, default="256,256,160")
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
+                    help="before timing, copy BYTES device-to-device once (a plain copy "
+                         "kernel of known size for scaling HBM counters)")
     a = ap.parse_args()
     if a.gpus != 1:
         print(f"bench.py measures one GPU only (asked for --gpus {a.gpus}); volumes are "
@@ -116,34 +138,59 @@ def main():
     dims = tuple(int(v) for v in a.dims.split(","))
     vs = (0.9, 0.9, 1.0)
     params = qsm.TvParams(lambda_=1e-3, rho=0.1, tol=0.0, max_iter=a.iters)
-    ph = phantom.make_phantom(dims, voxel_size=vs, bdir=(0.0, 0.0, 1.0), n_ellipsoids=40, seed=1001)
     grid = qsm.Grid(dims, vs)
+    phantoms = [phantom.make_phantom(dims, voxel_size=vs, bdir=(0.0, 0.0, 1.0), n_ellipsoids=40,
+                                     seed=1001 + 1000 * v) for v in range(max(1, a.batch))]
+    ph = phantoms[0]
     keep, vols = [], []
-    for _ in range(a.batch):
-        tf = qsm.DeviceArray.from_numpy(ph.local_field)
-        tm = qsm.DeviceArray.from_numpy(ph.mask)
-        tc = qsm.DeviceArray(ph.local_field.size, np.float64)
+    for p_ in phantoms:
+        tf = qsm.DeviceArray.from_numpy(p_.local_field)
+        tm = qsm.DeviceArray.from_numpy(p_.mask)
+        tc = qsm.DeviceArray(p_.local_field.size, np.float64)
         keep.append((tf, tm, tc))
-        vols.append((grid, ph.bdir, tf.ptr, tm.ptr, tc.ptr))
+        vols.append((grid, p_.bdir, tf.ptr, tm.ptr, tc.ptr))
     qsm.synchronize()
-    batch = qsm.TvBatch(vols, params)
+    if a.copy_calibration:
+        n = a.copy_calibration // 4
+        src = qsm.DeviceArray.from_numpy(np.ones(n, np.float32))
+        dst = qsm.DeviceArray(n, np.float32)
+        qsm.copy_device(dst, src)
+        qsm.synchronize()
+        src.free()
+        dst.free()
 
-    for _ in range(a.warmup):
-        batch.run()
-    qsm.synchronize()
-    t0 = time.perf_counter()
-    for _ in range(a.steps):
-        batch.run()
-    qsm.synchronize()
-    dt = time.perf_counter() - t0
-    solves = a.steps * a.batch
-    value = solves / dt
-    ms_per_step = dt / a.steps * 1e3
-    iters_done = batch.status(0)[0]
+    def timed(batch, nvol):
+        for _ in range(a.warmup):
+            batch.run()
+        qsm.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(a.steps):
+            batch.run()
+        qsm.synchronize()
+        dt = time.perf_counter() - t0
+        return a.steps * nvol / dt, dt / a.steps * 1e3
+
+    nk = len(KERNELS)
+    plain_value = plain_ms = plain_prof = None
+    if not (a.skip_plain and a.batch > 1):
+        # plain: one volume per launch
+        plain = qsm.TvBatch(vols[:1], params)
+        plain_value, plain_ms = timed(plain, 1)
+        iters_done = plain.status(0)[0]
+        plain_prof = plain.profile(a.profile_iters)[:nk]
+        plain.close()
+    # batched: every volume in one launch per kernel
+    if a.batch > 1:
+        batch = qsm.TvBatch(vols, params)
+        value, ms_per_step = timed(batch, a.batch)
+        iters_done = batch.status(0)[0]
+        ms = batch.profile(a.profile_iters)[:nk]
+        batch.close()
+    else:
+        value, ms_per_step, ms = plain_value, plain_ms, plain_prof
 
     # per-kernel mean launch time (HIP events on the solver's stream)
-    ms = batch.profile(a.profile_iters)
-    nbytes = [b * a.batch for b in alg_bytes(dims)]
+    nbytes = [b * max(1, a.batch) for b in alg_bytes(dims)]
     dom = int(np.argmax(ms))
     achieved = nbytes[dom] / (ms[dom] * 1e-3) / 1e9
     iter_bytes = sum(nbytes)
@@ -154,7 +201,7 @@ def main():
         "peak": HBM_PEAK_GBS,
         "unit": "GB/s",
         "frac": round(achieved / HBM_PEAK_GBS, 4),
-        "traffic": None,
+        "traffic": traffic_for(dom, max(1, a.batch), dims),
         "kernel": KERNELS[dom],
         "kernel_ms": round(float(ms[dom]), 5),
         "alg_bytes_per_launch": int(nbytes[dom]),
@@ -165,6 +212,7 @@ def main():
     if not a.no_cpu:
         threads = min(16, os.cpu_count() or 1)
         cpu = cpu_baseline(ph, params, a.cpu_iters, threads)
+        cpu["ratio_gpu_over_cpu"] = round(value / cpu["value"], 1)
 
     out = {
         "metric": METRIC,
@@ -187,11 +235,14 @@ def main():
             "iterations": iters_done,
             "path": "batched" if a.batch > 1 else "plain",
         },
+        "plain": None if plain_value is None else {
+            "value": round(plain_value, 3), "ms_per_step": round(plain_ms, 3),
+            "per_kernel_ms": {k: round(float(v), 5) for k, v in zip(KERNELS, plain_prof)}},
+        "batched_over_plain": None if plain_value is None else round(value / plain_value, 3),
         "roofline": roofline,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out))
-    batch.close()
 
 
 if __name__ == "__main__":

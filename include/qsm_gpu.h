@@ -173,10 +173,10 @@ int qsm_tv_batch_status(const qsm_tv_batch* batch, size_t v, size_t* iters, doub
 void qsm_tv_batch_destroy(qsm_tv_batch* batch);
 /* Mean duration (ms, HIP events on the batch's stream) of each kernel of the
  * iteration over `iters` timed iterations after a fresh spectral setup:
- * ms_out[0] z/u update + divergence + R2C x, [1] C2C y, [2] C2C z + x-update
- * + inverse z, [3] inverse C2C y, [4] C2R x + relative-change sums,
- * [5] relative-change reduction.  n_out >= 6.  Leaves the batch ready for
- * qsm_tv_batch_run. */
+ * ms_out[0] previous iteration's relative-change reduction + z/u update +
+ * divergence + R2C x, [1] C2C y, [2] C2C z + x-update + inverse z,
+ * [3] inverse C2C y, [4] C2R x + relative-change sums.  n_out >= 5.  Leaves
+ * the batch ready for qsm_tv_batch_run. */
 int qsm_tv_batch_profile(qsm_tv_batch* batch, size_t iters, double* ms_out, size_t n_out);
 
 /* ---- device memory helpers (no reference counterpart) ------------------ */
@@ -187,6 +187,7 @@ int qsm_device_alloc(size_t bytes, void** out);
 int qsm_device_free(void* ptr);
 int qsm_memcpy_htod(void* dst, const void* src, size_t bytes);
 int qsm_memcpy_dtoh(void* dst, const void* src, size_t bytes);
+int qsm_memcpy_dtod(void* dst, const void* src, size_t bytes);
 int qsm_device_synchronize(void);
 
 /* ---- device-resident batched V-SHARP (no reference counterpart) ------- */

@@ -108,6 +108,7 @@ _SIGS = {
     "qsm_device_free": (C.c_int, [C.c_void_p]),
     "qsm_memcpy_htod": (C.c_int, [C.c_void_p, C.c_void_p, _SZ]),
     "qsm_memcpy_dtoh": (C.c_int, [C.c_void_p, C.c_void_p, _SZ]),
+    "qsm_memcpy_dtod": (C.c_int, [C.c_void_p, C.c_void_p, _SZ]),
     "qsm_device_synchronize": (C.c_int, []),
 }
 for _name, (_res, _args) in _SIGS.items():
@@ -462,6 +463,13 @@ def synchronize():
     _check(_lib.qsm_device_synchronize())
 
 
+def copy_device(dst: DeviceArray, src: DeviceArray):
+    """Device-to-device copy of src into dst (same byte size)."""
+    nbytes = src.n * src.dtype.itemsize
+    assert dst.n * dst.dtype.itemsize == nbytes
+    _check(_lib.qsm_memcpy_dtod(C.c_void_p(dst.ptr), C.c_void_p(src.ptr), nbytes))
+
+
 # ------------------------------------------------ device-resident batches --
 class TvBatch:
     """Batched TV-ADMM over device-resident volumes (``qsm_tv_batch_*``).
@@ -493,10 +501,10 @@ class TvBatch:
         return int(it.value), float(rel.value)
 
     def profile(self, iters: int = 20):
-        """Mean per-launch milliseconds of the six kernels of one iteration
+        """Mean per-launch milliseconds of the five kernels of one iteration
         (see qsm_tv_batch_profile in include/qsm_gpu.h)."""
-        out = np.zeros(6)
-        _check(_lib.qsm_tv_batch_profile(self._h, iters, out.ctypes.data_as(_D), 6))
+        out = np.zeros(5)
+        _check(_lib.qsm_tv_batch_profile(self._h, iters, out.ctypes.data_as(_D), 5))
         return out
 
     def close(self):

@@ -180,4 +180,11 @@ constexpr int fixed_rows(int n) {
     return rb < 1 ? 1 : (rb > 32 ? 32 : rb);
 }
 
+// Pair-rows per block of the TV C2R pass: half of the other row passes, so
+// the previous iterate can be held in registers across the FFT.
+constexpr int fixed_rows_c(int n) {
+    int rb = fixed_rows(n) / 2;
+    return rb < 1 ? 1 : rb;
+}
+
 }  // namespace qsmfft

@@ -40,6 +40,7 @@ enum SpecSel : int { SPEC_WORK = 0, SPEC_A = 1, SPEC_B = 2, SPEC_C = 3, SPEC_COU
 struct VolDesc {
     int nx, ny, nz, nxh;
     int rows_x;                  // pair-rows per block in the x passes
+    int rows_c;                  // pair-rows per block in the TV C2R pass
     int ntx;                     // y/z tiles along kx: ceil(nxh / kTile)
     uint64_t m_nx, m_nxh, m_ny;  // magic divisors
     qsmfft::Plan px, py, pz;
@@ -106,6 +107,8 @@ struct PassArgs {
     const VolDesc* vols;
     const float2* tw;
     const VolStatus* status;  // convergence flags checked at block entry (may be null)
+    VolStatus* status_rw;     // convergence state updated by the TV R2C pass (null: no update)
+    VolStatus* mirror;        // host-visible copy of status_rw (may be null)
     int sel, sel2;            // spectrum selectors (y / z passes, R2C output)
     int parity;               // TV iteration parity
     int ridx, first, divide;  // V-SHARP radius index / first radius / SHARP division

@@ -27,9 +27,35 @@ using qsmfft::fastdiv;
 
 constexpr int kMaxLen = 512;
 
+// Device pointers held in the volume descriptor are generic (flat) to the
+// compiler; accessed through global-address-space types they compile to
+// global loads/stores with a scalar base instead of flat operations, which
+// also keeps them off the LDS wait counter.
+#define QSM_GLOBAL __attribute__((address_space(1)))
+template <class T>
+__device__ __forceinline__ QSM_GLOBAL T* glb(T* p) { return (QSM_GLOBAL T*)p; }
+template <class T>
+__device__ __forceinline__ const QSM_GLOBAL T* glb(const T* p) { return (const QSM_GLOBAL T*)p; }
+
+// 8-byte complex access through a global pointer (one dwordx2 load/store).
+typedef float qsm_f2v __attribute__((ext_vector_type(2)));
+__device__ __forceinline__ float2 ldg2(const QSM_GLOBAL float2* p) {
+    const qsm_f2v v = *(const QSM_GLOBAL qsm_f2v*)p;
+    return make_float2(v.x, v.y);
+}
+__device__ __forceinline__ void stg2(QSM_GLOBAL float2* p, float2 a) {
+    qsm_f2v v;
+    v.x = a.x;
+    v.y = a.y;
+    *(QSM_GLOBAL qsm_f2v*)p = v;
+}
+
 // elements per thread of an x (row) pass / a column pass
 __host__ __device__ constexpr int x_emax(int n) {
     return n > 0 ? (qsmfft::fixed_rows(n) * n + kThreads - 1) / kThreads : 16;
+}
+__host__ __device__ constexpr int xc_emax(int n) {
+    return n > 0 ? (qsmfft::fixed_rows_c(n) * n + kThreads - 1) / kThreads : 16;
 }
 __host__ __device__ constexpr int col_emax(int n, int eg) {
     return n > 0 ? (kTile * n + kThreads - 1) / kThreads : eg;
@@ -38,17 +64,18 @@ __host__ __device__ constexpr int x_lds(int n) { return n > 0 ? qsmfft::fixed_ro
 __host__ __device__ constexpr int col_lds(int n, int eg) { return n > 0 ? kTile * n : kThreads * eg; }
 __host__ __device__ constexpr int tw_lds(int n) { return n > 0 ? n : kMaxLen; }
 
-template <int N, int EMAX, bool COLFAST, int LOG2C, bool INV>
+template <int N, int EMAX, bool COLFAST, int LOG2C, bool INV, int NC = 0>
 __device__ __forceinline__ void run_fft(float2* lds, const qsmfft::Plan& P, int ncols, int es, int cs,
                                         const float2* twl) {
     if constexpr (N > 0)
-        qsmfft::fixed_fft<N, COLFAST ? kTile : qsmfft::fixed_rows(N), COLFAST, INV>(lds, twl);
+        qsmfft::fixed_fft<N, NC ? NC : (COLFAST ? kTile : qsmfft::fixed_rows(N)), COLFAST, INV>(lds, twl);
     else
         qsmfft::lds_fft<EMAX, COLFAST, LOG2C, INV>(lds, P, ncols, es, cs, twl);
 }
 
 __device__ __forceinline__ void load_twiddles(float2* twl, const float2* __restrict__ tw, int n) {
-    for (int q = threadIdx.x; q < n; q += kThreads) twl[q] = tw[q];
+    const QSM_GLOBAL float2* g = glb(tw);
+    for (int q = threadIdx.x; q < n; q += kThreads) twl[q] = ldg2(g + q);
 }
 
 __device__ __forceinline__ float shrinkf(float x, float t) {
@@ -66,6 +93,38 @@ __device__ inline double block_sum(double v, double* sh) {
     if (threadIdx.x == 0)
         for (int w = 0; w < kThreads / 64; ++w) s += sh[w];
     return s;
+}
+
+// relative_change = ||x - x_prev|| / (||x|| + 1e-20), converged when < tol
+// (src/inversion/admm.rs:117-119, src/utils/mod.rs:69-77), from the per-block
+// FP64 sums (sum d^2, sum x^2) that the TV C2R pass left in V.partial.  One
+// block, fixed summation order: repeated runs give identical results.
+__device__ __forceinline__ void tv_reduce_block(const VolDesc& V, int gv, VolStatus* status_rw,
+                                                VolStatus* mirror, double* sh) {
+    typedef double d2v __attribute__((ext_vector_type(2)));
+    const QSM_GLOBAL d2v* part = (const QSM_GLOBAL d2v*)glb(V.partial);
+    double a = 0.0, b = 0.0;
+#pragma unroll 4
+    for (int q = threadIdx.x; q < V.npartial; q += kThreads) {
+        const d2v v = part[q];
+        a += v.x;
+        b += v.y;
+    }
+    const double ta = block_sum(a, sh);
+    const double tb = block_sum(b, sh);
+    if (threadIdx.x == 0) {
+        const double rel = sqrt(ta) / (sqrt(tb) + 1e-20);
+        VolStatus st = status_rw[gv];
+        st.iters += 1;
+        st.rel = rel;
+        if (rel < V.tol) st.done = 1;
+        status_rw[gv] = st;
+        if (mirror) {
+            // host-visible copy, read by the host loop after an event wait
+            mirror[gv] = st;
+            __threadfence_system();
+        }
+    }
 }
 
 __device__ __forceinline__ double fftfreq_at(int i, int n, double d) {
@@ -97,8 +156,8 @@ __device__ __forceinline__ float sphere_element(const VolDesc& V, int i, int r, 
 // Column tile: element (row, c) at lds[row*kTile + c] <-> base[kx0 + c + row*stride];
 // columns kx >= nxh read as zero and are not written.
 template <int EMAX>
-__device__ __forceinline__ void col_load(float2* lds, const float2* __restrict__ base, size_t stride,
-                                         int n, int kx0, int nxh) {
+__device__ __forceinline__ void col_load(float2* lds, const QSM_GLOBAL float2* __restrict__ base,
+                                         size_t stride, int n, int kx0, int nxh) {
     float2 r[EMAX];
     const int total = n * kTile;
 #pragma unroll
@@ -106,7 +165,7 @@ __device__ __forceinline__ void col_load(float2* lds, const float2* __restrict__
         const int e = threadIdx.x + t * kThreads;
         const int row = e >> kLog2Tile;
         const int kx = kx0 + (e & (kTile - 1));
-        r[t] = (e < total && kx < nxh) ? base[kx + row * stride] : make_float2(0.0f, 0.0f);
+        r[t] = (e < total && kx < nxh) ? ldg2(base + kx + row * stride) : make_float2(0.0f, 0.0f);
     }
 #pragma unroll
     for (int t = 0; t < EMAX; ++t) {
@@ -116,15 +175,15 @@ __device__ __forceinline__ void col_load(float2* lds, const float2* __restrict__
 }
 
 template <int EMAX>
-__device__ __forceinline__ void col_store(const float2* lds, float2* __restrict__ base, size_t stride,
-                                          int n, int kx0, int nxh) {
+__device__ __forceinline__ void col_store(const float2* lds, QSM_GLOBAL float2* __restrict__ base,
+                                          size_t stride, int n, int kx0, int nxh) {
     const int total = n * kTile;
 #pragma unroll
     for (int t = 0; t < EMAX; ++t) {
         const int e = threadIdx.x + t * kThreads;
         const int row = e >> kLog2Tile;
         const int kx = kx0 + (e & (kTile - 1));
-        if (e < total && kx < nxh) base[kx + row * stride] = lds[e];
+        if (e < total && kx < nxh) stg2(base + kx + row * stride, lds[e]);
     }
 }
 
@@ -133,9 +192,9 @@ enum SrcKind : int { SRC_F64 = 0, SRC_U8 = 1, SRC_F32 = 2, SRC_SPHERE = 3, SRC_T
 
 template <int SRC>
 __device__ __forceinline__ float simple_src(const VolDesc& V, int i, int r, int aux) {
-    if constexpr (SRC == SRC_F64) return (float)V.field[i + (size_t)r * V.nx];
-    else if constexpr (SRC == SRC_U8) return (float)V.mask[i + (size_t)r * V.nx];
-    else if constexpr (SRC == SRC_F32) return V.vs_local[i + (size_t)r * V.nx];
+    if constexpr (SRC == SRC_F64) return (float)glb(V.field)[i + (size_t)r * V.nx];
+    else if constexpr (SRC == SRC_U8) return (float)glb(V.mask)[i + (size_t)r * V.nx];
+    else if constexpr (SRC == SRC_F32) return glb(V.vs_local)[i + (size_t)r * V.nx];
     else return sphere_element(V, i, r, aux);
 }
 
@@ -170,83 +229,121 @@ __device__ __forceinline__ void fill_simple(float2* lds, const VolDesc& V, int n
 // CH elements (2*CH voxels) per round, all their loads issued before the
 // arithmetic; restrict-qualified so the u' stores of one round do not order
 // the next round's loads.
-template <int EMAX>
-__device__ __forceinline__ void fill_tvdiv(float2* lds, const VolDesc& V, int nx, int RB,
-                                           const float* __restrict__ x, const float* __restrict__ u,
-                                           float* __restrict__ uo, int row0, int nrows) {
-    constexpr int CH = 2;
+// The operands of CH elements (2*CH voxels) of the divergence.
+template <int CH>
+struct TvDivRound {
+    float xv[CH][2][7], uv[CH][2][6];
+    int idx[CH][2];
+    bool ok[CH][2];
+};
+
+template <int CH>
+__device__ __forceinline__ void tvdiv_load(TvDivRound<CH>& R, const VolDesc& V, int nx, int t0, int EMAX,
+                                           int total, int row0, int nrows,
+                                           const QSM_GLOBAL float* __restrict__ x,
+                                           const QSM_GLOBAL float* __restrict__ u) {
     const int ny = V.ny, nz = V.nz;
     const int nxy = nx * ny, N = nxy * nz;
-    const int total = RB * nx;
+#pragma unroll
+    for (int c = 0; c < CH; ++c) {
+        const int e = threadIdx.x + (t0 + c) * kThreads;
+        const int p = e / nx;
+        const int i = e - p * nx;
+#pragma unroll
+        for (int h = 0; h < 2; ++h) {
+            const int r = row0 + 2 * p + h;
+            const bool ok = (t0 + c < EMAX) && (e < total) && (r < nrows);
+            R.ok[c][h] = ok;
+            const int rr = ok ? r : 0;
+            const int ii = ok ? i : 0;
+            const int k = (int)fastdiv((uint32_t)rr, V.m_ny);
+            const int j = rr - k * ny;
+            const int idx = ii + rr * nx;
+            R.idx[c][h] = idx;
+            const int dxp = (ii + 1 < nx) ? 1 : 1 - nx;
+            const int dxm = (ii > 0) ? -1 : nx - 1;
+            const int dyp = (j + 1 < ny) ? nx : nx - nxy;
+            const int dym = (j > 0) ? -nx : nxy - nx;
+            const int dzp = (k + 1 < nz) ? nxy : nxy - N;
+            const int dzm = (k > 0) ? -nxy : N - nxy;
+            R.xv[c][h][0] = x[idx];
+            R.xv[c][h][1] = x[idx + dxp];
+            R.xv[c][h][2] = x[idx + dyp];
+            R.xv[c][h][3] = x[idx + dzp];
+            R.xv[c][h][4] = x[idx + dxm];
+            R.xv[c][h][5] = x[idx + dym];
+            R.xv[c][h][6] = x[idx + dzm];
+            R.uv[c][h][0] = u[idx];
+            R.uv[c][h][1] = u[N + idx];
+            R.uv[c][h][2] = u[2 * N + idx];
+            R.uv[c][h][3] = u[idx + dxm];
+            R.uv[c][h][4] = u[N + idx + dym];
+            R.uv[c][h][5] = u[2 * N + idx + dzm];
+        }
+    }
+}
+
+template <int CH>
+__device__ __forceinline__ void tvdiv_compute(const TvDivRound<CH>& R, float2* lds, const VolDesc& V,
+                                              int t0, int EMAX, int total,
+                                              QSM_GLOBAL float* __restrict__ uo) {
+    const int N = V.nx * V.ny * V.nz;
     const float t = V.thr, hx = V.hx, hy = V.hy, hz = V.hz;
 #pragma unroll
-    for (int t0 = 0; t0 < EMAX; t0 += CH) {
-        float xv[CH][2][7], uv[CH][2][6];
-        int idxv[CH][2];
-        bool ok[CH][2];
+    for (int c = 0; c < CH; ++c) {
+        const int e = threadIdx.x + (t0 + c) * kThreads;
+        float d[2];
 #pragma unroll
-        for (int c = 0; c < CH; ++c) {
-            const int e = threadIdx.x + (t0 + c) * kThreads;
-            const int p = e / nx;
-            const int i = e - p * nx;
-#pragma unroll
-            for (int h = 0; h < 2; ++h) {
-                const int r = row0 + 2 * p + h;
-                ok[c][h] = (t0 + c < EMAX) && (e < total) && (r < nrows);
-                const int rr = ok[c][h] ? r : 0;
-                const int ii = ok[c][h] ? i : 0;
-                const int k = (int)fastdiv((uint32_t)rr, V.m_ny);
-                const int j = rr - k * ny;
-                const int idx = ii + rr * nx;
-                idxv[c][h] = idx;
-                const int dxp = (ii + 1 < nx) ? 1 : 1 - nx;
-                const int dxm = (ii > 0) ? -1 : nx - 1;
-                const int dyp = (j + 1 < ny) ? nx : nx - nxy;
-                const int dym = (j > 0) ? -nx : nxy - nx;
-                const int dzp = (k + 1 < nz) ? nxy : nxy - N;
-                const int dzm = (k > 0) ? -nxy : N - nxy;
-                xv[c][h][0] = x[idx];
-                xv[c][h][1] = x[idx + dxp];
-                xv[c][h][2] = x[idx + dyp];
-                xv[c][h][3] = x[idx + dzp];
-                xv[c][h][4] = x[idx + dxm];
-                xv[c][h][5] = x[idx + dym];
-                xv[c][h][6] = x[idx + dzm];
-                uv[c][h][0] = u[idx];
-                uv[c][h][1] = u[N + idx];
-                uv[c][h][2] = u[2 * N + idx];
-                uv[c][h][3] = u[idx + dxm];
-                uv[c][h][4] = u[N + idx + dym];
-                uv[c][h][5] = u[2 * N + idx + dzm];
+        for (int h = 0; h < 2; ++h) {
+            const float xc = R.xv[c][h][0];
+            const float vx = (R.xv[c][h][1] - xc) * hx + R.uv[c][h][0];
+            const float vy = (R.xv[c][h][2] - xc) * hy + R.uv[c][h][1];
+            const float vz = (R.xv[c][h][3] - xc) * hz + R.uv[c][h][2];
+            const float vxm = (xc - R.xv[c][h][4]) * hx + R.uv[c][h][3];
+            const float vym = (xc - R.xv[c][h][5]) * hy + R.uv[c][h][4];
+            const float vzm = (xc - R.xv[c][h][6]) * hz + R.uv[c][h][5];
+            const float zx = shrinkf(vx, t), zy = shrinkf(vy, t), zz = shrinkf(vz, t);
+            if (R.ok[c][h]) {
+                const int idx = R.idx[c][h];
+                uo[idx] = vx - zx;
+                uo[N + idx] = vy - zy;
+                uo[2 * N + idx] = vz - zz;
             }
+            const float gx = 2.0f * zx - vx, gy = 2.0f * zy - vy, gz = 2.0f * zz - vz;
+            const float gxm = 2.0f * shrinkf(vxm, t) - vxm;
+            const float gym = 2.0f * shrinkf(vym, t) - vym;
+            const float gzm = 2.0f * shrinkf(vzm, t) - vzm;
+            d[h] = R.ok[c][h] ? (gx - gxm) * hx + (gy - gym) * hy + (gz - gzm) * hz : 0.0f;
         }
+        if (t0 + c < EMAX && e < total) lds[e] = make_float2(d[0], d[1]);
+    }
+}
+
+// Divergence of g = 2z - v for the z/u update fused in front of the
+// x-subproblem: v = grad(x) + u (periodic forward differences,
+// src/utils/gradient.rs:21-58), z = shrink(v, lambda/rho), u' = v - z,
+// g = 2z - v (src/inversion/admm.rs:127-142), div = bdiv(g)
+// (src/utils/gradient.rs:71-108).  The x component of g at i-1 is recomputed
+// from x and u at i-1 (likewise y and z), so g never goes to memory.
+// Rounds of CH elements (2*CH voxels) are software-pipelined: the loads of
+// round r+1 are issued before round r is computed and stored.
+template <int EMAX>
+__device__ __forceinline__ void fill_tvdiv(float2* lds, const VolDesc& V, int nx, int RB,
+                                           const QSM_GLOBAL float* __restrict__ x,
+                                           const QSM_GLOBAL float* __restrict__ u,
+                                           QSM_GLOBAL float* __restrict__ uo, int row0, int nrows) {
+    constexpr int CH = 2;
+    constexpr int NR = (EMAX + CH - 1) / CH;
+    const int total = RB * nx;
+    TvDivRound<CH> A, B;
+    tvdiv_load<CH>(A, V, nx, 0, EMAX, total, row0, nrows, x, u);
 #pragma unroll
-        for (int c = 0; c < CH; ++c) {
-            const int e = threadIdx.x + (t0 + c) * kThreads;
-            float d[2];
-#pragma unroll
-            for (int h = 0; h < 2; ++h) {
-                const float xc = xv[c][h][0];
-                const float vx = (xv[c][h][1] - xc) * hx + uv[c][h][0];
-                const float vy = (xv[c][h][2] - xc) * hy + uv[c][h][1];
-                const float vz = (xv[c][h][3] - xc) * hz + uv[c][h][2];
-                const float vxm = (xc - xv[c][h][4]) * hx + uv[c][h][3];
-                const float vym = (xc - xv[c][h][5]) * hy + uv[c][h][4];
-                const float vzm = (xc - xv[c][h][6]) * hz + uv[c][h][5];
-                const float zx = shrinkf(vx, t), zy = shrinkf(vy, t), zz = shrinkf(vz, t);
-                if (ok[c][h]) {
-                    const int idx = idxv[c][h];
-                    uo[idx] = vx - zx;
-                    uo[N + idx] = vy - zy;
-                    uo[2 * N + idx] = vz - zz;
-                }
-                const float gx = 2.0f * zx - vx, gy = 2.0f * zy - vy, gz = 2.0f * zz - vz;
-                const float gxm = 2.0f * shrinkf(vxm, t) - vxm;
-                const float gym = 2.0f * shrinkf(vym, t) - vym;
-                const float gzm = 2.0f * shrinkf(vzm, t) - vzm;
-                d[h] = ok[c][h] ? (gx - gxm) * hx + (gy - gym) * hy + (gz - gzm) * hz : 0.0f;
-            }
-            if (t0 + c < EMAX && e < total) lds[e] = make_float2(d[0], d[1]);
+    for (int r = 0; r < NR; r += 2) {
+        if (r + 1 < NR) tvdiv_load<CH>(B, V, nx, (r + 1) * CH, EMAX, total, row0, nrows, x, u);
+        tvdiv_compute<CH>(A, lds, V, r * CH, EMAX, total, uo);
+        if (r + 1 < NR) {
+            if (r + 2 < NR) tvdiv_load<CH>(A, V, nx, (r + 2) * CH, EMAX, total, row0, nrows, x, u);
+            tvdiv_compute<CH>(B, lds, V, (r + 1) * CH, EMAX, total, uo);
         }
     }
 }
@@ -259,7 +356,8 @@ __device__ __forceinline__ void fill_tvdiv(float2* lds, const VolDesc& V, int nx
 template <int N, int EG, int SRC>
 __global__ __launch_bounds__(kThreads) void k_r2c_x(const VolDesc* __restrict__ vols, LaunchTable lt,
                                                     const float2* __restrict__ tw, int out_sel, int aux,
-                                                    const VolStatus* __restrict__ status) {
+                                                    const VolStatus* status, VolStatus* status_rw,
+                                                    VolStatus* mirror) {
     constexpr int EMAX = x_emax(N);
     __shared__ float2 lds[x_lds(N)];
     __shared__ float2 twl[tw_lds(N)];
@@ -268,6 +366,15 @@ __global__ __launch_bounds__(kThreads) void k_r2c_x(const VolDesc* __restrict__ 
     const int gv = locate(lt, bid, lb);
     const VolDesc& V = vols[gv];
     if (status && status[gv].done) return;
+    if constexpr (SRC == SRC_TVDIV) {
+        // The first block of a volume closes the previous iteration: it
+        // reduces the C2R pass's sums and may mark the volume converged.
+        // Blocks of this pass that still see it running do work that is
+        // discarded (they write only the spectrum and the dual variable,
+        // never the iterate x that is returned).
+        __shared__ double sh[kThreads / 64];
+        if (status_rw && lb == 0) tv_reduce_block(V, gv, status_rw, mirror, sh);
+    }
     const int nx = N > 0 ? N : V.nx;
     const int nxh = nx / 2 + 1;
     const int RB = N > 0 ? qsmfft::fixed_rows(N) : V.rows_x;
@@ -275,12 +382,13 @@ __global__ __launch_bounds__(kThreads) void k_r2c_x(const VolDesc* __restrict__ 
     const int row0 = lb * RB * 2;
     load_twiddles(twl, tw + V.px.tw_off, nx);
     if constexpr (SRC == SRC_TVDIV)
-        fill_tvdiv<EMAX>(lds, V, nx, RB, V.x[aux], V.u[aux ^ 1], V.u[aux], row0, nrows);
+        fill_tvdiv<EMAX>(lds, V, nx, RB, glb(V.x[aux]), glb(V.u[aux ^ 1]), glb(V.u[aux]), row0,
+                         nrows);
     else
         fill_simple<EMAX, SRC>(lds, V, nx, RB, row0, nrows, aux);
     __syncthreads();
     run_fft<N, EMAX, false, 0, false>(lds, V.px, RB, 1, nx, twl);
-    float2* __restrict__ out = V.spec[out_sel];
+    QSM_GLOBAL float2* __restrict__ out = glb(V.spec[out_sel]);
     const int total = RB * nxh;
 #pragma unroll
     for (int t = 0; t < EMAX; ++t) {
@@ -293,8 +401,8 @@ __global__ __launch_bounds__(kThreads) void k_r2c_x(const VolDesc* __restrict__ 
             const float2 zm = lds[p * nx + (k == 0 ? 0 : nx - k)];
             const float2 A = make_float2(0.5f * (zk.x + zm.x), 0.5f * (zk.y - zm.y));
             const float2 B = make_float2(0.5f * (zk.y + zm.y), -0.5f * (zk.x - zm.x));
-            if (ra < nrows) out[k + (size_t)ra * nxh] = A;
-            if (rb < nrows) out[k + (size_t)rb * nxh] = B;
+            if (ra < nrows) stg2(out + k + (size_t)ra * nxh, A);
+            if (rb < nrows) stg2(out + k + (size_t)rb * nxh, B);
         }
     }
 }
@@ -322,7 +430,7 @@ __global__ __launch_bounds__(kThreads) void k_y_pass(const VolDesc* __restrict__
     const int kx0 = tx * kTile;
     const int nxh = V.nxh;
     const int ny = N > 0 ? N : V.ny;
-    float2* buf = V.spec[s] + (size_t)kz * nxh * ny;
+    QSM_GLOBAL float2* buf = glb(V.spec[s]) + (size_t)kz * nxh * ny;
     load_twiddles(twl, tw + V.py.tw_off, ny);
     col_load<EMAX>(lds, buf, nxh, ny, kx0, nxh);
     __syncthreads();
@@ -381,22 +489,22 @@ __global__ __launch_bounds__(kThreads) void k_z_tv(const VolDesc* __restrict__ v
     const size_t zs = (size_t)nxh * V.ny;
     const size_t base = (size_t)ky * nxh;
     const int total = nz * kTile;
-    float2* buf = V.spec[SPEC_WORK] + base;
+    QSM_GLOBAL float2* buf = glb(V.spec[SPEC_WORK]) + base;
     load_twiddles(twl, tw + V.pz.tw_off, nz);
     col_load<EMAX>(lds, buf, zs, nz, kx0, nxh);
     // x-update operands, in flight during the forward FFT
     float2 fr[OP == ZOP_TV_SOLVE ? EMAX : 1];
     float qr[OP == ZOP_TV_SOLVE ? EMAX : 1];
     if constexpr (OP == ZOP_TV_SOLVE) {
-        const float2* __restrict__ F = V.spec[SPEC_A] + base;
-        const float* __restrict__ Q = V.q + base;
+        const QSM_GLOBAL float2* __restrict__ F = glb(V.spec[SPEC_A]) + base;
+        const QSM_GLOBAL float* __restrict__ Q = glb(V.q) + base;
 #pragma unroll
         for (int t = 0; t < EMAX; ++t) {
             const int e = threadIdx.x + t * kThreads;
             const int kz = e >> kLog2Tile;
             const int kx = kx0 + (e & (kTile - 1));
             const bool in = e < total && kx < nxh;
-            fr[t] = in ? F[kx + kz * zs] : make_float2(0.0f, 0.0f);
+            fr[t] = in ? ldg2(F + kx + kz * zs) : make_float2(0.0f, 0.0f);
             qr[t] = in ? Q[kx + kz * zs] : 0.0f;
         }
     }
@@ -404,8 +512,8 @@ __global__ __launch_bounds__(kThreads) void k_z_tv(const VolDesc* __restrict__ v
     run_fft<N, EMAX, true, kLog2Tile, false>(lds, V.pz, kTile, kTile, 1, twl);
     if constexpr (OP == ZOP_TV_SETUP) {
         const double invN = 1.0 / ((double)V.nx * V.ny * V.nz);
-        float2* F = V.spec[SPEC_A] + base;
-        float* Q = V.q + base;
+        QSM_GLOBAL float2* F = glb(V.spec[SPEC_A]) + base;
+        QSM_GLOBAL float* Q = glb(V.q) + base;
         for (int e = threadIdx.x; e < total; e += kThreads) {
             const int kz = e >> kLog2Tile, c = e & (kTile - 1);
             const int kx = kx0 + c;
@@ -417,7 +525,7 @@ __global__ __launch_bounds__(kThreads) void k_z_tv(const VolDesc* __restrict__ v
                 const double m = 0.5 * (d1 + d2) * invN;
                 const double qv = V.rho * 0.5 * (i1 + i2) * invN;
                 const float2 sv = lds[e];
-                F[kx + kz * zs] = make_float2((float)((double)sv.x * m), (float)((double)sv.y * m));
+                stg2(F + kx + kz * zs, make_float2((float)((double)sv.x * m), (float)((double)sv.y * m)));
                 Q[kx + kz * zs] = (float)qv;
             }
         }
@@ -443,8 +551,9 @@ __global__ __launch_bounds__(kThreads) void k_z_tv(const VolDesc* __restrict__ v
 // step 2: A, B = rows 2p, 2p+1 of sa; step 1: A = row p of sa, B = row p of sb.
 template <int EMAX>
 __device__ __forceinline__ void c2r_load(float2* lds, int nx, int nxh, int RB,
-                                         const float2* __restrict__ sa, const float2* __restrict__ sb,
-                                         int row0, int step, int nrows) {
+                                         const QSM_GLOBAL float2* __restrict__ sa,
+                                         const QSM_GLOBAL float2* __restrict__ sb, int row0, int step,
+                                         int nrows) {
     const int total = RB * nx;
     float2 r[EMAX];
 #pragma unroll
@@ -456,8 +565,8 @@ __device__ __forceinline__ void c2r_load(float2* lds, int nx, int nxh, int RB,
         const int kk = neg ? nx - k : k;
         const int ra = row0 + step * p;
         const int rb = (step == 2) ? ra + 1 : ra;
-        float2 A = (e < total && ra < nrows) ? sa[kk + (size_t)ra * nxh] : make_float2(0.0f, 0.0f);
-        float2 B = (e < total && rb < nrows) ? sb[kk + (size_t)rb * nxh] : make_float2(0.0f, 0.0f);
+        float2 A = (e < total && ra < nrows) ? ldg2(sa + kk + (size_t)ra * nxh) : make_float2(0.0f, 0.0f);
+        float2 B = (e < total && rb < nrows) ? ldg2(sb + kk + (size_t)rb * nxh) : make_float2(0.0f, 0.0f);
         if (kk == 0 || 2 * kk == nx) { A.y = 0.0f; B.y = 0.0f; }
         if (neg) { A.y = -A.y; B.y = -B.y; }
         r[t] = make_float2(A.x - B.y, A.y + B.x);
@@ -470,13 +579,16 @@ __device__ __forceinline__ void c2r_load(float2* lds, int nx, int nxh, int RB,
 }
 
 // TV: x_it = Re IFFT(x^) (src/inversion/admm.rs:111-114) and the two sums of
-// relative_change (src/utils/mod.rs:69-77) accumulated in FP64 per block.
+// relative_change (src/utils/mod.rs:69-77) accumulated in FP64 per block;
+// the next pass's first block (or k_tv_reduce after the last iteration)
+// reduces them.
 template <int N, int EG>
 __global__ __launch_bounds__(kThreads) void k_c2r_tv(const VolDesc* __restrict__ vols, LaunchTable lt,
                                                      const float2* __restrict__ tw, int parity,
                                                      const VolStatus* __restrict__ status) {
-    constexpr int EMAX = x_emax(N);
-    __shared__ float2 lds[x_lds(N)];
+    constexpr int EMAX = xc_emax(N);
+    constexpr int RBF = N > 0 ? qsmfft::fixed_rows_c(N) : 0;
+    __shared__ float2 lds[N > 0 ? RBF * N : kThreads * 16];
     __shared__ float2 twl[tw_lds(N)];
     __shared__ double sh[kThreads / 64];
     const int bid = xcd_remap(blockIdx.x, gridDim.x);
@@ -486,27 +598,28 @@ __global__ __launch_bounds__(kThreads) void k_c2r_tv(const VolDesc* __restrict__
     if (status && status[gv].done) return;
     const int nx = N > 0 ? N : V.nx;
     const int nxh = nx / 2 + 1;
-    const int RB = N > 0 ? qsmfft::fixed_rows(N) : V.rows_x;
+    const int RB = N > 0 ? RBF : V.rows_c;
     const int nrows = V.ny * V.nz;
     const int row0 = lb * RB * 2;
     const int total = RB * nx;
-    const float2* s = V.spec[SPEC_WORK];
+    const QSM_GLOBAL float2* s = glb(V.spec[SPEC_WORK]);
     load_twiddles(twl, tw + V.px.tw_off, nx);
     c2r_load<EMAX>(lds, nx, nxh, RB, s, s, row0, 2, nrows);
-    const float* __restrict__ xo = V.x[parity];
-    float* __restrict__ xn = V.x[parity ^ 1];
-    float2 xp[EMAX];  // previous iterate, loaded during the inverse FFT
+    const QSM_GLOBAL float* __restrict__ xo = glb(V.x[parity]);
+    QSM_GLOBAL float* __restrict__ xn = glb(V.x[parity ^ 1]);
+    float2 xp[EMAX];  // previous iterate, in flight during the inverse FFT
 #pragma unroll
     for (int t = 0; t < EMAX; ++t) {
         const int e = threadIdx.x + t * kThreads;
         const int p = e / nx;
         const int i = e - p * nx;
-        const int ra = row0 + 2 * p, rb = ra + 1;
-        xp[t].x = (e < total && ra < nrows) ? xo[i + (size_t)ra * nx] : 0.0f;
-        xp[t].y = (e < total && rb < nrows) ? xo[i + (size_t)rb * nx] : 0.0f;
+        // rows past the volume read its last row: always in bounds, never used
+        const int ra = min(row0 + 2 * p, nrows - 1), rb = min(row0 + 2 * p + 1, nrows - 1);
+        xp[t].x = xo[i + (size_t)ra * nx];
+        xp[t].y = xo[i + (size_t)rb * nx];
     }
     __syncthreads();
-    run_fft<N, EMAX, false, 0, true>(lds, V.px, RB, 1, nx, twl);
+    run_fft<N, EMAX, false, 0, true, RBF>(lds, V.px, RB, 1, nx, twl);
     double sd = 0.0, sx = 0.0;
 #pragma unroll
     for (int t = 0; t < EMAX; ++t) {
@@ -533,8 +646,9 @@ __global__ __launch_bounds__(kThreads) void k_c2r_tv(const VolDesc* __restrict__
     const double tsd = block_sum(sd, sh);
     const double tsx = block_sum(sx, sh);
     if (threadIdx.x == 0) {
-        V.partial[2 * lb] = tsd;
-        V.partial[2 * lb + 1] = tsx;
+        QSM_GLOBAL double* part = glb(V.partial);
+        part[2 * lb] = tsd;
+        part[2 * lb + 1] = tsx;
     }
 }
 
@@ -568,7 +682,7 @@ __global__ __launch_bounds__(kThreads) void k_z_vs(const VolDesc* __restrict__ v
     const size_t zs = (size_t)nxh * V.ny;
     const size_t base = (size_t)ky * nxh;
     const int total = nz * kTile;
-    float2* buf = V.spec[OP == ZOP_VS_FWD ? sel : SPEC_WORK] + base;
+    QSM_GLOBAL float2* buf = glb(V.spec[OP == ZOP_VS_FWD ? sel : SPEC_WORK]) + base;
     load_twiddles(twl, tw + V.pz.tw_off, nz);
     col_load<EMAX>(lds, buf, zs, nz, kx0, nxh);
     __syncthreads();
@@ -578,8 +692,9 @@ __global__ __launch_bounds__(kThreads) void k_z_vs(const VolDesc* __restrict__ v
     } else if constexpr (OP == ZOP_VS_RADIUS) {
         const float inv_count = (float)(1.0 / V.vs_counts[ridx]);
         const float invN = (float)(1.0 / ((double)V.nx * V.ny * V.nz));
-        const float2* __restrict__ FH = V.spec[SPEC_A] + base;
-        const float2* __restrict__ MH = V.spec[SPEC_B] + base;
+        const QSM_GLOBAL float2* __restrict__ FH = glb(V.spec[SPEC_A]) + base;
+        const QSM_GLOBAL float2* __restrict__ MH = glb(V.spec[SPEC_B]) + base;
+        QSM_GLOBAL float* sfirst = glb(V.vs_sfirst) + base;
         float sreg[EMAX];
         float2 fh[EMAX];
 #pragma unroll
@@ -591,9 +706,9 @@ __global__ __launch_bounds__(kThreads) void k_z_vs(const VolDesc* __restrict__ v
                 const float sv = lds[e].x * inv_count;
                 sreg[t] = sv;
                 if (kx < nxh) {
-                    if (flag) V.vs_sfirst[base + kx + kz * zs] = sv;
-                    const float2 m = MH[kx + kz * zs];
-                    fh[t] = FH[kx + kz * zs];
+                    if (flag) sfirst[kx + kz * zs] = sv;
+                    const float2 m = ldg2(MH + kx + kz * zs);
+                    fh[t] = ldg2(FH + kx + kz * zs);
                     lds[e] = make_float2(m.x * sv * invN, m.y * sv * invN);
                 } else {
                     fh[t] = make_float2(0.0f, 0.0f);
@@ -615,14 +730,14 @@ __global__ __launch_bounds__(kThreads) void k_z_vs(const VolDesc* __restrict__ v
         }
         __syncthreads();
         run_fft<N, EMAX, true, kLog2Tile, true>(lds, V.pz, kTile, kTile, 1, twl);
-        col_store<EMAX>(lds, V.spec[SPEC_C] + base, zs, nz, kx0, nxh);
+        col_store<EMAX>(lds, glb(V.spec[SPEC_C]) + base, zs, nz, kx0, nxh);
     } else {  // ZOP_VS_DECONV
         const double invN = 1.0 / ((double)V.nx * V.ny * V.nz);
         for (int e = threadIdx.x; e < total; e += kThreads) {
             const int kz = e >> kLog2Tile, c = e & (kTile - 1);
             const int kx = kx0 + c;
             if (kx < nxh) {
-                const double oms = 1.0 - (double)V.vs_sfirst[base + kx + kz * zs];
+                const double oms = 1.0 - (double)glb(V.vs_sfirst)[base + kx + kz * zs];
                 const float2 xv = lds[e];
                 float2 y;
                 if (fabs(oms) < V.vs_threshold) {
@@ -669,7 +784,9 @@ __global__ __launch_bounds__(kThreads) void k_c2r_vs_select(const VolDesc* __res
     const int nrows = V.ny * V.nz;
     const int row0 = lb * RB;
     load_twiddles(twl, tw + V.px.tw_off, nx);
-    c2r_load<EMAX>(lds, nx, nxh, RB, V.spec[SPEC_WORK], V.spec[SPEC_C], row0, 1, nrows);
+    c2r_load<EMAX>(lds, nx, nxh, RB, glb(V.spec[SPEC_WORK]), glb(V.spec[SPEC_C]), row0, 1, nrows);
+    QSM_GLOBAL uint8_t* fin = glb(V.vs_final);
+    QSM_GLOBAL float* loc = glb(V.vs_local);
     __syncthreads();
     run_fft<N, EMAX, false, 0, true>(lds, V.px, RB, 1, nx, twl);
     const double count = V.vs_counts[ridx];
@@ -686,9 +803,9 @@ __global__ __launch_bounds__(kThreads) void k_c2r_vs_select(const VolDesc* __res
             const size_t idx = i + (size_t)r * nx;
             const float2 z = lds[e];
             const float k = rintf(z.x * countf);
-            if ((double)k * norm > delta && !V.vs_final[idx]) {
-                V.vs_local[idx] = z.y;
-                V.vs_final[idx] = 1;
+            if ((double)k * norm > delta && !fin[idx]) {
+                loc[idx] = z.y;
+                fin[idx] = 1;
             }
         }
     }
@@ -711,12 +828,15 @@ __global__ __launch_bounds__(kThreads) void k_c2r_vs_out(const VolDesc* __restri
     const int RB = N > 0 ? qsmfft::fixed_rows(N) : V.rows_x;
     const int nrows = V.ny * V.nz;
     const int row0 = lb * RB * 2;
-    const float2* s = V.spec[SPEC_WORK];
+    const QSM_GLOBAL float2* s = glb(V.spec[SPEC_WORK]);
     load_twiddles(twl, tw + V.px.tw_off, nx);
     c2r_load<EMAX>(lds, nx, nxh, RB, s, s, row0, 2, nrows);
     __syncthreads();
     run_fft<N, EMAX, false, 0, true>(lds, V.px, RB, 1, nx, twl);
     const int total = RB * nx;
+    const QSM_GLOBAL uint8_t* fin = glb(V.vs_final);
+    QSM_GLOBAL double* out = glb(V.out);
+    QSM_GLOBAL uint8_t* mout = glb(V.mask_out);
 #pragma unroll
     for (int t = 0; t < EMAX; ++t) {
         const int e = threadIdx.x + t * kThreads;
@@ -727,42 +847,17 @@ __global__ __launch_bounds__(kThreads) void k_c2r_vs_out(const VolDesc* __restri
             const float2 z = lds[e];
             if (ra < nrows) {
                 const size_t idx = i + (size_t)ra * nx;
-                const uint8_t fm = V.vs_final[idx];
-                V.out[idx] = fm == 1 ? (double)z.x : 0.0;
-                V.mask_out[idx] = fm;
+                const uint8_t fm = fin[idx];
+                out[idx] = fm == 1 ? (double)z.x : 0.0;
+                mout[idx] = fm;
             }
             if (rb < nrows) {
                 const size_t idx = i + (size_t)rb * nx;
-                const uint8_t fm = V.vs_final[idx];
-                V.out[idx] = fm == 1 ? (double)z.y : 0.0;
-                V.mask_out[idx] = fm;
+                const uint8_t fm = fin[idx];
+                out[idx] = fm == 1 ? (double)z.y : 0.0;
+                mout[idx] = fm;
             }
         }
-    }
-}
-
-// ---------------------------------------------------------- launching ----
-template <int N, int EG>
-void launch_kind(int kind, const LaunchTable& lt, const PassArgs& a) {
-    const dim3 g(lt.start[lt.nvol]), b(kThreads);
-    hipStream_t st = a.stream;
-    switch (kind) {
-        case P_R2C_TVDIV: k_r2c_x<N, EG, SRC_TVDIV><<<g, b, 0, st>>>(a.vols, lt, a.tw, a.sel, a.parity, a.status); break;
-        case P_R2C_F64: k_r2c_x<N, EG, SRC_F64><<<g, b, 0, st>>>(a.vols, lt, a.tw, a.sel, 0, a.status); break;
-        case P_R2C_U8: k_r2c_x<N, EG, SRC_U8><<<g, b, 0, st>>>(a.vols, lt, a.tw, a.sel, 0, a.status); break;
-        case P_R2C_F32: k_r2c_x<N, EG, SRC_F32><<<g, b, 0, st>>>(a.vols, lt, a.tw, a.sel, 0, a.status); break;
-        case P_R2C_SPHERE: k_r2c_x<N, EG, SRC_SPHERE><<<g, b, 0, st>>>(a.vols, lt, a.tw, a.sel, a.ridx, a.status); break;
-        case P_Y_FWD: k_y_pass<N, EG, false><<<g, b, 0, st>>>(a.vols, lt, a.tw, a.sel, a.sel2, a.status); break;
-        case P_Y_INV: k_y_pass<N, EG, true><<<g, b, 0, st>>>(a.vols, lt, a.tw, a.sel, a.sel2, a.status); break;
-        case P_Z_TV_SETUP: k_z_tv<N, EG, ZOP_TV_SETUP><<<g, b, 0, st>>>(a.vols, lt, a.tw, a.status); break;
-        case P_Z_TV_SOLVE: k_z_tv<N, EG, ZOP_TV_SOLVE><<<g, b, 0, st>>>(a.vols, lt, a.tw, a.status); break;
-        case P_Z_VS_FWD: k_z_vs<N, EG, ZOP_VS_FWD><<<g, b, 0, st>>>(a.vols, lt, a.tw, a.sel, 0, 0); break;
-        case P_Z_VS_RADIUS: k_z_vs<N, EG, ZOP_VS_RADIUS><<<g, b, 0, st>>>(a.vols, lt, a.tw, 0, a.ridx, a.first); break;
-        case P_Z_VS_DECONV: k_z_vs<N, EG, ZOP_VS_DECONV><<<g, b, 0, st>>>(a.vols, lt, a.tw, 0, 0, a.divide); break;
-        case P_C2R_TV: k_c2r_tv<N, EG><<<g, b, 0, st>>>(a.vols, lt, a.tw, a.parity, a.status); break;
-        case P_C2R_VS_SELECT: k_c2r_vs_select<N, EG><<<g, b, 0, st>>>(a.vols, lt, a.tw, a.ridx, a.delta); break;
-        case P_C2R_VS_OUT: k_c2r_vs_out<N, EG><<<g, b, 0, st>>>(a.vols, lt, a.tw); break;
-        default: break;
     }
 }
 

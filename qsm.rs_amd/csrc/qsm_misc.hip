@@ -1,40 +1,18 @@
-// Small kernels outside the FFT passes: the FP64 relative-change reduction,
-// masked f64 output, fills, and the pass dispatcher over specialised lengths.
+// Small kernels outside the FFT passes: the closing relative-change
+// reduction, masked f64 output, fills, and the pass dispatcher over
+// specialised lengths.
 #include "qsm_kernels.hpp"
 
 namespace qsm {
 
-// relative_change = ||x - x_prev|| / (||x|| + 1e-20); converged when < tol
-// (src/inversion/admm.rs:117-119, src/utils/mod.rs:69-77).  One block per
-// volume, fixed summation order, so repeated runs give identical results.
+// Relative change of the last iteration of a run (the earlier ones are
+// reduced by the next iteration's first pass).  One block per volume.
 __global__ __launch_bounds__(kThreads) void k_tv_reduce(const VolDesc* __restrict__ vols, int nvol,
-                                                        VolStatus* __restrict__ status,
-                                                        VolStatus* __restrict__ mirror) {
+                                                        VolStatus* status, VolStatus* mirror) {
     __shared__ double sh[kThreads / 64];
     const int v = blockIdx.x;
-    if (v >= nvol) return;
-    const VolDesc& V = vols[v];
-    if (status[v].done) return;
-    double sd = 0.0, sx = 0.0;
-    for (int b = threadIdx.x; b < V.npartial; b += kThreads) {
-        sd += V.partial[2 * b];
-        sx += V.partial[2 * b + 1];
-    }
-    const double tsd = block_sum(sd, sh);
-    const double tsx = block_sum(sx, sh);
-    if (threadIdx.x == 0) {
-        const double rel = sqrt(tsd) / (sqrt(tsx) + 1e-20);
-        VolStatus st = status[v];
-        st.iters += 1;
-        st.rel = rel;
-        if (rel < V.tol) st.done = 1;
-        status[v] = st;
-        if (mirror) {
-            // host-visible copy, read by the host loop after an event wait
-            mirror[v] = st;
-            __threadfence_system();
-        }
-    }
+    if (v >= nvol || status[v].done) return;
+    tv_reduce_block(vols[v], v, status, mirror, sh);
 }
 
 // chi = x where mask != 0, else 0 (src/utils/mask.rs:47-53), widened to f64.

@@ -15,7 +15,6 @@
 
 namespace qsm {
 
-void launch_tv_reduce(hipStream_t st, const VolDesc* d, int nvol, VolStatus* status, VolStatus* mirror);
 void launch_tv_output(hipStream_t st, const VolDesc* d, const LaunchTable& lt, const VolStatus* status);
 void launch_fill_zero(hipStream_t st, float* p, size_t n);
 void launch_status_reset(hipStream_t st, VolStatus* s, int nvol);
@@ -115,7 +114,7 @@ int validate_grid(const qsm_grid* g) {
 
 // Fill the geometry part of a descriptor and the block counts of each pass.
 struct PassBlocks {
-    int xpass, ypass, zpass, out;
+    int xpass, cpass, ypass, zpass, out;
 };
 
 PassBlocks fill_geometry(VolDesc& V, const qsm_grid& g, TwiddleBank& tb) {
@@ -123,6 +122,7 @@ PassBlocks fill_geometry(VolDesc& V, const qsm_grid& g, TwiddleBank& tb) {
     V.nxh = V.nx / 2 + 1;
     int rb = (qsmfft::kThreads * 16) / V.nx;
     V.rows_x = std::max(1, std::min(32, rb));
+    V.rows_c = std::max(1, V.rows_x / 2);
     V.ntx = (V.nxh + kTile - 1) / kTile;
     V.m_nx = qsmfft::magic_for((uint32_t)V.nx);
     V.m_nxh = qsmfft::magic_for((uint32_t)V.nxh);
@@ -135,6 +135,7 @@ PassBlocks fill_geometry(VolDesc& V, const qsm_grid& g, TwiddleBank& tb) {
     PassBlocks pb;
     const int npairs = (V.ny * V.nz + 1) / 2;
     pb.xpass = (npairs + V.rows_x - 1) / V.rows_x;
+    pb.cpass = (npairs + V.rows_c - 1) / V.rows_c;
     pb.ypass = V.ntx * V.nz;
     pb.zpass = V.ntx * V.ny;
     const size_t N = (size_t)V.nx * V.ny * V.nz;
@@ -150,6 +151,8 @@ struct PassGroup {
     LaunchTable lt;
 };
 using Groups = std::vector<PassGroup>;
+
+void launch_tv_reduce(hipStream_t st, const VolDesc* d, int nvol, VolStatus* status, VolStatus* mirror);
 
 Groups make_groups(const std::vector<int>& len, const std::vector<int>& blocks, bool column) {
     Groups gs;
@@ -202,7 +205,7 @@ struct qsm_tv_batch {
     VolStatus* h_status = nullptr;   // pinned mirror written by the reduce kernel
     VolStatus* h_status_dev = nullptr;
     float2* d_tw = nullptr;
-    Groups gx, gy, gz;              // x / y / z passes grouped by axis length
+    Groups gx, gxc, gy, gz;         // x / TV C2R / y / z passes grouped by axis length
     LaunchTable lt_out{};
     std::vector<void*> allocs;
     hipStream_t stream = nullptr;
@@ -214,6 +217,8 @@ struct qsm_tv_batch {
         a.vols = d_vols;
         a.tw = d_tw;
         a.status = status;
+        a.status_rw = nullptr;
+        a.mirror = nullptr;
         a.sel = SPEC_WORK;
         a.sel2 = -1;
         return a;
@@ -253,7 +258,7 @@ static int tv_batch_create_impl(const qsm_tv_volume* vols, size_t n_vols, const 
     B->params = *params;
     B->h.resize(n_vols);
     TwiddleBank tb;
-    std::vector<int> lx, ly, lz, bx, by, bz, bo;
+    std::vector<int> lx, ly, lz, bx, bc, by, bz, bo;
     auto cleanup = [&](int rc) { batch_free(B); return rc; };
     for (size_t v = 0; v < n_vols; ++v) {
         VolDesc& V = B->h[v];
@@ -268,11 +273,11 @@ static int tv_batch_create_impl(const qsm_tv_volume* vols, size_t n_vols, const 
         V.field = vols[v].d_field;
         V.mask = vols[v].d_mask;
         V.out = vols[v].d_chi;
-        lx.push_back(V.nx); bx.push_back(pb.xpass);
+        lx.push_back(V.nx); bx.push_back(pb.xpass); bc.push_back(pb.cpass);
         ly.push_back(V.ny); by.push_back(pb.ypass);
         lz.push_back(V.nz); bz.push_back(pb.zpass);
         bo.push_back(pb.out);
-        V.npartial = pb.xpass;
+        V.npartial = pb.cpass;
         const size_t N = (size_t)V.nx * V.ny * V.nz;
         const size_t Nh = (size_t)V.nxh * V.ny * V.nz;
         int rc;
@@ -288,10 +293,11 @@ static int tv_batch_create_impl(const qsm_tv_volume* vols, size_t n_vols, const 
         }
         if ((rc = dev_alloc(&V.q, Nh))) return cleanup(rc);
         B->allocs.push_back(V.q);
-        if ((rc = dev_alloc(&V.partial, 2 * (size_t)pb.xpass))) return cleanup(rc);
+        if ((rc = dev_alloc(&V.partial, 2 * (size_t)pb.cpass))) return cleanup(rc);
         B->allocs.push_back(V.partial);
     }
     B->gx = make_groups(lx, bx, false);
+    B->gxc = make_groups(lx, bc, false);
     B->gy = make_groups(ly, by, true);
     B->gz = make_groups(lz, bz, true);
     B->lt_out = identity_table(bo);
@@ -331,15 +337,23 @@ static int tv_batch_create_impl(const qsm_tv_volume* vols, size_t n_vols, const 
 }
 
 // One TV-ADMM iteration (src/inversion/admm.rs:76-145) for every volume.
-static int enqueue_iteration(qsm_tv_batch* B, size_t it) {
+// Its first pass also closes iteration it-1 (relative change, convergence);
+// `closed` (if set) is recorded right after that pass.
+static int enqueue_iteration(qsm_tv_batch* B, size_t it, hipEvent_t closed) {
     PassArgs a = B->args(B->d_status);
     a.parity = (int)((it - 1) & 1);
+    if (it >= 2) {
+        a.status_rw = B->d_status;
+        a.mirror = B->h_status_dev;
+    }
     run_groups(P_R2C_TVDIV, B->gx, a);
+    if (closed) QSM_HIP(hipEventRecord(closed, B->stream));
+    a.status_rw = nullptr;
+    a.mirror = nullptr;
     run_groups(P_Y_FWD, B->gy, a);
     run_groups(P_Z_TV_SOLVE, B->gz, a);
     run_groups(P_Y_INV, B->gy, a);
-    run_groups(P_C2R_TV, B->gx, a);
-    launch_tv_reduce(B->stream, B->d_vols, B->nvol, B->d_status, B->h_status_dev);
+    run_groups(P_C2R_TV, B->gxc, a);
     QSM_CHECK_LAUNCH();
     return QSM_OK;
 }
@@ -375,25 +389,32 @@ static int tv_batch_run_impl(qsm_tv_batch* B, qsm_progress_fn progress, void* us
     // relative_change >= 0 (or NaN), so with tol <= 0 no iteration can stop
     // early and the host never needs to look at the device state.
     const bool can_converge = B->params.tol > 0.0;
+    constexpr size_t W = qsm_tv_batch::kWindow;
     if (!progress && !can_converge) {
         for (size_t it = 1; it <= max_iter; ++it) {
-            int rc = enqueue_iteration(B, it);
+            int rc = enqueue_iteration(B, it, nullptr);
             if (rc) return rc;
         }
+        if (max_iter > 0) launch_tv_reduce(st, B->d_vols, B->nvol, B->d_status, B->h_status_dev);
     } else {
-        // Keep up to kWindow iterations in flight; report progress for
-        // iteration it+1 only once iteration it is known not to have
-        // converged, so the callback sequence equals the reference's.
+        // Iteration j is closed by the first pass of iteration j+1 (or by
+        // k_tv_reduce after the last one); ev[j % W] marks that point.  Keep
+        // up to W iterations in flight; report progress for iteration it+1
+        // only once iteration it is known not to have converged, so the
+        // callback sequence equals the reference's.
         size_t next = 1;
         if (max_iter > 0 && progress) progress(1, max_iter, user);
         for (size_t it = 1; it <= max_iter; ++it) {
-            while (next <= max_iter && next < it + qsm_tv_batch::kWindow) {
-                int rc = enqueue_iteration(B, next);
+            while (next <= max_iter && next <= it + W) {
+                int rc = enqueue_iteration(B, next, next >= 2 ? B->ev[(next - 1) % W] : nullptr);
                 if (rc) return rc;
-                QSM_HIP(hipEventRecord(B->ev[next % qsm_tv_batch::kWindow], st));
                 ++next;
             }
-            QSM_HIP(hipEventSynchronize(B->ev[it % qsm_tv_batch::kWindow]));
+            if (it == max_iter) {
+                launch_tv_reduce(st, B->d_vols, B->nvol, B->d_status, B->h_status_dev);
+                QSM_HIP(hipEventRecord(B->ev[it % W], st));
+            }
+            QSM_HIP(hipEventSynchronize(B->ev[it % W]));
             bool all_done = true;
             for (int v = 0; v < B->nvol; ++v) {
                 const volatile VolStatus* hs = B->h_status + v;
@@ -418,22 +439,25 @@ static int tv_batch_run_impl(qsm_tv_batch* B, qsm_progress_fn progress, void* us
 // Event-timed copy of the iteration loop: every kernel of `iters` iterations
 // is bracketed by HIP events on the batch's stream; ms_out[k] receives the
 // mean duration of kernel slot k (0 r2c+div, 1 y fwd, 2 z solve, 3 y inv,
-// 4 c2r, 5 reduce).  Runs the spectral setup first so the data are realistic.
+// 4 c2r).  The r2c slot includes the reduction of the previous iteration's
+// relative change.  Runs the spectral setup first so the data are realistic.
 static int tv_batch_profile_impl(qsm_tv_batch* B, size_t iters, double* ms_out, size_t n_out) {
-    if (!B || !ms_out || n_out < 6 || iters == 0) return fail(QSM_EPARAM, "bad profile arguments");
+    if (!B || !ms_out || n_out < 5 || iters == 0) return fail(QSM_EPARAM, "bad profile arguments");
     hipStream_t st = B->stream;
     {
         int rc = enqueue_setup(B);
         if (rc) return rc;
     }
-    hipEvent_t ev[7];
+    hipEvent_t ev[6];
     for (auto& e : ev) QSM_HIP(hipEventCreate(&e));
-    double acc[6] = {0, 0, 0, 0, 0, 0};
+    double acc[5] = {0, 0, 0, 0, 0};
     PassArgs a = B->args(nullptr);
     for (size_t it = 1; it <= iters; ++it) {
         a.parity = (int)((it - 1) & 1);
+        a.status_rw = it >= 2 ? B->d_status : nullptr;
         QSM_HIP(hipEventRecord(ev[0], st));
         run_groups(P_R2C_TVDIV, B->gx, a);
+        a.status_rw = nullptr;
         QSM_HIP(hipEventRecord(ev[1], st));
         run_groups(P_Y_FWD, B->gy, a);
         QSM_HIP(hipEventRecord(ev[2], st));
@@ -441,20 +465,18 @@ static int tv_batch_profile_impl(qsm_tv_batch* B, size_t iters, double* ms_out, 
         QSM_HIP(hipEventRecord(ev[3], st));
         run_groups(P_Y_INV, B->gy, a);
         QSM_HIP(hipEventRecord(ev[4], st));
-        run_groups(P_C2R_TV, B->gx, a);
+        run_groups(P_C2R_TV, B->gxc, a);
         QSM_HIP(hipEventRecord(ev[5], st));
-        launch_tv_reduce(st, B->d_vols, B->nvol, B->d_status, nullptr);
-        QSM_HIP(hipEventRecord(ev[6], st));
         QSM_CHECK_LAUNCH();
-        QSM_HIP(hipEventSynchronize(ev[6]));
-        for (int k = 0; k < 6; ++k) {
+        QSM_HIP(hipEventSynchronize(ev[5]));
+        for (int k = 0; k < 5; ++k) {
             float ms = 0.f;
             QSM_HIP(hipEventElapsedTime(&ms, ev[k], ev[k + 1]));
             acc[k] += ms;
         }
     }
     for (auto& e : ev) (void)hipEventDestroy(e);
-    for (int k = 0; k < 6; ++k) ms_out[k] = acc[k] / (double)iters;
+    for (int k = 0; k < 5; ++k) ms_out[k] = acc[k] / (double)iters;
     return QSM_OK;
 }
 
@@ -932,6 +954,13 @@ int qsm_memcpy_dtoh(void* dst, const void* src, size_t bytes) {
     g_err.clear();
     if ((!dst || !src) && bytes) return fail(QSM_EPARAM, "null pointer");
     if (bytes) QSM_HIP(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost));
+    return QSM_OK;
+}
+
+int qsm_memcpy_dtod(void* dst, const void* src, size_t bytes) {
+    g_err.clear();
+    if ((!dst || !src) && bytes) return fail(QSM_EPARAM, "null pointer");
+    if (bytes) QSM_HIP(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToDevice));
     return QSM_OK;
 }
 
