@@ -17,6 +17,18 @@ The JSON line also carries
                 CPU path: OpenMP over FFT axis loops only) timed on this host
                 on a bounded number of iterations, scaled to one solve.
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--batch B]
+
+Other workloads (--workload), each printing one JSON line of its own:
+  uneven32  32 volumes, each axis from {144,...,256}, voxel sizes U(0.6,1.2) mm,
+            B0 tilted U(0,20) deg, 12-ellipsoid phantoms, lambda 1e-3,
+            rho 0.1, tol 0, 200 iterations, one batch
+  chain8    8 volumes 384x384x256 at 0.6 mm: total field (phantom + 3
+            external dipoles + noise) -> batched V-SHARP (radii 12..1 mm)
+            -> batched TV-ADMM (lambda 1e-3, rho 0.1, tol 0, 200 iterations)
+            on the V-SHARP outputs, device-resident end to end
+  vjp       192x192x128 at 1 mm: K = 20 unrolled TV-ADMM iterations and the
+            vector-Jacobian product of ||chi_K - chi_true||^2 w.r.t. the
+            local field and lambda (one step = forward + backward)
 """
 from __future__ import annotations
 
@@ -80,6 +92,19 @@ def traffic_for(kernel_index, nvol, dims):
     return None
 
 
+def vsharp_alg_bytes(dims, R):
+    """Algorithmic bytes of one V-SHARP run with R radii: FFT(field),
+    FFT(mask), per radius FFT(kernel) and two C2R with the fused multiply and
+    select, then the deconvolution R2C + C2R and the mask pass:
+    (3R+4) U + R (8Nh + 18N) + 4Nh + 9N with U = 4N + 40Nh (one 3-pass
+    real transform)."""
+    nx, ny, nz = dims
+    N = nx * ny * nz
+    Nh = (nx // 2 + 1) * ny * nz
+    U = 4 * N + 40 * Nh
+    return (3 * R + 4) * U + R * (8 * Nh + 18 * N) + 4 * Nh + 9 * N
+
+
 def cpu_baseline(ph, params, iters_sample, threads):
     """Oracle TV-ADMM on a bounded number of iterations, scaled to one solve."""
     os.environ["OMP_NUM_THREADS"] = str(threads)
@@ -122,6 +147,7 @@ def main():
     ap.add_argument("--skip-plain", action="store_true",
                     help="time only the batched path (so a profiler sees batched launches only)")
     ap.add_argument("--dims", type=str, default="256,256,160")
+    ap.add_argument("--workload", choices=["tv256", "uneven32", "chain8", "vjp"], default="tv256")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
                     help="before timing, copy BYTES device-to-device once (a plain copy "
                          "kernel of known size for scaling HBM counters)")
@@ -134,6 +160,12 @@ def main():
 
     qsm = _load("qsm_rs_amd", os.path.join(PKG_DIR, "__init__.py"), [PKG_DIR])
     phantom = _load("qsm_phantom", os.path.join(PKG_DIR, "phantom.py"))
+    if a.workload == "uneven32":
+        return bench_uneven(a, qsm, phantom)
+    if a.workload == "chain8":
+        return bench_chain(a, qsm, phantom)
+    if a.workload == "vjp":
+        return bench_vjp(a, qsm, phantom)
 
     dims = tuple(int(v) for v in a.dims.split(","))
     vs = (0.9, 0.9, 1.0)
@@ -243,6 +275,164 @@ def main():
         "cpu_baseline": cpu,
     }
     print(json.dumps(out))
+
+
+def _timed(a, run, nvol, sync):
+    for _ in range(a.warmup):
+        run()
+    sync()
+    t0 = time.perf_counter()
+    for _ in range(a.steps):
+        run()
+    sync()
+    dt = time.perf_counter() - t0
+    return a.steps * nvol / dt, dt / a.steps * 1e3
+
+
+def _roofline(nbytes, ms, names):
+    dom = int(np.argmax(ms))
+    achieved = nbytes[dom] / (ms[dom] * 1e-3) / 1e9
+    return {
+        "bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+        "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None, "kernel": names[dom],
+        "kernel_ms": round(float(ms[dom]), 5), "alg_bytes_per_launch": int(nbytes[dom]),
+        "iteration_GBs": round(sum(nbytes) / (float(sum(ms)) * 1e-3) / 1e9, 1),
+        "per_kernel_ms": {k: round(float(v), 5) for k, v in zip(names, ms)},
+    }
+
+
+def bench_uneven(a, qsm, phantom):
+    """32 volumes of uneven dims / voxel sizes / B0 tilts in one TV batch."""
+    count = 32
+    shapes = phantom.uneven_batch_shapes(3002, count)
+    params = qsm.TvParams(lambda_=1e-3, rho=0.1, tol=0.0, max_iter=a.iters)
+    keep, vols, nbytes = [], [], np.zeros(len(KERNELS))
+    t0 = time.perf_counter()
+    for v, (dims, vs) in enumerate(shapes):
+        b = phantom.tilted_bdir(3002 * 1000 + v)
+        ph = phantom.make_phantom(dims, voxel_size=vs, bdir=b, n_ellipsoids=12, seed=3002 * 1000 + v)
+        tf = qsm.DeviceArray.from_numpy(ph.local_field)
+        tm = qsm.DeviceArray.from_numpy(ph.mask)
+        tc = qsm.DeviceArray(ph.local_field.size, np.float64)
+        keep.append((tf, tm, tc))
+        vols.append((qsm.Grid(dims, vs), b, tf.ptr, tm.ptr, tc.ptr))
+        nbytes += np.array(alg_bytes(dims), dtype=np.float64)
+        print(f"# volume {v + 1}/{count} {dims} ready ({time.perf_counter() - t0:.1f} s)",
+              file=sys.stderr, flush=True)
+    qsm.synchronize()
+    batch = qsm.TvBatch(vols, params)
+    value, ms_per_step = _timed(a, batch.run, count, qsm.synchronize)
+    iters_done = batch.status(0)[0]
+    ms = batch.profile(a.profile_iters)[: len(KERNELS)]
+    batch.close()
+    voxels = sum(int(np.prod(d)) for d, _ in shapes)
+    print(json.dumps({
+        "metric": "TV-ADMM solves/s, 32 uneven volumes in one batch",
+        "value": round(value, 3), "unit": "solves/s", "n_gpus": 1, "steps": a.steps,
+        "warmup": a.warmup, "ms_per_step": round(ms_per_step, 3), "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+        "config": {"workload": "32 volumes, axes from {144..256 step 16}, voxel U(0.6,1.2) mm, "
+                               "B0 tilt U(0,20) deg, 12-ellipsoid phantoms, lambda 1e-3, rho 0.1, "
+                               f"tol 0, {a.iters} iterations", "volumes": count,
+                   "total_voxels": voxels, "iterations": iters_done,
+                   "distinct_lengths": sorted({n for d, _ in shapes for n in d})},
+        "voxel_iterations_per_s": round(voxels * iters_done / (ms_per_step * 1e-3), 1),
+        "roofline": _roofline(nbytes, ms, KERNELS),
+    }))
+
+
+def bench_chain(a, qsm, phantom):
+    """V-SHARP then TV-ADMM on 8 volumes of 384x384x256, device-resident."""
+    count = 8
+    dims, vs = (384, 384, 256), (0.6, 0.6, 0.6)
+    grid = qsm.Grid(dims, vs)
+    vp = qsm.VsharpParams()          # 0.05, 12 mm, 1 mm: radii 12, 11, ..., 1
+    radii = qsm.vsharp_radii(vp)
+    params = qsm.TvParams(lambda_=1e-3, rho=0.1, tol=0.0, max_iter=a.iters)
+    keep, vvols, tvols = [], [], []
+    n = int(np.prod(dims))
+    t0 = time.perf_counter()
+    for v in range(count):
+        seed = 4004 * 1000 + v
+        ph = phantom.add_background(phantom.make_phantom(dims, voxel_size=vs, n_ellipsoids=12,
+                                                         seed=seed), seed=seed)
+        tf = qsm.DeviceArray.from_numpy(ph.total_field)
+        tm = qsm.DeviceArray.from_numpy(ph.mask)
+        tl = qsm.DeviceArray(n, np.float64)
+        te = qsm.DeviceArray(n, np.uint8)
+        tc = qsm.DeviceArray(n, np.float64)
+        keep.append((tf, tm, tl, te, tc))
+        vvols.append((grid, tf.ptr, tm.ptr, tl.ptr, te.ptr))
+        tvols.append((grid, ph.bdir, tl.ptr, te.ptr, tc.ptr))
+        print(f"# volume {v + 1}/{count} ready ({time.perf_counter() - t0:.1f} s)",
+              file=sys.stderr, flush=True)
+        del ph
+    qsm.synchronize()
+    vsb = qsm.VsharpBatch(vvols, radii, vp.threshold)
+    tvb = qsm.TvBatch(tvols, params)
+    # stage times
+    vs_vps, vs_ms = _timed(a, vsb.run, count, qsm.synchronize)
+    tv_vps, tv_ms = _timed(a, tvb.run, count, qsm.synchronize)
+
+    def chain():
+        vsb.run()
+        tvb.run()
+    value, ms_per_step = _timed(a, chain, count, qsm.synchronize)
+    ms = tvb.profile(a.profile_iters)[: len(KERNELS)]
+    vsb.close()
+    tvb.close()
+    vs_bytes = vsharp_alg_bytes(dims, len(radii)) * count
+    print(json.dumps({
+        "metric": "V-SHARP -> TV-ADMM volumes/s, 8 x 384x384x256 chained on device",
+        "value": round(value, 4), "unit": "volumes/s", "n_gpus": 1, "steps": a.steps,
+        "warmup": a.warmup, "ms_per_step": round(ms_per_step, 3), "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+        "config": {"workload": "8 x 384x384x256 @0.6 mm, total field = 12-ellipsoid phantom + 3 "
+                               "external dipoles + N(0,0.005); V-SHARP radii 12..1 mm, thr 0.05; "
+                               f"TV lambda 1e-3, rho 0.1, tol 0, {a.iters} iterations",
+                   "volumes": count, "radii": radii},
+        "vsharp": {"ms_per_batch": round(vs_ms, 3), "alg_bytes": int(vs_bytes),
+                   "alg_GBs": round(vs_bytes / (vs_ms * 1e-3) / 1e9, 1)},
+        "tv": {"ms_per_batch": round(tv_ms, 3), "solves_per_s": round(tv_vps, 3)},
+        "roofline": _roofline(np.array(alg_bytes(dims), dtype=np.float64) * count, ms, KERNELS),
+    }))
+
+
+def bench_vjp(a, qsm, phantom):
+    """K = 20 unrolled iterations + VJP at 192x192x128 (one volume)."""
+    dims, K = (192, 192, 128), 20
+    ph = phantom.make_phantom(dims, n_ellipsoids=12, seed=3003)
+    grid = qsm.Grid(dims, ph.voxel_size)
+    params = qsm.TvParams(lambda_=1e-3, rho=0.1, tol=0.0, max_iter=K)
+    n = ph.mask.size
+    tf, tm = qsm.DeviceArray.from_numpy(ph.local_field), qsm.DeviceArray.from_numpy(ph.mask)
+    tc, tfb = qsm.DeviceArray(n, np.float64), qsm.DeviceArray(n, np.float64)
+    u = qsm.TvUnrolled([(grid, ph.bdir, tf.ptr, tm.ptr, tc.ptr)], params)
+    u.forward()
+    tcb = qsm.DeviceArray.from_numpy(2.0 * (tc.numpy() - ph.chi * ph.mask))
+
+    def step():
+        u.forward()
+        u.backward([tcb.ptr], [tfb.ptr])
+    fwd_ps, fwd_ms = _timed(a, u.forward, 1, qsm.synchronize)
+    value, ms_per_step = _timed(a, step, 1, qsm.synchronize)
+    u.close()
+    N = n
+    Nh = (dims[0] // 2 + 1) * dims[1] * dims[2]
+    # backward iteration: adjoint source (c-bar 4N, s-bar 12N in, 12N out, q 3N,
+    # acc 8N) + spectrum 8Nh, y/z/y passes 60Nh, C2R 8Nh + 4N
+    bwd_bytes = K * (43 * N + 76 * Nh)
+    fwd_bytes = sum(alg_bytes(dims)) * K + 3 * N * (K - 1)
+    print(json.dumps({
+        "metric": "unrolled TV-ADMM (K=20) forward+VJP per second at 192x192x128",
+        "value": round(value, 3), "unit": "gradients/s", "n_gpus": 1, "steps": a.steps,
+        "warmup": a.warmup, "ms_per_step": round(ms_per_step, 3), "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+        "config": {"workload": "192x192x128 @1 mm, 12-ellipsoid phantom, lambda 1e-3, rho 0.1, "
+                               "K=20, loss ||chi_K - chi_true||^2", "iterations": K},
+        "forward_ms": round(fwd_ms, 3), "backward_ms": round(ms_per_step - fwd_ms, 3),
+        "alg_GBs": round((fwd_bytes + bwd_bytes) / (ms_per_step * 1e-3) / 1e9, 1),
+    }))
 
 
 if __name__ == "__main__":

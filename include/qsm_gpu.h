@@ -190,6 +190,34 @@ int qsm_memcpy_dtoh(void* dst, const void* src, size_t bytes);
 int qsm_memcpy_dtod(void* dst, const void* src, size_t bytes);
 int qsm_device_synchronize(void);
 
+/* ---- differentiable unrolled TV-ADMM (no reference counterpart) ------- */
+/* K = params->max_iter iterations of tv_admm (src/inversion/tv.rs:63-97)
+ * with tol ignored (treated as 0), over device-resident volumes as in
+ * qsm_tv_batch_create; then the vector-Jacobian product of chi = mask * x_K
+ * with respect to the local field and lambda.  The forward keeps, per
+ * iteration, one byte per gradient component (the soft-threshold pattern);
+ * the backward runs one adjoint iteration per forward iteration. */
+typedef struct qsm_tv_unrolled qsm_tv_unrolled;
+int qsm_tv_unrolled_create(const qsm_tv_volume* vols, size_t n_vols, const qsm_tv_params* params,
+                           void* stream, qsm_tv_unrolled** out);
+/* Runs the K iterations and writes each volume's d_chi. */
+int qsm_tv_unrolled_forward(qsm_tv_unrolled* h);
+/* After a forward: given d_chi_bar[v] = dL/dchi (N f64, device), writes
+ * d_field_bar[v] = dL/d(local field) (N f64, device) and, if lambda_bar is
+ * non-null, lambda_bar[v] = dL/dlambda (host). */
+int qsm_tv_unrolled_backward(qsm_tv_unrolled* h, const double* const* d_chi_bar,
+                             double* const* d_field_bar, double* lambda_bar);
+void qsm_tv_unrolled_destroy(qsm_tv_unrolled* h);
+
+/* Dipole forward operator field = Re IFFT(D FFT(chi)) (the forward model of
+ * tests/chisep_integration.rs:18-62 with the kernel of
+ * src/kernels/dipole.rs:24-72) and its adjoint, device pointers (N f64),
+ * FP32 transforms; stream may be null.  The operator is self-adjoint. */
+int qsm_dipole_forward(const double* d_chi, const qsm_grid* grid, const double bdir[3], double* d_field,
+                       void* stream);
+int qsm_dipole_adjoint(const double* d_field, const qsm_grid* grid, const double bdir[3], double* d_chi,
+                       void* stream);
+
 /* ---- device-resident batched V-SHARP (no reference counterpart) ------- */
 
 typedef struct qsm_vsharp_volume {

@@ -59,6 +59,8 @@ def lib():
             "orc_tv_admm": (None, [_D, _U8] + [_SZ] * 3 + [_F] * 9 + [_SZ, PROGRESS_FN,
                                    C.c_void_p, _D, C.POINTER(_SZ), _D]),
             "orc_tikhonov_gradient": (None, [_D, _U8] + [_SZ] * 3 + [_F] * 7 + [_D]),
+            "orc_tv_unrolled_vjp": (None, [_D, _U8] + [_SZ] * 3 + [_F] * 8 + [_SZ, _D, _D, _D,
+                                                                            C.POINTER(_F)]),
             "orc_vsharp_radii": (_SZ, [_F, _F, _D, _SZ]),
             "orc_vsharp_with_radii": (None, [_D, _U8] + [_SZ] * 3 + [_F] * 3 +
                                       [_D, _SZ, _F, PROGRESS_FN, C.c_void_p, _D, _U8]),
@@ -220,6 +222,20 @@ def tv_admm(local_field, mask, dims, vs, bdir, lam, rho, tol, max_iter, progress
     if return_info:
         return out, int(iters.value), float(rel.value)
     return out
+
+
+def tv_unrolled_vjp(local_field, mask, dims, vs, bdir, lam, rho, K, chi_bar):
+    """VJP of K unrolled tv_admm iterations (tol = 0): returns (chi,
+    field_bar, lambda_bar) for chi_bar = dL/dchi."""
+    f = _f64(local_field)
+    m = _mask(mask)
+    cb = _f64(chi_bar)
+    chi = np.empty(f.size)
+    fb = np.empty(f.size)
+    lb = _F(0.0)
+    lib().orc_tv_unrolled_vjp(_d(f), _u8(m), *dims, *vs, *bdir, lam, rho, K, _d(cb), _d(chi),
+                              _d(fb), C.byref(lb))
+    return chi, fb, float(lb.value)
 
 
 def tikhonov_gradient(local_field, mask, dims, vs, bdir, lam):

@@ -453,6 +453,97 @@ void orc_tv_admm(const double* local_field, const uint8_t* mask, size_t nx,
     if (relchange_out) *relchange_out = rel;
 }
 
+// Adjoint of the unrolled loop of orc_tv_admm (tol = 0).  Iteration j:
+//   c_j = bdiv(g_{j-1}), g = z - u;  x_j = Re IFFT(f_hat - rho FFT(c_j) inv_a)
+//   v_j = fgrad(x_j) + u_{j-1};  z_j = shrink(v_j, t);  u_j = v_j - z_j
+// (src/inversion/admm.rs:97-142).  With q_j = sign(v_j) [|v_j| > t],
+// A = Re IFFT(D inv_a FFT .) and M = Re IFFT(inv_a FFT .) (both symmetric
+// for real even multipliers) and c-bar_j = -rho M x-bar_j:
+//   x-bar_K = mask chi-bar,  s-bar_K = 0
+//   z-bar_j = -fgrad(c-bar_{j+1}),  u-bar_j = s-bar_{j+1} + fgrad(c-bar_{j+1})
+//   s-bar_j = |q_j| z-bar_j + (1 - |q_j|) u-bar_j,  x-bar_j = -bdiv(s-bar_j)
+//   t-bar  += sum (u-bar_j - z-bar_j) q_j
+//   field-bar = A sum_j x-bar_j,  lambda-bar = t-bar / rho.
+void orc_tv_unrolled_vjp(const double* local_field, const uint8_t* mask, size_t nx,
+                         size_t ny, size_t nz, double vsx, double vsy, double vsz,
+                         double bx, double by, double bz, double lambda, double rho,
+                         size_t K, const double* chi_bar, double* chi_out,
+                         double* field_bar, double* lambda_bar) {
+    const size_t n = nx * ny * nz;
+    std::vector<double> inv_a(n), d(n);
+    std::vector<cd> f_hat(n);
+    orc_prepare_admm_spectral(local_field, nx, ny, nz, vsx, vsy, vsz, bx, by, bz, rho,
+                              inv_a.data(), reinterpret_cast<double*>(f_hat.data()));
+    orc_dipole_kernel(nx, ny, nz, vsx, vsy, vsz, bx, by, bz, d.data());
+    // forward, recording q_j for j = 1..K-1
+    std::vector<double> x(n, 0.0), ux(n, 0.0), uy(n, 0.0), uz(n, 0.0), gx(n, 0.0), gy(n, 0.0),
+        gz(n, 0.0), div(n, 0.0);
+    std::vector<cd> work(n);
+    std::vector<std::vector<int8_t>> q(K > 1 ? K - 1 : 0, std::vector<int8_t>(3 * n));
+    const double t = lambda / rho;
+    auto sgn = [t](double v) -> int8_t { return v > t ? 1 : (v < -t ? -1 : 0); };
+    for (size_t j = 1; j <= K; ++j) {
+        orc_bdiv(div.data(), gx.data(), gy.data(), gz.data(), nx, ny, nz, vsx, vsy, vsz);
+        for (size_t i = 0; i < n; ++i) work[i] = cd(div[i], 0.0);
+        fft3d(work.data(), nx, ny, nz, false);
+        for (size_t i = 0; i < n; ++i) work[i] = f_hat[i] - rho * work[i] * inv_a[i];
+        fft3d(work.data(), nx, ny, nz, true);
+        for (size_t i = 0; i < n; ++i) x[i] = work[i].real();
+        if (j == K) break;
+        orc_fgrad(gx.data(), gy.data(), gz.data(), x.data(), nx, ny, nz, vsx, vsy, vsz);
+        int8_t* qj = q[j - 1].data();
+        for (size_t i = 0; i < n; ++i) {
+            double vx = gx[i] + ux[i], vy = gy[i] + uy[i], vz = gz[i] + uz[i];
+            qj[i] = sgn(vx); qj[n + i] = sgn(vy); qj[2 * n + i] = sgn(vz);
+            double zx = orc_shrink(vx, t), zy = orc_shrink(vy, t), zz = orc_shrink(vz, t);
+            ux[i] = vx - zx; uy[i] = vy - zy; uz[i] = vz - zz;
+            gx[i] = 2.0 * zx - vx; gy[i] = 2.0 * zy - vy; gz[i] = 2.0 * zz - vz;
+        }
+    }
+    for (size_t i = 0; i < n; ++i) chi_out[i] = (K > 0 && mask[i]) ? x[i] : 0.0;
+    // backward
+    std::vector<double> xb(n), acc(n, 0.0), cb(n), sx(n, 0.0), sy(n, 0.0), sz(n, 0.0), cgx(n),
+        cgy(n), cgz(n);
+    double tbar = 0.0;
+    auto apply = [&](const std::vector<double>& in, const double* mult, const double* mult2,
+                     double scale, std::vector<double>& out) {
+        for (size_t i = 0; i < n; ++i) work[i] = cd(in[i], 0.0);
+        fft3d(work.data(), nx, ny, nz, false);
+        for (size_t i = 0; i < n; ++i) work[i] *= scale * mult[i] * (mult2 ? mult2[i] : 1.0);
+        fft3d(work.data(), nx, ny, nz, true);
+        for (size_t i = 0; i < n; ++i) out[i] = work[i].real();
+    };
+    for (size_t j = K; j >= 1; --j) {
+        if (j == K) {
+            for (size_t i = 0; i < n; ++i) xb[i] = mask[i] ? chi_bar[i] : 0.0;
+        } else {
+            orc_fgrad(cgx.data(), cgy.data(), cgz.data(), cb.data(), nx, ny, nz, vsx, vsy, vsz);
+            const int8_t* qj = q[j - 1].data();
+            for (size_t i = 0; i < n; ++i) {
+                const double g3[3] = {cgx[i], cgy[i], cgz[i]};
+                double* s3[3] = {&sx[i], &sy[i], &sz[i]};
+                for (int a = 0; a < 3; ++a) {
+                    const double zb = -g3[a], ub = *s3[a] + g3[a];
+                    const int qa = qj[a * n + i];
+                    tbar += (ub - zb) * (double)qa;
+                    *s3[a] = qa != 0 ? zb : ub;
+                }
+            }
+            orc_bdiv(xb.data(), sx.data(), sy.data(), sz.data(), nx, ny, nz, vsx, vsy, vsz);
+            for (size_t i = 0; i < n; ++i) xb[i] = -xb[i];
+        }
+        for (size_t i = 0; i < n; ++i) acc[i] += xb[i];
+        if (j > 1) apply(xb, inv_a.data(), nullptr, -rho, cb);
+    }
+    if (K > 0) {
+        apply(acc, d.data(), inv_a.data(), 1.0, xb);
+        std::memcpy(field_bar, xb.data(), n * sizeof(double));
+    } else {
+        std::fill(field_bar, field_bar + n, 0.0);
+    }
+    if (lambda_bar) *lambda_bar = tbar / rho;
+}
+
 // src/inversion/tikhonov.rs:62-131 (Regularization::Gradient arm)
 void orc_tikhonov_gradient(const double* local_field, const uint8_t* mask,
                            size_t nx, size_t ny, size_t nz, double vsx,

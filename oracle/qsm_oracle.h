@@ -75,6 +75,18 @@ void orc_tv_admm(const double* local_field, const uint8_t* mask, size_t nx,
                  void* user, double* chi_out, size_t* iters_out,
                  double* relchange_out);
 
+/* Vector-Jacobian product of K unrolled tv_admm iterations (tol = 0; the
+ * forward is orc_tv_admm's loop, src/inversion/admm.rs:76-145): given
+ * chi_bar = dL/dchi for chi = mask * x_K, writes chi (the forward output),
+ * field_bar = dL/d(local_field) and *lambda_bar = dL/dlambda.  Derived by
+ * hand from the forward (no reference counterpart); pinned by central
+ * differences of orc_tv_admm in the tests. */
+void orc_tv_unrolled_vjp(const double* local_field, const uint8_t* mask, size_t nx,
+                         size_t ny, size_t nz, double vsx, double vsy, double vsz,
+                         double bx, double by, double bz, double lambda, double rho,
+                         size_t K, const double* chi_bar, double* chi_out,
+                         double* field_bar, double* lambda_bar);
+
 /* src/inversion/tikhonov.rs:62-131 with Regularization::Gradient. */
 void orc_tikhonov_gradient(const double* local_field, const uint8_t* mask,
                            size_t nx, size_t ny, size_t nz, double vsx,

@@ -76,6 +76,11 @@ class _CTvVolume(C.Structure):
                 ("d_mask", C.c_void_p), ("d_chi", C.c_void_p)]
 
 
+class _CVsharpVolume(C.Structure):
+    _fields_ = [("grid", _CGrid), ("d_field", C.c_void_p), ("d_mask", C.c_void_p),
+                ("d_local", C.c_void_p), ("d_mask_out", C.c_void_p)]
+
+
 _D = C.POINTER(C.c_double)
 _U8 = C.POINTER(C.c_uint8)
 _SZ = C.c_size_t
@@ -104,6 +109,18 @@ _SIGS = {
     "qsm_tv_batch_status": (C.c_int, [C.c_void_p, _SZ, C.POINTER(_SZ), _D]),
     "qsm_tv_batch_destroy": (None, [C.c_void_p]),
     "qsm_tv_batch_profile": (C.c_int, [C.c_void_p, _SZ, _D, _SZ]),
+    "qsm_vsharp_batch_create": (C.c_int, [C.POINTER(_CVsharpVolume), _SZ, _D, _SZ, C.c_double,
+                                          C.c_void_p, C.POINTER(C.c_void_p)]),
+    "qsm_vsharp_batch_run": (C.c_int, [C.c_void_p, PROGRESS_FN, C.c_void_p]),
+    "qsm_vsharp_batch_destroy": (None, [C.c_void_p]),
+    "qsm_tv_unrolled_create": (C.c_int, [C.POINTER(_CTvVolume), _SZ, C.POINTER(_CTvParams),
+                                         C.c_void_p, C.POINTER(C.c_void_p)]),
+    "qsm_tv_unrolled_forward": (C.c_int, [C.c_void_p]),
+    "qsm_tv_unrolled_backward": (C.c_int, [C.c_void_p, C.POINTER(C.c_void_p),
+                                           C.POINTER(C.c_void_p), _D]),
+    "qsm_tv_unrolled_destroy": (None, [C.c_void_p]),
+    "qsm_dipole_forward": (C.c_int, [C.c_void_p, C.POINTER(_CGrid), _D, C.c_void_p, C.c_void_p]),
+    "qsm_dipole_adjoint": (C.c_int, [C.c_void_p, C.POINTER(_CGrid), _D, C.c_void_p, C.c_void_p]),
     "qsm_device_alloc": (C.c_int, [_SZ, C.POINTER(C.c_void_p)]),
     "qsm_device_free": (C.c_int, [C.c_void_p]),
     "qsm_memcpy_htod": (C.c_int, [C.c_void_p, C.c_void_p, _SZ]),
@@ -519,10 +536,167 @@ class TvBatch:
             pass
 
 
+class VsharpBatch:
+    """Batched V-SHARP over device-resident volumes (``qsm_vsharp_batch_*``).
+
+    ``volumes`` is a list of (grid, d_field_ptr, d_mask_ptr, d_local_ptr,
+    d_mask_out_ptr): f64 total field and u8 mask in, f64 local field and u8
+    eroded mask out, raw device addresses kept alive by the caller.  All
+    volumes share ``radii`` (mm, any order; one radius gives SHARP) and
+    ``threshold``.  The outputs can feed a TvBatch directly (the
+    run_bg_removal -> run_dipole_inversion chain without host copies)."""
+
+    def __init__(self, volumes, radii: Sequence[float], threshold: float,
+                 stream: Optional[int] = None):
+        arr = (_CVsharpVolume * len(volumes))()
+        for i, (grid, pf, pm, pl, pmo) in enumerate(volumes):
+            arr[i].grid = grid._c()
+            arr[i].d_field, arr[i].d_mask = int(pf), int(pm)
+            arr[i].d_local, arr[i].d_mask_out = int(pl), int(pmo)
+        r = np.ascontiguousarray(np.asarray(radii, dtype=np.float64))
+        h = C.c_void_p()
+        _check(_lib.qsm_vsharp_batch_create(arr, len(volumes), r.ctypes.data_as(_D), r.size,
+                                            float(threshold),
+                                            C.c_void_p(stream) if stream else None, C.byref(h)))
+        self._h = h
+        self.n = len(volumes)
+
+    def run(self, progress: Optional[Callable[[int, int], None]] = None):
+        cb = _progress(progress)
+        _check(_lib.qsm_vsharp_batch_run(self._h, cb, None))
+
+    def close(self):
+        if getattr(self, "_h", None):
+            _lib.qsm_vsharp_batch_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def _tv_volume_array(volumes):
+    arr = (_CTvVolume * len(volumes))()
+    for i, (grid, bdir, pf, pm, pc) in enumerate(volumes):
+        arr[i].grid = grid._c()
+        arr[i].bdir = (C.c_double * 3)(*[float(v) for v in bdir])
+        arr[i].d_field, arr[i].d_mask, arr[i].d_chi = int(pf), int(pm), int(pc)
+    return arr
+
+
+class TvUnrolled:
+    """K = params.max_iter unrolled TV-ADMM iterations (tol ignored) over
+    device-resident volumes, with the vector-Jacobian product of the masked
+    output chi = mask * x_K w.r.t. the local field and lambda
+    (``qsm_tv_unrolled_*``).  ``volumes`` as for TvBatch."""
+
+    def __init__(self, volumes, params: TvParams, stream: Optional[int] = None):
+        arr = _tv_volume_array(volumes)
+        self._p = params._c()
+        h = C.c_void_p()
+        _check(_lib.qsm_tv_unrolled_create(arr, len(volumes), C.byref(self._p),
+                                           C.c_void_p(stream) if stream else None, C.byref(h)))
+        self._h = h
+        self.n = len(volumes)
+
+    def forward(self):
+        _check(_lib.qsm_tv_unrolled_forward(self._h))
+
+    def backward(self, chi_bar_ptrs, field_bar_ptrs):
+        """chi_bar_ptrs / field_bar_ptrs: device addresses (N f64 each) per
+        volume; returns dL/dlambda per volume."""
+        cb = (C.c_void_p * self.n)(*[int(p) for p in chi_bar_ptrs])
+        fb = (C.c_void_p * self.n)(*[int(p) for p in field_bar_ptrs])
+        lb = np.zeros(self.n)
+        _check(_lib.qsm_tv_unrolled_backward(self._h, cb, fb, lb.ctypes.data_as(_D)))
+        return lb
+
+    def close(self):
+        if getattr(self, "_h", None):
+            _lib.qsm_tv_unrolled_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def tv_unrolled_vjp(local_field, mask, grid: Grid, bdir, params: TvParams, chi_bar):
+    """Host-array convenience over TvUnrolled for one volume: returns
+    (chi, field_bar, lambda_bar)."""
+    f, m, cbar = _flat64(local_field), _flatu8(mask), _flat64(chi_bar)
+    _check_len(grid, f, m, cbar)
+    tf, tm = DeviceArray.from_numpy(f), DeviceArray.from_numpy(m)
+    tc, tcb = DeviceArray(f.size, np.float64), DeviceArray.from_numpy(cbar)
+    tfb = DeviceArray(f.size, np.float64)
+    u = TvUnrolled([(grid, bdir, tf.ptr, tm.ptr, tc.ptr)], params)
+    try:
+        u.forward()
+        lb = u.backward([tcb.ptr], [tfb.ptr])
+    finally:
+        u.close()
+    return tc.numpy(), tfb.numpy(), float(lb[0])
+
+
+class UnrolledTvFunction:
+    """Autograd-style wrapper (forward / backward) of the unrolled solver for
+    test code: ``y = F.forward(f, lam)``, ``gf, glam = F.backward(ybar)``.
+    A torch.autograd.Function is not used because torch's bundled HIP
+    runtime and this library's would both be loaded in one process."""
+
+    def __init__(self, mask, grid: Grid, bdir, rho: float, K: int):
+        self.mask, self.grid, self.bdir, self.rho, self.K = mask, grid, bdir, rho, K
+        self._saved = None
+
+    def forward(self, local_field, lam: float):
+        self._saved = (np.array(local_field, dtype=np.float64, copy=True), float(lam))
+        p = TvParams(lambda_=lam, rho=self.rho, tol=0.0, max_iter=self.K)
+        chi, _, _ = tv_unrolled_vjp(local_field, self.mask, self.grid, self.bdir, p,
+                                    np.zeros(np.size(local_field)))
+        return chi
+
+    def backward(self, chi_bar):
+        f, lam = self._saved
+        p = TvParams(lambda_=lam, rho=self.rho, tol=0.0, max_iter=self.K)
+        _, fb, lb = tv_unrolled_vjp(f, self.mask, self.grid, self.bdir, p, chi_bar)
+        return fb, lb
+
+
+def dipole_forward(chi, grid: Grid, bdir, adjoint: bool = False):
+    """Re IFFT(D FFT(chi)) on the GPU (host arrays in and out)."""
+    c = _flat64(chi)
+    _check_len(grid, c)
+    tin, tout = DeviceArray.from_numpy(c), DeviceArray(c.size, np.float64)
+    g = grid._c()
+    b = np.ascontiguousarray(np.asarray(bdir, dtype=np.float64))
+    fn = _lib.qsm_dipole_adjoint if adjoint else _lib.qsm_dipole_forward
+    _check(fn(C.c_void_p(tin.ptr), C.byref(g), b.ctypes.data_as(_D), C.c_void_p(tout.ptr), None))
+    return tout.numpy()
+
+
+def vsharp_radii(params: VsharpParams):
+    """The radius list vsharp uses for ``params`` (src/bgremove/vsharp.rs:70-82):
+    max_radius down to min_radius in steps of min_radius, or [max_radius]
+    when that is empty.  min_radius must be > 0 (the reference would loop)."""
+    if not params.min_radius > 0:
+        raise ValueError("min_radius must be > 0 (it is the step between radii)")
+    out = []
+    r = params.max_radius
+    while r >= params.min_radius:
+        out.append(r)
+        r -= params.min_radius
+    return out or [params.max_radius]
+
+
 __all__ = [
     "Grid", "TvParams", "VsharpParams", "tv_admm", "vsharp", "vsharp_with_radii",
     "ScanMetadata", "InversionAlgorithm", "BgRemovalAlgorithm", "InversionConfig",
     "BgRemovalConfig", "BgRemovalResult", "PipelineError", "run_dipole_inversion",
     "run_bg_removal", "TvBatch", "QsmError", "backend_info", "fft_length_supported",
-    "c_library", "LIB_PATH", "DeviceArray", "synchronize",
+    "c_library", "LIB_PATH", "DeviceArray", "synchronize", "VsharpBatch", "vsharp_radii",
+    "TvUnrolled", "tv_unrolled_vjp", "UnrolledTvFunction", "dipole_forward",
 ]

@@ -68,6 +68,16 @@ struct VolDesc {
     const double* vs_radii;      // radii in processing order (device)
     const double* vs_counts;     // voxels inside each sphere (device)
     double vs_threshold;
+    // unrolled TV-ADMM adjoint (vector-Jacobian product) state
+    int8_t* ckpt;                // shrink pattern q_k in {-1,0,1} of iterations 1..K-1 (3N each)
+    float* adj_c;                // c-bar of the last adjoint step (N)
+    float* adj_s[2];             // ping-pong s-bar (3N each)
+    float* adj_acc;              // sum over iterations of x-bar (N)
+    double* adj_tpart;           // per (iteration, block) FP64 partials of t-bar
+    int adj_nblk;                // x-pass blocks per iteration (stride of adj_tpart)
+    const double* adj_chibar;    // dL/dchi (device, f64)
+    double* adj_out;             // f64 output of the C2R F64 mode (field-bar, dipole output)
+    double* adj_lbar;            // dL/dlambda (device, one double)
 };
 
 __device__ __host__ inline int find_vol(const LaunchTable& lt, int bid) {
@@ -99,6 +109,10 @@ enum PassKind : int {
     P_Y_FWD, P_Y_INV,
     P_Z_TV_SETUP, P_Z_TV_SOLVE, P_Z_VS_FWD, P_Z_VS_RADIUS, P_Z_VS_DECONV,
     P_C2R_TV, P_C2R_VS_SELECT, P_C2R_VS_OUT,
+    // unrolled TV-ADMM adjoint and the dipole operator
+    P_R2C_TVDIV_REC, P_R2C_TVADJ_FIRST, P_R2C_TVADJ, P_R2C_ACC,
+    P_Z_TV_ADJ, P_Z_TV_APPLYA, P_Z_DIPOLE,
+    P_C2R_ADJ, P_C2R_F64OUT,
     P_KIND_COUNT
 };
 
@@ -112,6 +126,7 @@ struct PassArgs {
     int sel, sel2;            // spectrum selectors (y / z passes, R2C output)
     int parity;               // TV iteration parity
     int ridx, first, divide;  // V-SHARP radius index / first radius / SHARP division
+                              // (TV record / adjoint passes: ridx = checkpoint index)
     double delta;             // erosion threshold
 };
 

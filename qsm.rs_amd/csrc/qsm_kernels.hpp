@@ -188,13 +188,20 @@ __device__ __forceinline__ void col_store(const float2* lds, QSM_GLOBAL float2* 
 }
 
 // ------------------------------------------------ x-pass source fills ----
-enum SrcKind : int { SRC_F64 = 0, SRC_U8 = 1, SRC_F32 = 2, SRC_SPHERE = 3, SRC_TVDIV = 4 };
+enum SrcKind : int {
+    SRC_F64 = 0, SRC_U8 = 1, SRC_F32 = 2, SRC_SPHERE = 3, SRC_TVDIV = 4,
+    SRC_TVDIV_REC = 5,    // SRC_TVDIV that also records the shrink pattern
+    SRC_TVADJ_FIRST = 6,  // adjoint source of the last iteration: x-bar = mask * chi-bar
+    SRC_TVADJ = 7,        // adjoint source of an earlier iteration
+    SRC_ACC = 8           // the adjoint accumulator
+};
 
 template <int SRC>
 __device__ __forceinline__ float simple_src(const VolDesc& V, int i, int r, int aux) {
     if constexpr (SRC == SRC_F64) return (float)glb(V.field)[i + (size_t)r * V.nx];
     else if constexpr (SRC == SRC_U8) return (float)glb(V.mask)[i + (size_t)r * V.nx];
     else if constexpr (SRC == SRC_F32) return glb(V.vs_local)[i + (size_t)r * V.nx];
+    else if constexpr (SRC == SRC_ACC) return glb(V.adj_acc)[i + (size_t)r * V.nx];
     else return sphere_element(V, i, r, aux);
 }
 
@@ -283,10 +290,15 @@ __device__ __forceinline__ void tvdiv_load(TvDivRound<CH>& R, const VolDesc& V, 
     }
 }
 
-template <int CH>
+__device__ __forceinline__ int8_t shrink_sign(float v, float t) {
+    return v > t ? (int8_t)1 : (v < -t ? (int8_t)-1 : (int8_t)0);
+}
+
+template <int CH, bool REC>
 __device__ __forceinline__ void tvdiv_compute(const TvDivRound<CH>& R, float2* lds, const VolDesc& V,
                                               int t0, int EMAX, int total,
-                                              QSM_GLOBAL float* __restrict__ uo) {
+                                              QSM_GLOBAL float* __restrict__ uo,
+                                              QSM_GLOBAL int8_t* __restrict__ rec) {
     const int N = V.nx * V.ny * V.nz;
     const float t = V.thr, hx = V.hx, hy = V.hy, hz = V.hz;
 #pragma unroll
@@ -308,6 +320,12 @@ __device__ __forceinline__ void tvdiv_compute(const TvDivRound<CH>& R, float2* l
                 uo[idx] = vx - zx;
                 uo[N + idx] = vy - zy;
                 uo[2 * N + idx] = vz - zz;
+                if constexpr (REC) {
+                    // d z / d v of this update, for the unrolled adjoint
+                    rec[idx] = shrink_sign(vx, t);
+                    rec[N + idx] = shrink_sign(vy, t);
+                    rec[2 * N + idx] = shrink_sign(vz, t);
+                }
             }
             const float gx = 2.0f * zx - vx, gy = 2.0f * zy - vy, gz = 2.0f * zz - vz;
             const float gxm = 2.0f * shrinkf(vxm, t) - vxm;
@@ -327,11 +345,12 @@ __device__ __forceinline__ void tvdiv_compute(const TvDivRound<CH>& R, float2* l
 // from x and u at i-1 (likewise y and z), so g never goes to memory.
 // Rounds of CH elements (2*CH voxels) are software-pipelined: the loads of
 // round r+1 are issued before round r is computed and stored.
-template <int EMAX>
+template <int EMAX, bool REC>
 __device__ __forceinline__ void fill_tvdiv(float2* lds, const VolDesc& V, int nx, int RB,
                                            const QSM_GLOBAL float* __restrict__ x,
                                            const QSM_GLOBAL float* __restrict__ u,
-                                           QSM_GLOBAL float* __restrict__ uo, int row0, int nrows) {
+                                           QSM_GLOBAL float* __restrict__ uo, int row0, int nrows,
+                                           QSM_GLOBAL int8_t* __restrict__ rec) {
     constexpr int CH = 2;
     constexpr int NR = (EMAX + CH - 1) / CH;
     const int total = RB * nx;
@@ -340,11 +359,105 @@ __device__ __forceinline__ void fill_tvdiv(float2* lds, const VolDesc& V, int nx
 #pragma unroll
     for (int r = 0; r < NR; r += 2) {
         if (r + 1 < NR) tvdiv_load<CH>(B, V, nx, (r + 1) * CH, EMAX, total, row0, nrows, x, u);
-        tvdiv_compute<CH>(A, lds, V, r * CH, EMAX, total, uo);
+        tvdiv_compute<CH, REC>(A, lds, V, r * CH, EMAX, total, uo, rec);
         if (r + 1 < NR) {
             if (r + 2 < NR) tvdiv_load<CH>(A, V, nx, (r + 2) * CH, EMAX, total, row0, nrows, x, u);
-            tvdiv_compute<CH>(B, lds, V, (r + 1) * CH, EMAX, total, uo);
+            tvdiv_compute<CH, REC>(B, lds, V, (r + 1) * CH, EMAX, total, uo, rec);
         }
+    }
+}
+
+// Adjoint of one unrolled TV-ADMM iteration, fused in front of the R2C of
+// x-bar.  Forward iteration j (tol = 0, src/inversion/admm.rs:76-145):
+//   c_j = bdiv(z_{j-1} - u_{j-1}),  x_j = A f - rho M c_j,
+//   v_j = grad(x_j) + u_{j-1},  z_j = shrink(v_j, t),  u_j = v_j - z_j,
+// with A = Re IFFT(D inv_a FFT .), M = Re IFFT(inv_a FFT .) (both
+// symmetric) and t = lambda / rho.  Given c-bar_{j+1} = -rho M x-bar_{j+1}
+// and s-bar_{j+1} (the adjoint of v_{j+1}), with q_j = sign(v_j) [|v_j| > t]:
+//   z-bar_j = -grad c-bar_{j+1},  u-bar_j = s-bar_{j+1} + grad c-bar_{j+1},
+//   s-bar_j = |q_j| z-bar_j + (1 - |q_j|) u-bar_j,
+//   x-bar_j = grad^T s-bar_j = -bdiv(s-bar_j),
+//   t-bar  += sum (u-bar_j - z-bar_j) q_j.
+// The components of s-bar at i - e_a (needed by bdiv) are recomputed from
+// c-bar, s-bar_{j+1} and q_j at i - e_a.  x-bar_j is added to the
+// accumulator (field-bar = A sum_j x-bar_j); t-bar's FP64 partial is
+// returned to the caller.  FIRST (j = K): x-bar_K = mask * chi-bar (chi = the
+// masked x_K, src/utils/mask.rs:47-53), s-bar_K = 0.
+__device__ __forceinline__ float adj_sbar(float g, float so, int8_t q) {
+    return q != 0 ? -g : so + g;
+}
+
+template <int EMAX, bool FIRST>
+__device__ __forceinline__ void fill_tvadj(float2* lds, const VolDesc& V, int nx, int RB, int row0,
+                                           int nrows, int ck, int par, double& tsum) {
+    const int ny = V.ny, nz = V.nz;
+    const int nxy = nx * ny, N = nxy * nz;
+    const float hx = V.hx, hy = V.hy, hz = V.hz;
+    const QSM_GLOBAL float* __restrict__ c = glb(V.adj_c);
+    const QSM_GLOBAL float* __restrict__ so = glb(V.adj_s[par ^ 1]);
+    QSM_GLOBAL float* __restrict__ sn = glb(V.adj_s[par]);
+    const QSM_GLOBAL int8_t* __restrict__ q = glb(V.ckpt) + (size_t)(ck < 0 ? 0 : ck) * 3 * N;
+    QSM_GLOBAL float* __restrict__ acc = glb(V.adj_acc);
+    const int total = RB * nx;
+    tsum = 0.0;
+#pragma unroll
+    for (int t = 0; t < EMAX; ++t) {
+        const int e = threadIdx.x + t * kThreads;
+        const int p = e / nx;
+        const int i = e - p * nx;
+        float d[2];
+#pragma unroll
+        for (int h = 0; h < 2; ++h) {
+            const int r = row0 + 2 * p + h;
+            const bool ok = (e < total) && (r < nrows);
+            const int rr = ok ? r : 0;
+            const int ii = ok ? i : 0;
+            const int idx = ii + rr * nx;
+            float xb;
+            if constexpr (FIRST) {
+                xb = (ok && glb(V.mask)[idx]) ? (float)glb(V.adj_chibar)[idx] : 0.0f;
+                if (ok) {
+                    sn[idx] = 0.0f;
+                    sn[N + idx] = 0.0f;
+                    sn[2 * N + idx] = 0.0f;
+                    acc[idx] = xb;
+                }
+            } else {
+                const int k = (int)fastdiv((uint32_t)rr, V.m_ny);
+                const int j = rr - k * ny;
+                const int dxp = (ii + 1 < nx) ? 1 : 1 - nx;
+                const int dxm = (ii > 0) ? -1 : nx - 1;
+                const int dyp = (j + 1 < ny) ? nx : nx - nxy;
+                const int dym = (j > 0) ? -nx : nxy - nx;
+                const int dzp = (k + 1 < nz) ? nxy : nxy - N;
+                const int dzm = (k > 0) ? -nxy : N - nxy;
+                const float cc = c[idx];
+                const float cxp = c[idx + dxp], cyp = c[idx + dyp], czp = c[idx + dzp];
+                const float cxm = c[idx + dxm], cym = c[idx + dym], czm = c[idx + dzm];
+                const float sox = so[idx], soy = so[N + idx], soz = so[2 * N + idx];
+                const float soxm = so[idx + dxm], soym = so[N + idx + dym], sozm = so[2 * N + idx + dzm];
+                const int8_t qx = q[idx], qy = q[N + idx], qz = q[2 * N + idx];
+                const int8_t qxm = q[idx + dxm], qym = q[N + idx + dym], qzm = q[2 * N + idx + dzm];
+                const float gx = (cxp - cc) * hx, gy = (cyp - cc) * hy, gz = (czp - cc) * hz;
+                const float gxm = (cc - cxm) * hx, gym = (cc - cym) * hy, gzm = (cc - czm) * hz;
+                const float sx = adj_sbar(gx, sox, qx), sy = adj_sbar(gy, soy, qy), sz = adj_sbar(gz, soz, qz);
+                const float sxm = adj_sbar(gxm, soxm, qxm), sym = adj_sbar(gym, soym, qym),
+                            szm = adj_sbar(gzm, sozm, qzm);
+                xb = -((sx - sxm) * hx + (sy - sym) * hy + (sz - szm) * hz);
+                if (ok) {
+                    sn[idx] = sx;
+                    sn[N + idx] = sy;
+                    sn[2 * N + idx] = sz;
+                    acc[idx] += xb;
+                    tsum += (double)(sox + 2.0f * gx) * (double)qx + (double)(soy + 2.0f * gy) * (double)qy +
+                            (double)(soz + 2.0f * gz) * (double)qz;
+                } else {
+                    xb = 0.0f;
+                }
+            }
+            d[h] = xb;
+        }
+        if (e < total) lds[e] = make_float2(d[0], d[1]);
     }
 }
 
@@ -353,11 +466,13 @@ __device__ __forceinline__ void fill_tvdiv(float2* lds, const VolDesc& V, int nx
 // z = a + i b; after a complex FFT of length nx the half spectra are
 //   A[k] = (Z[k] + conj Z[-k]) / 2,   B[k] = (Z[k] - conj Z[-k]) / (2i).
 // aux: TV iteration parity for SRC_TVDIV, radius index for SRC_SPHERE.
+// ck: checkpoint index (SRC_TVDIV_REC: the record written; SRC_TVADJ: the
+// record read, and the row of adj_tpart written).
 template <int N, int EG, int SRC>
 __global__ __launch_bounds__(kThreads) void k_r2c_x(const VolDesc* __restrict__ vols, LaunchTable lt,
                                                     const float2* __restrict__ tw, int out_sel, int aux,
                                                     const VolStatus* status, VolStatus* status_rw,
-                                                    VolStatus* mirror) {
+                                                    VolStatus* mirror, int ck) {
     constexpr int EMAX = x_emax(N);
     __shared__ float2 lds[x_lds(N)];
     __shared__ float2 twl[tw_lds(N)];
@@ -366,7 +481,7 @@ __global__ __launch_bounds__(kThreads) void k_r2c_x(const VolDesc* __restrict__ 
     const int gv = locate(lt, bid, lb);
     const VolDesc& V = vols[gv];
     if (status && status[gv].done) return;
-    if constexpr (SRC == SRC_TVDIV) {
+    if constexpr (SRC == SRC_TVDIV || SRC == SRC_TVDIV_REC) {
         // The first block of a volume closes the previous iteration: it
         // reduces the C2R pass's sums and may mark the volume converged.
         // Blocks of this pass that still see it running do work that is
@@ -381,11 +496,24 @@ __global__ __launch_bounds__(kThreads) void k_r2c_x(const VolDesc* __restrict__ 
     const int nrows = V.ny * V.nz;
     const int row0 = lb * RB * 2;
     load_twiddles(twl, tw + V.px.tw_off, nx);
-    if constexpr (SRC == SRC_TVDIV)
-        fill_tvdiv<EMAX>(lds, V, nx, RB, glb(V.x[aux]), glb(V.u[aux ^ 1]), glb(V.u[aux]), row0,
-                         nrows);
-    else
+    if constexpr (SRC == SRC_TVDIV) {
+        fill_tvdiv<EMAX, false>(lds, V, nx, RB, glb(V.x[aux]), glb(V.u[aux ^ 1]), glb(V.u[aux]), row0,
+                                nrows, nullptr);
+    } else if constexpr (SRC == SRC_TVDIV_REC) {
+        const size_t N3 = (size_t)3 * V.nx * V.ny * V.nz;
+        fill_tvdiv<EMAX, true>(lds, V, nx, RB, glb(V.x[aux]), glb(V.u[aux ^ 1]), glb(V.u[aux]), row0,
+                               nrows, glb(V.ckpt) + (size_t)ck * N3);
+    } else if constexpr (SRC == SRC_TVADJ_FIRST || SRC == SRC_TVADJ) {
+        __shared__ double sh[kThreads / 64];
+        double ts;
+        fill_tvadj<EMAX, SRC == SRC_TVADJ_FIRST>(lds, V, nx, RB, row0, nrows, ck, aux, ts);
+        if constexpr (SRC == SRC_TVADJ) {
+            const double bt = block_sum(ts, sh);
+            if (threadIdx.x == 0) glb(V.adj_tpart)[(size_t)ck * V.adj_nblk + lb] = bt;
+        }
+    } else {
         fill_simple<EMAX, SRC>(lds, V, nx, RB, row0, nrows, aux);
+    }
     __syncthreads();
     run_fft<N, EMAX, false, 0, false>(lds, V.px, RB, 1, nx, twl);
     QSM_GLOBAL float2* __restrict__ out = glb(V.spec[out_sel]);
@@ -460,7 +588,22 @@ __device__ inline void tv_multipliers(const VolDesc& V, int kx, int ky, int kz, 
     dinv = D * inv;
 }
 
-enum ZOp : int { ZOP_TV_SETUP = 0, ZOP_TV_SOLVE = 1, ZOP_VS_FWD = 2, ZOP_VS_RADIUS = 3, ZOP_VS_DECONV = 4 };
+// The dipole kernel alone (src/kernels/dipole.rs:24-72).
+__device__ inline double dipole_at(const VolDesc& V, int kx, int ky, int kz) {
+    const double fx = fftfreq_at(kx, V.nx, V.vsx);
+    const double fy = fftfreq_at(ky, V.ny, V.vsy);
+    const double fz = fftfreq_at(kz, V.nz, V.vsz);
+    const double kdb = fx * V.bx + fy * V.by + fz * V.bz;
+    const double k2 = fx * fx + fy * fy + fz * fz;
+    return (k2 > 1e-20) ? (1.0 / 3.0) - (kdb * kdb) / k2 : 0.0;
+}
+
+enum ZOp : int {
+    ZOP_TV_SETUP = 0, ZOP_TV_SOLVE = 1, ZOP_VS_FWD = 2, ZOP_VS_RADIUS = 3, ZOP_VS_DECONV = 4,
+    ZOP_TV_ADJ = 5,     // c-bar^ = -rho inv_a x-bar^ / N (the adjoint of the x-update in c)
+    ZOP_TV_APPLYA = 6,  // field-bar^ = D inv_a acc^ / N  (the adjoint in f, A is symmetric)
+    ZOP_DIPOLE = 7      // D chi^ / N  (the dipole forward operator, self-adjoint)
+};
 
 // TV z pass.  SETUP: forward z of FFT(f) (after x, y), then
 //   f^ = FFT(f) * D * inv_a / N  -> SPEC_A,   rho * inv_a / N -> q,
@@ -469,6 +612,8 @@ enum ZOp : int { ZOP_TV_SETUP = 0, ZOP_TV_SOLVE = 1, ZOP_VS_FWD = 2, ZOP_VS_RADI
 // part, and on Nyquist planes D(k) != D(-k) when B0 is tilted.
 // SOLVE: forward z, x^ = f^ - rho * FFT(div) * inv_a (src/inversion/admm.rs:
 // 106-108, scaled by 1/N), inverse z.
+// ADJ: forward z, -q * spectrum, inverse z.  APPLYA / DIPOLE: forward z,
+// times the Hermitian part of D inv_a / N or of D / N (in FP64), inverse z.
 template <int N, int EG, int OP>
 __global__ __launch_bounds__(kThreads) void k_z_tv(const VolDesc* __restrict__ vols, LaunchTable lt,
                                                    const float2* __restrict__ tw,
@@ -493,9 +638,10 @@ __global__ __launch_bounds__(kThreads) void k_z_tv(const VolDesc* __restrict__ v
     load_twiddles(twl, tw + V.pz.tw_off, nz);
     col_load<EMAX>(lds, buf, zs, nz, kx0, nxh);
     // x-update operands, in flight during the forward FFT
+    constexpr bool USE_Q = OP == ZOP_TV_SOLVE || OP == ZOP_TV_ADJ;
     float2 fr[OP == ZOP_TV_SOLVE ? EMAX : 1];
-    float qr[OP == ZOP_TV_SOLVE ? EMAX : 1];
-    if constexpr (OP == ZOP_TV_SOLVE) {
+    float qr[USE_Q ? EMAX : 1];
+    if constexpr (USE_Q) {
         const QSM_GLOBAL float2* __restrict__ F = glb(V.spec[SPEC_A]) + base;
         const QSM_GLOBAL float* __restrict__ Q = glb(V.q) + base;
 #pragma unroll
@@ -504,7 +650,7 @@ __global__ __launch_bounds__(kThreads) void k_z_tv(const VolDesc* __restrict__ v
             const int kz = e >> kLog2Tile;
             const int kx = kx0 + (e & (kTile - 1));
             const bool in = e < total && kx < nxh;
-            fr[t] = in ? ldg2(F + kx + kz * zs) : make_float2(0.0f, 0.0f);
+            if constexpr (OP == ZOP_TV_SOLVE) fr[t] = in ? ldg2(F + kx + kz * zs) : make_float2(0.0f, 0.0f);
             qr[t] = in ? Q[kx + kz * zs] : 0.0f;
         }
     }
@@ -530,12 +676,37 @@ __global__ __launch_bounds__(kThreads) void k_z_tv(const VolDesc* __restrict__ v
             }
         }
     } else {
-#pragma unroll
-        for (int t = 0; t < EMAX; ++t) {
-            const int e = threadIdx.x + t * kThreads;
-            if (e < total) {
+        if constexpr (OP == ZOP_TV_APPLYA || OP == ZOP_DIPOLE) {
+            const double invN = 1.0 / ((double)V.nx * V.ny * V.nz);
+            for (int e = threadIdx.x; e < total; e += kThreads) {
+                const int kz = e >> kLog2Tile, c = e & (kTile - 1);
+                const int kx = kx0 + c;
+                double m = 0.0;
+                if (kx < nxh) {
+                    const int mx = mirror_idx(kx, V.nx), my = mirror_idx(ky, V.ny), mz = mirror_idx(kz, nz);
+                    if constexpr (OP == ZOP_TV_APPLYA) {
+                        double d1, i1, d2, i2;
+                        tv_multipliers(V, kx, ky, kz, d1, i1);
+                        tv_multipliers(V, mx, my, mz, d2, i2);
+                        m = 0.5 * (d1 + d2) * invN;
+                    } else {
+                        m = 0.5 * (dipole_at(V, kx, ky, kz) + dipole_at(V, mx, my, mz)) * invN;
+                    }
+                }
                 const float2 sv = lds[e];
-                lds[e] = make_float2(fr[t].x - qr[t] * sv.x, fr[t].y - qr[t] * sv.y);
+                lds[e] = make_float2((float)((double)sv.x * m), (float)((double)sv.y * m));
+            }
+        } else {
+#pragma unroll
+            for (int t = 0; t < EMAX; ++t) {
+                const int e = threadIdx.x + t * kThreads;
+                if (e < total) {
+                    const float2 sv = lds[e];
+                    if constexpr (OP == ZOP_TV_SOLVE)
+                        lds[e] = make_float2(fr[t].x - qr[t] * sv.x, fr[t].y - qr[t] * sv.y);
+                    else
+                        lds[e] = make_float2(-qr[t] * sv.x, -qr[t] * sv.y);
+                }
             }
         }
         __syncthreads();
@@ -582,7 +753,11 @@ __device__ __forceinline__ void c2r_load(float2* lds, int nx, int nxh, int RB,
 // relative_change (src/utils/mod.rs:69-77) accumulated in FP64 per block;
 // the next pass's first block (or k_tv_reduce after the last iteration)
 // reduces them.
-template <int N, int EG>
+// MODE C2R_TV: as above.  C2R_ADJ: Re IFFT to adj_c (c-bar), nothing
+// else.  C2R_F64: Re IFFT widened to f64 into adj_out.
+enum C2RMode : int { C2R_TV = 0, C2R_ADJ = 1, C2R_F64 = 2 };
+
+template <int N, int EG, int MODE>
 __global__ __launch_bounds__(kThreads) void k_c2r_tv(const VolDesc* __restrict__ vols, LaunchTable lt,
                                                      const float2* __restrict__ tw, int parity,
                                                      const VolStatus* __restrict__ status) {
@@ -605,6 +780,30 @@ __global__ __launch_bounds__(kThreads) void k_c2r_tv(const VolDesc* __restrict__
     const QSM_GLOBAL float2* s = glb(V.spec[SPEC_WORK]);
     load_twiddles(twl, tw + V.px.tw_off, nx);
     c2r_load<EMAX>(lds, nx, nxh, RB, s, s, row0, 2, nrows);
+    if constexpr (MODE != C2R_TV) {
+        __syncthreads();
+        run_fft<N, EMAX, false, 0, true, RBF>(lds, V.px, RB, 1, nx, twl);
+#pragma unroll
+        for (int t = 0; t < EMAX; ++t) {
+            const int e = threadIdx.x + t * kThreads;
+            if (e < total) {
+                const int p = e / nx;
+                const int i = e - p * nx;
+                const int ra = row0 + 2 * p, rb = ra + 1;
+                const float2 z = lds[e];
+                if constexpr (MODE == C2R_ADJ) {
+                    QSM_GLOBAL float* __restrict__ o = glb(V.adj_c);
+                    if (ra < nrows) o[i + (size_t)ra * nx] = z.x;
+                    if (rb < nrows) o[i + (size_t)rb * nx] = z.y;
+                } else {
+                    QSM_GLOBAL double* __restrict__ o = glb(V.adj_out);
+                    if (ra < nrows) o[i + (size_t)ra * nx] = (double)z.x;
+                    if (rb < nrows) o[i + (size_t)rb * nx] = (double)z.y;
+                }
+            }
+        }
+        return;
+    }
     const QSM_GLOBAL float* __restrict__ xo = glb(V.x[parity]);
     QSM_GLOBAL float* __restrict__ xn = glb(V.x[parity ^ 1]);
     float2 xp[EMAX];  // previous iterate, in flight during the inverse FFT

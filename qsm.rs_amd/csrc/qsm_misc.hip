@@ -15,6 +15,20 @@ __global__ __launch_bounds__(kThreads) void k_tv_reduce(const VolDesc* __restric
     tv_reduce_block(vols[v], v, status, mirror, sh);
 }
 
+// dL/dlambda = t-bar / rho (t = lambda / rho), t-bar summed over the
+// (iteration, block) partials of the adjoint R2C passes in a fixed order.
+__global__ __launch_bounds__(kThreads) void k_tv_lbar(const VolDesc* __restrict__ vols, int nvol, int nrows) {
+    __shared__ double sh[kThreads / 64];
+    const int v = blockIdx.x;
+    if (v >= nvol) return;
+    const VolDesc& V = vols[v];
+    const size_t n = (size_t)(nrows > 0 ? nrows : 0) * V.adj_nblk;
+    double a = 0.0;
+    for (size_t q = threadIdx.x; q < n; q += kThreads) a += V.adj_tpart[q];
+    const double t = block_sum(a, sh);
+    if (threadIdx.x == 0) V.adj_lbar[0] = t / V.rho;
+}
+
 // chi = x where mask != 0, else 0 (src/utils/mask.rs:47-53), widened to f64.
 __global__ __launch_bounds__(kThreads) void k_tv_output(const VolDesc* __restrict__ vols, LaunchTable lt,
                                                         const VolStatus* __restrict__ status) {
@@ -60,6 +74,9 @@ __global__ void k_status_reset(VolStatus* status, int nvol) {
 
 void launch_tv_reduce(hipStream_t st, const VolDesc* d, int nvol, VolStatus* status, VolStatus* mirror) {
     k_tv_reduce<<<dim3(nvol), dim3(kThreads), 0, st>>>(d, nvol, status, mirror);
+}
+void launch_tv_lbar(hipStream_t st, const VolDesc* d, int nvol, int nrows) {
+    k_tv_lbar<<<dim3(nvol), dim3(kThreads), 0, st>>>(d, nvol, nrows);
 }
 void launch_tv_output(hipStream_t st, const VolDesc* d, const LaunchTable& lt, const VolStatus* status) {
     k_tv_output<<<dim3(lt.start[lt.nvol]), dim3(kThreads), 0, st>>>(d, lt, status);

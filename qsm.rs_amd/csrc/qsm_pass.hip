@@ -11,7 +11,9 @@ namespace qsm {
 
 static bool is_row_kind(int kind) {
     return kind == P_R2C_TVDIV || kind == P_R2C_F64 || kind == P_R2C_U8 || kind == P_R2C_F32 ||
-           kind == P_R2C_SPHERE || kind == P_C2R_TV || kind == P_C2R_VS_SELECT || kind == P_C2R_VS_OUT;
+           kind == P_R2C_SPHERE || kind == P_C2R_TV || kind == P_C2R_VS_SELECT || kind == P_C2R_VS_OUT ||
+           kind == P_R2C_TVDIV_REC || kind == P_R2C_TVADJ_FIRST || kind == P_R2C_TVADJ ||
+           kind == P_R2C_ACC || kind == P_C2R_ADJ || kind == P_C2R_F64OUT;
 }
 
 template <int N, int EG>
@@ -21,13 +23,28 @@ static void launch_rows(int kind, const LaunchTable& lt, const PassArgs& a) {
     switch (kind) {
         case P_R2C_TVDIV:
             k_r2c_x<N, EG, SRC_TVDIV><<<g, b, 0, st>>>(a.vols, lt, a.tw, a.sel, a.parity, a.status,
-                                                       a.status_rw, a.mirror);
+                                                       a.status_rw, a.mirror, 0);
             break;
-        case P_R2C_F64: k_r2c_x<N, EG, SRC_F64><<<g, b, 0, st>>>(a.vols, lt, a.tw, a.sel, 0, a.status, nullptr, nullptr); break;
-        case P_R2C_U8: k_r2c_x<N, EG, SRC_U8><<<g, b, 0, st>>>(a.vols, lt, a.tw, a.sel, 0, a.status, nullptr, nullptr); break;
-        case P_R2C_F32: k_r2c_x<N, EG, SRC_F32><<<g, b, 0, st>>>(a.vols, lt, a.tw, a.sel, 0, a.status, nullptr, nullptr); break;
-        case P_R2C_SPHERE: k_r2c_x<N, EG, SRC_SPHERE><<<g, b, 0, st>>>(a.vols, lt, a.tw, a.sel, a.ridx, a.status, nullptr, nullptr); break;
-        case P_C2R_TV: k_c2r_tv<N, EG><<<g, b, 0, st>>>(a.vols, lt, a.tw, a.parity, a.status); break;
+        case P_R2C_TVDIV_REC:
+            k_r2c_x<N, EG, SRC_TVDIV_REC><<<g, b, 0, st>>>(a.vols, lt, a.tw, a.sel, a.parity, a.status,
+                                                           a.status_rw, a.mirror, a.ridx);
+            break;
+        case P_R2C_TVADJ_FIRST:
+            k_r2c_x<N, EG, SRC_TVADJ_FIRST><<<g, b, 0, st>>>(a.vols, lt, a.tw, a.sel, a.parity, nullptr,
+                                                             nullptr, nullptr, a.ridx);
+            break;
+        case P_R2C_TVADJ:
+            k_r2c_x<N, EG, SRC_TVADJ><<<g, b, 0, st>>>(a.vols, lt, a.tw, a.sel, a.parity, nullptr, nullptr,
+                                                       nullptr, a.ridx);
+            break;
+        case P_R2C_F64: k_r2c_x<N, EG, SRC_F64><<<g, b, 0, st>>>(a.vols, lt, a.tw, a.sel, 0, a.status, nullptr, nullptr, 0); break;
+        case P_R2C_U8: k_r2c_x<N, EG, SRC_U8><<<g, b, 0, st>>>(a.vols, lt, a.tw, a.sel, 0, a.status, nullptr, nullptr, 0); break;
+        case P_R2C_F32: k_r2c_x<N, EG, SRC_F32><<<g, b, 0, st>>>(a.vols, lt, a.tw, a.sel, 0, a.status, nullptr, nullptr, 0); break;
+        case P_R2C_ACC: k_r2c_x<N, EG, SRC_ACC><<<g, b, 0, st>>>(a.vols, lt, a.tw, a.sel, 0, nullptr, nullptr, nullptr, 0); break;
+        case P_R2C_SPHERE: k_r2c_x<N, EG, SRC_SPHERE><<<g, b, 0, st>>>(a.vols, lt, a.tw, a.sel, a.ridx, a.status, nullptr, nullptr, 0); break;
+        case P_C2R_TV: k_c2r_tv<N, EG, C2R_TV><<<g, b, 0, st>>>(a.vols, lt, a.tw, a.parity, a.status); break;
+        case P_C2R_ADJ: k_c2r_tv<N, EG, C2R_ADJ><<<g, b, 0, st>>>(a.vols, lt, a.tw, 0, nullptr); break;
+        case P_C2R_F64OUT: k_c2r_tv<N, EG, C2R_F64><<<g, b, 0, st>>>(a.vols, lt, a.tw, 0, nullptr); break;
         case P_C2R_VS_SELECT: k_c2r_vs_select<N, EG><<<g, b, 0, st>>>(a.vols, lt, a.tw, a.ridx, a.delta); break;
         case P_C2R_VS_OUT: k_c2r_vs_out<N, EG><<<g, b, 0, st>>>(a.vols, lt, a.tw); break;
         default: break;
@@ -43,6 +60,9 @@ static void launch_cols(int kind, const LaunchTable& lt, const PassArgs& a) {
         case P_Y_INV: k_y_pass<N, EG, true><<<g, b, 0, st>>>(a.vols, lt, a.tw, a.sel, a.sel2, a.status); break;
         case P_Z_TV_SETUP: k_z_tv<N, EG, ZOP_TV_SETUP><<<g, b, 0, st>>>(a.vols, lt, a.tw, a.status); break;
         case P_Z_TV_SOLVE: k_z_tv<N, EG, ZOP_TV_SOLVE><<<g, b, 0, st>>>(a.vols, lt, a.tw, a.status); break;
+        case P_Z_TV_ADJ: k_z_tv<N, EG, ZOP_TV_ADJ><<<g, b, 0, st>>>(a.vols, lt, a.tw, nullptr); break;
+        case P_Z_TV_APPLYA: k_z_tv<N, EG, ZOP_TV_APPLYA><<<g, b, 0, st>>>(a.vols, lt, a.tw, nullptr); break;
+        case P_Z_DIPOLE: k_z_tv<N, EG, ZOP_DIPOLE><<<g, b, 0, st>>>(a.vols, lt, a.tw, nullptr); break;
         case P_Z_VS_FWD: k_z_vs<N, EG, ZOP_VS_FWD><<<g, b, 0, st>>>(a.vols, lt, a.tw, a.sel, 0, 0); break;
         case P_Z_VS_RADIUS: k_z_vs<N, EG, ZOP_VS_RADIUS><<<g, b, 0, st>>>(a.vols, lt, a.tw, 0, a.ridx, a.first); break;
         case P_Z_VS_DECONV: k_z_vs<N, EG, ZOP_VS_DECONV><<<g, b, 0, st>>>(a.vols, lt, a.tw, 0, 0, a.divide); break;

@@ -19,6 +19,7 @@ void launch_tv_output(hipStream_t st, const VolDesc* d, const LaunchTable& lt, c
 void launch_fill_zero(hipStream_t st, float* p, size_t n);
 void launch_status_reset(hipStream_t st, VolStatus* s, int nvol);
 void launch_vs_init(hipStream_t st, const VolDesc* d, const LaunchTable& lt);
+void launch_tv_lbar(hipStream_t st, const VolDesc* d, int nvol, int nrows);
 bool fixed_length(int n);
 
 // ------------------------------------------------------------ error state --
@@ -339,14 +340,17 @@ static int tv_batch_create_impl(const qsm_tv_volume* vols, size_t n_vols, const 
 // One TV-ADMM iteration (src/inversion/admm.rs:76-145) for every volume.
 // Its first pass also closes iteration it-1 (relative change, convergence);
 // `closed` (if set) is recorded right after that pass.
-static int enqueue_iteration(qsm_tv_batch* B, size_t it, hipEvent_t closed) {
+// rec >= 0 also records the shrink pattern of iteration it-1 as checkpoint
+// `rec` (unrolled adjoint).
+static int enqueue_iteration(qsm_tv_batch* B, size_t it, hipEvent_t closed, int rec = -1) {
     PassArgs a = B->args(B->d_status);
     a.parity = (int)((it - 1) & 1);
     if (it >= 2) {
         a.status_rw = B->d_status;
         a.mirror = B->h_status_dev;
     }
-    run_groups(P_R2C_TVDIV, B->gx, a);
+    a.ridx = rec;
+    run_groups(rec >= 0 ? P_R2C_TVDIV_REC : P_R2C_TVDIV, B->gx, a);
     if (closed) QSM_HIP(hipEventRecord(closed, B->stream));
     a.status_rw = nullptr;
     a.mirror = nullptr;
@@ -478,6 +482,211 @@ static int tv_batch_profile_impl(qsm_tv_batch* B, size_t iters, double* ms_out, 
     for (auto& e : ev) (void)hipEventDestroy(e);
     for (int k = 0; k < 5; ++k) ms_out[k] = acc[k] / (double)iters;
     return QSM_OK;
+}
+
+// ============================================ unrolled TV-ADMM adjoint ====
+// K iterations of tv_admm with tol = 0 (src/inversion/tv.rs:63-97) and the
+// vector-Jacobian product of chi = mask * x_K with respect to the local
+// field and lambda.  The forward records, for iterations 1..K-1, the shrink
+// pattern q = sign(v) [|v| > lambda/rho] (one byte per component): with it
+// the backward recursion is linear and needs no recomputation of the
+// forward iterates.
+struct qsm_tv_unrolled {
+    qsm_tv_batch* B = nullptr;
+    size_t K = 0;
+    std::vector<int> nblk;  // x-pass blocks per volume
+    std::vector<void*> allocs;
+    double* d_lbar = nullptr;
+    bool forward_done = false;
+};
+
+static void unrolled_free(qsm_tv_unrolled* U) {
+    if (!U) return;
+    if (U->B) {
+        if (U->B->stream) (void)hipStreamSynchronize(U->B->stream);
+        batch_free(U->B);
+    }
+    for (void* p : U->allocs) (void)hipFree(p);
+    delete U;
+}
+
+static int unrolled_create_impl(const qsm_tv_volume* vols, size_t n_vols, const qsm_tv_params* params,
+                                void* stream, qsm_tv_unrolled** out) {
+    if (!params || !out) return fail(QSM_EPARAM, "null argument");
+    qsm_tv_params p = *params;
+    p.tol = 0.0;  // unrolled: exactly max_iter iterations
+    qsm_tv_unrolled* U = new qsm_tv_unrolled();
+    U->K = p.max_iter;
+    auto cleanup = [&](int rc) { unrolled_free(U); return rc; };
+    int rc = tv_batch_create_impl(vols, n_vols, &p, stream, &U->B);
+    if (rc) return cleanup(rc);
+    qsm_tv_batch* B = U->B;
+    const size_t nrec = U->K > 1 ? U->K - 1 : 0;
+    if ((rc = dev_alloc(&U->d_lbar, n_vols))) return cleanup(rc);
+    U->allocs.push_back(U->d_lbar);
+    for (int v = 0; v < B->nvol; ++v) {
+        VolDesc& V = B->h[v];
+        const size_t N = (size_t)V.nx * V.ny * V.nz;
+        const int npairs = (V.ny * V.nz + 1) / 2;
+        const int nb = (npairs + V.rows_x - 1) / V.rows_x;
+        U->nblk.push_back(nb);
+        V.adj_nblk = nb;
+        if ((rc = dev_alloc(&V.ckpt, std::max<size_t>(1, nrec) * 3 * N))) return cleanup(rc);
+        U->allocs.push_back(V.ckpt);
+        if ((rc = dev_alloc(&V.adj_c, N))) return cleanup(rc);
+        U->allocs.push_back(V.adj_c);
+        for (int q = 0; q < 2; ++q) {
+            if ((rc = dev_alloc(&V.adj_s[q], 3 * N))) return cleanup(rc);
+            U->allocs.push_back(V.adj_s[q]);
+        }
+        if ((rc = dev_alloc(&V.adj_acc, N))) return cleanup(rc);
+        U->allocs.push_back(V.adj_acc);
+        if ((rc = dev_alloc(&V.adj_tpart, std::max<size_t>(1, nrec) * (size_t)nb))) return cleanup(rc);
+        U->allocs.push_back(V.adj_tpart);
+        V.adj_lbar = U->d_lbar + v;
+    }
+    hipError_t e = hipMemcpy(B->d_vols, B->h.data(), B->h.size() * sizeof(VolDesc), hipMemcpyHostToDevice);
+    if (e != hipSuccess) return cleanup(fail(QSM_EHIP, "hipMemcpy descriptors: %s", hipGetErrorString(e)));
+    *out = U;
+    return QSM_OK;
+}
+
+static int unrolled_forward_impl(qsm_tv_unrolled* U) {
+    if (!U) return fail(QSM_EPARAM, "handle is null");
+    qsm_tv_batch* B = U->B;
+    hipStream_t st = B->stream;
+    std::memset(B->h_status, 0, B->nvol * sizeof(VolStatus));
+    int rc = enqueue_setup(B);
+    if (rc) return rc;
+    for (size_t it = 1; it <= U->K; ++it) {
+        if ((rc = enqueue_iteration(B, it, nullptr, it >= 2 ? (int)(it - 2) : -1))) return rc;
+    }
+    if (U->K > 0) launch_tv_reduce(st, B->d_vols, B->nvol, B->d_status, B->h_status_dev);
+    launch_tv_output(st, B->d_vols, B->lt_out, B->d_status);
+    QSM_CHECK_LAUNCH();
+    QSM_HIP(hipStreamSynchronize(st));
+    U->forward_done = true;
+    return QSM_OK;
+}
+
+static int unrolled_backward_impl(qsm_tv_unrolled* U, const double* const* d_chi_bar,
+                                  double* const* d_field_bar, double* lambda_bar) {
+    if (!U || !d_chi_bar || !d_field_bar) return fail(QSM_EPARAM, "null argument");
+    if (!U->forward_done) return fail(QSM_EPARAM, "backward needs a forward pass first");
+    qsm_tv_batch* B = U->B;
+    hipStream_t st = B->stream;
+    for (int v = 0; v < B->nvol; ++v) {
+        if (!d_chi_bar[v] || !d_field_bar[v]) return fail(QSM_EPARAM, "volume %d has a null pointer", v);
+        B->h[v].adj_chibar = d_chi_bar[v];
+        B->h[v].adj_out = d_field_bar[v];
+    }
+    QSM_HIP(hipMemcpyAsync(B->d_vols, B->h.data(), B->h.size() * sizeof(VolDesc), hipMemcpyHostToDevice, st));
+    PassArgs a = B->args(nullptr);
+    const size_t K = U->K;
+    if (K == 0) {
+        // chi = mask * x_0 = 0 does not depend on the field or lambda
+        for (int v = 0; v < B->nvol; ++v) {
+            const VolDesc& V = B->h[v];
+            const size_t N = (size_t)V.nx * V.ny * V.nz;
+            launch_fill_zero(st, reinterpret_cast<float*>(V.adj_out), 2 * N);
+        }
+        QSM_HIP(hipStreamSynchronize(st));
+        if (lambda_bar)
+            for (int v = 0; v < B->nvol; ++v) lambda_bar[v] = 0.0;
+        return QSM_OK;
+    }
+    for (size_t j = K; j >= 1; --j) {
+        a.parity = (int)(j & 1);
+        a.ridx = (j == K) ? -1 : (int)(j - 1);
+        run_groups(j == K ? P_R2C_TVADJ_FIRST : P_R2C_TVADJ, B->gx, a);
+        if (j > 1) {  // c-bar_j feeds iteration j-1; c-bar_1 is not needed
+            run_groups(P_Y_FWD, B->gy, a);
+            run_groups(P_Z_TV_ADJ, B->gz, a);
+            run_groups(P_Y_INV, B->gy, a);
+            run_groups(P_C2R_ADJ, B->gxc, a);
+        }
+        QSM_CHECK_LAUNCH();
+    }
+    // field-bar = A (sum_j x-bar_j)
+    run_groups(P_R2C_ACC, B->gx, a);
+    run_groups(P_Y_FWD, B->gy, a);
+    run_groups(P_Z_TV_APPLYA, B->gz, a);
+    run_groups(P_Y_INV, B->gy, a);
+    run_groups(P_C2R_F64OUT, B->gxc, a);
+    launch_tv_lbar(st, B->d_vols, B->nvol, (int)(K - 1));
+    QSM_CHECK_LAUNCH();
+    if (lambda_bar)
+        QSM_HIP(hipMemcpyAsync(lambda_bar, U->d_lbar, B->nvol * sizeof(double), hipMemcpyDeviceToHost, st));
+    QSM_HIP(hipStreamSynchronize(st));
+    return QSM_OK;
+}
+
+// =================================================== dipole operator ======
+// field = Re IFFT(D FFT(chi)) (tests/chisep_integration.rs:18-62, kernel of
+// src/kernels/dipole.rs:24-72), device pointers, f64 in and out, FP32
+// transforms.  D is real and even after taking its Hermitian part, so the
+// operator is self-adjoint and the adjoint entry point runs the same passes.
+static int dipole_apply(const double* d_in, const qsm_grid* grid, const double bdir[3], double* d_out,
+                        void* stream) {
+    if (!d_in || !grid || !bdir || !d_out) return fail(QSM_EPARAM, "null argument");
+    int rc = validate_grid(grid);
+    if (rc) return rc;
+    const double bn = std::sqrt(bdir[0] * bdir[0] + bdir[1] * bdir[1] + bdir[2] * bdir[2]);
+    if (!(bn > 0)) return fail(QSM_EPARAM, "B0 direction has zero length");
+    VolDesc V;
+    std::memset(&V, 0, sizeof V);
+    TwiddleBank tb;
+    PassBlocks pb = fill_geometry(V, *grid, tb);
+    V.bx = bdir[0] / bn; V.by = bdir[1] / bn; V.bz = bdir[2] / bn;
+    V.field = d_in;
+    V.adj_out = d_out;
+    const size_t Nh = (size_t)V.nxh * V.ny * V.nz;
+    std::vector<void*> allocs;
+    float2* d_tw = nullptr;
+    VolDesc* d_vols = nullptr;
+    hipStream_t st = (hipStream_t)stream;
+    bool own = false;
+    auto done = [&](int code) {
+        if (st) (void)hipStreamSynchronize(st);
+        for (void* p : allocs) (void)hipFree(p);
+        if (own && st) (void)hipStreamDestroy(st);
+        return code;
+    };
+    if ((rc = dev_alloc(&V.spec[SPEC_WORK], Nh))) return done(rc);
+    allocs.push_back(V.spec[SPEC_WORK]);
+    if ((rc = dev_alloc(&d_tw, tb.host.size()))) return done(rc);
+    allocs.push_back(d_tw);
+    if ((rc = dev_alloc(&d_vols, 1))) return done(rc);
+    allocs.push_back(d_vols);
+    if (!st) {
+        hipError_t e = hipStreamCreateWithFlags(&st, hipStreamNonBlocking);
+        if (e != hipSuccess) { st = nullptr; return done(fail(QSM_EHIP, "hipStreamCreate: %s", hipGetErrorString(e))); }
+        own = true;
+    }
+    hipError_t e1 = hipMemcpy(d_tw, tb.host.data(), tb.host.size() * sizeof(float2), hipMemcpyHostToDevice);
+    hipError_t e2 = hipMemcpy(d_vols, &V, sizeof V, hipMemcpyHostToDevice);
+    if (e1 != hipSuccess || e2 != hipSuccess)
+        return done(fail(QSM_EHIP, "hipMemcpy of plan tables: %s", hipGetErrorString(e1 != hipSuccess ? e1 : e2)));
+    Groups gx = make_groups({V.nx}, {pb.xpass}, false);
+    Groups gxc = make_groups({V.nx}, {pb.cpass}, false);
+    Groups gy = make_groups({V.ny}, {pb.ypass}, true);
+    Groups gz = make_groups({V.nz}, {pb.zpass}, true);
+    PassArgs a{};
+    a.stream = st;
+    a.vols = d_vols;
+    a.tw = d_tw;
+    a.sel = SPEC_WORK;
+    a.sel2 = -1;
+    run_groups(P_R2C_F64, gx, a);
+    run_groups(P_Y_FWD, gy, a);
+    run_groups(P_Z_DIPOLE, gz, a);
+    run_groups(P_Y_INV, gy, a);
+    run_groups(P_C2R_F64OUT, gxc, a);
+    hipError_t le = hipGetLastError();
+    if (le != hipSuccess) return done(fail(QSM_EHIP, "kernel launch: %s", hipGetErrorString(le)));
+    hipError_t se = hipStreamSynchronize(st);
+    if (se != hipSuccess) return done(fail(QSM_EHIP, "hipStreamSynchronize: %s", hipGetErrorString(se)));
+    return done(QSM_OK);
 }
 
 // ========================================================= V-SHARP batch ==
@@ -874,6 +1083,37 @@ int qsm_run_dipole_inversion(const double* local_field_ppm, size_t field_len, co
     qsm_tv_params p;
     if (tv_params) p = *tv_params; else qsm_tv_params_default(&p);
     return qsm_tv_admm(local_field_ppm, mask, &g, md->b0_direction, &p, progress, user, chi_out);
+}
+
+int qsm_tv_unrolled_create(const qsm_tv_volume* vols, size_t n_vols, const qsm_tv_params* params,
+                           void* stream, qsm_tv_unrolled** out) {
+    g_err.clear();
+    return unrolled_create_impl(vols, n_vols, params, stream, out);
+}
+
+int qsm_tv_unrolled_forward(qsm_tv_unrolled* h) {
+    g_err.clear();
+    return unrolled_forward_impl(h);
+}
+
+int qsm_tv_unrolled_backward(qsm_tv_unrolled* h, const double* const* d_chi_bar,
+                             double* const* d_field_bar, double* lambda_bar) {
+    g_err.clear();
+    return unrolled_backward_impl(h, d_chi_bar, d_field_bar, lambda_bar);
+}
+
+void qsm_tv_unrolled_destroy(qsm_tv_unrolled* h) { unrolled_free(h); }
+
+int qsm_dipole_forward(const double* d_chi, const qsm_grid* grid, const double bdir[3], double* d_field,
+                       void* stream) {
+    g_err.clear();
+    return dipole_apply(d_chi, grid, bdir, d_field, stream);
+}
+
+int qsm_dipole_adjoint(const double* d_field, const qsm_grid* grid, const double bdir[3], double* d_chi,
+                       void* stream) {
+    g_err.clear();
+    return dipole_apply(d_field, grid, bdir, d_chi, stream);
 }
 
 int qsm_vsharp_batch_create(const qsm_vsharp_volume* vols, size_t n_vols, const double* radii,

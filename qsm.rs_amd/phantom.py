@@ -5,7 +5,8 @@ The random stream is SplitMix64 (counter based, so any draw can be recomputed
 from the seed alone) with Box-Muller normals.  The local field is the
 reference's forward model Re IFFT(D * FFT(chi)) (tests/chisep_integration.rs:
 18-62, dipole kernel of src/kernels/dipole.rs:24-72), computed here in float64
-with numpy's FFT; noise is added and the field is zeroed outside the mask.
+with a real-to-complex FFT (scipy.fft, multithreaded; numpy's FFT if scipy is
+absent); noise is added and the field is zeroed outside the mask.
 
 Arrays are flat float64 / uint8 vectors in Fortran voxel order
 (idx = i + j*nx + k*nx*ny).
@@ -79,9 +80,45 @@ def dipole_kernel(dims, vs, bdir) -> np.ndarray:
 
 
 def dipole_forward(chi3: np.ndarray, vs, bdir) -> np.ndarray:
-    """Re IFFT(D * FFT(chi)) (tests/chisep_integration.rs:18-62)."""
-    D = dipole_kernel(chi3.shape, vs, bdir)
-    return np.real(np.fft.ifftn(np.fft.fftn(chi3) * D))
+    """Re IFFT(D * FFT(chi)) (tests/chisep_integration.rs:18-62), through a
+    real FFT.  Re IFFT(F D) = IFFT(F D_h) with D_h(k) = (D(k) + D(k'))/2,
+    k' = k with every component that sits on its axis's Nyquist index
+    negated (fftfreq puts -n/2 there, so the DFT's mirror of k is -k' there);
+    D_h differs from D only on those planes and only for a tilted B0."""
+    dims = chi3.shape
+    nx, ny, nz = dims
+    try:
+        import scipy.fft as sfft
+        F = sfft.rfftn(chi3, workers=-1)
+        irfftn = lambda a: sfft.irfftn(a, s=dims, workers=-1)  # noqa: E731
+    except ImportError:  # pragma: no cover
+        F = np.fft.rfftn(chi3)
+        irfftn = lambda a: np.fft.irfftn(a, s=dims)  # noqa: E731
+    b = np.asarray(bdir, dtype=np.float64)
+    b = b / math.sqrt(float(b @ b))
+
+    def freqs(n, d, half):
+        f = np.fft.fftfreq(n, d)
+        if half:
+            f = f[: n // 2 + 1]
+        fl = f.copy()
+        if n % 2 == 0:
+            fl[n // 2] = -f[n // 2]
+        return f, fl
+
+    kx, kxf = freqs(nx, vs[0], False)
+    ky, kyf = freqs(ny, vs[1], False)
+    kz, kzf = freqs(nz, vs[2], True)
+
+    def dk(ax, ay, az):
+        ax, ay, az = ax[:, None, None], ay[None, :, None], az[None, None, :]
+        kdb = ax * b[0] + ay * b[1] + az * b[2]
+        k2 = ax * ax + ay * ay + az * az
+        with np.errstate(divide="ignore", invalid="ignore"):
+            return np.where(k2 > 1e-20, 1.0 / 3.0 - kdb * kdb / k2, 0.0)
+
+    D = 0.5 * (dk(kx, ky, kz) + dk(kxf, kyf, kzf))
+    return irfftn(F * D)
 
 
 @dataclass
@@ -97,8 +134,8 @@ class Phantom:
 
 def ellipsoid_mask(dims) -> np.ndarray:
     """Ellipsoid with semi-axes 0.4*dims centred at dims/2."""
-    g = np.meshgrid(*[np.arange(n, dtype=np.float64) for n in dims], indexing="ij")
-    r2 = sum(((g[a] - dims[a] / 2.0) / (0.4 * dims[a])) ** 2 for a in range(3))
+    t = [((np.arange(n, dtype=np.float64) - n / 2.0) / (0.4 * n)) ** 2 for n in dims]
+    r2 = t[0][:, None, None] + t[1][None, :, None] + t[2][None, None, :]
     return (r2 <= 1.0).astype(np.uint8)
 
 
@@ -122,12 +159,14 @@ def make_phantom(dims, voxel_size=(1.0, 1.0, 1.0), bdir=(0.0, 0.0, 1.0), n_ellip
                 break
         ax = [st.u(*semi_axes) for _ in range(3)]
         val = st.u(*chi_range)
-        # separable evaluation: sum of three 1-D terms
-        tx = ((g[0] - c[0]) / ax[0]) ** 2
-        ty = ((g[1] - c[1]) / ax[1]) ** 2
-        tz = ((g[2] - c[2]) / ax[2]) ** 2
+        # separable evaluation (sum of three 1-D terms) over the bounding box
+        lo = [max(0, int(math.floor(c[a] - ax[a]))) for a in range(3)]
+        hi = [min(dims[a], int(math.ceil(c[a] + ax[a])) + 1) for a in range(3)]
+        tx = ((g[0][lo[0]:hi[0]] - c[0]) / ax[0]) ** 2
+        ty = ((g[1][lo[1]:hi[1]] - c[1]) / ax[1]) ** 2
+        tz = ((g[2][lo[2]:hi[2]] - c[2]) / ax[2]) ** 2
         inside = (tx[:, None, None] + ty[None, :, None] + tz[None, None, :]) <= 1.0
-        chi3[inside] += val
+        chi3[lo[0]:hi[0], lo[1]:hi[1], lo[2]:hi[2]][inside] += val
     field3 = dipole_forward(chi3, voxel_size, bdir)
     n = int(np.prod(dims))
     noise = normal(seed, n).reshape(dims, order="F") * noise_sigma
@@ -147,7 +186,8 @@ def add_background(ph: Phantom, n_sources=3, seed=None, moment_range=(1.0, 5.0),
     b = b / math.sqrt(float(b @ b))
     centre = np.array([dims[a] / 2.0 * vs[a] for a in range(3)])
     mask_radius = max(0.4 * dims[a] * vs[a] for a in range(3))
-    X, Y, Z = np.meshgrid(*[np.arange(n) * vs[a] for a, n in enumerate(dims)], indexing="ij")
+    X, Y, Z = [(np.arange(n) * vs[a]).reshape([n if q == a else 1 for q in range(3)])
+               for a, n in enumerate(dims)]
     bg = np.zeros(dims)
     for _ in range(n_sources):
         m = st.u(*moment_range) * 1000.0  # ppm * mm^3

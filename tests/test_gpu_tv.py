@@ -132,3 +132,56 @@ def test_tv_rejects_bad_input(qsm):
     assert ei.value.code == 6
     with pytest.raises(ValueError):
         qsm.tv_admm(np.zeros(10), np.ones(10, np.uint8), qsm.Grid((4, 4, 4)), (0, 0, 1), p)
+
+
+def test_tv_full_size_256x256x160_vs_oracle(qsm, oracle, phantom):
+    # The benchmark workload at full size (0.9x0.9x1.0 mm, 40 ellipsoids,
+    # lambda 1e-3, rho 0.1, tol 0), cut to 10 iterations so the f64 oracle
+    # finishes in seconds.  Bar: NRMSE <= 1e-3 vs f64 (range-normalised,
+    # tests/common/mod.rs:34-51), after removing the mean difference that the
+    # reference's DC term leaves for anisotropic voxels; raw NRMSE and
+    # max-abs are printed.
+    ph = phantom.make_phantom((256, 256, 160), voxel_size=(0.9, 0.9, 1.0), n_ellipsoids=40,
+                              seed=1001)
+    grid = qsm.Grid(ph.dims, ph.voxel_size)
+    p = qsm.TvParams(lambda_=1e-3, rho=0.1, tol=0.0, max_iter=10)
+    x, it, rel = qsm.tv_admm(ph.local_field, ph.mask, grid, ph.bdir, p, return_info=True)
+    xo, oit, orel = oracle.tv_admm(ph.local_field, ph.mask, ph.dims, ph.voxel_size, ph.bdir,
+                                   1e-3, 0.1, 0.0, 10, return_info=True)
+    e = nrmse_dc(x, xo, ph.mask)
+    print("256x256x160 x10: nrmse(dc removed)", e, "raw", oracle.nrmse(x, xo, ph.mask),
+          "maxabs", float(np.max(np.abs(x - xo))), "rel", rel, orel)
+    assert it == oit == 10
+    assert e <= 1e-3
+
+
+def test_tv_uneven_batch_fixed_lengths(qsm, oracle, phantom):
+    # Volumes of different specialised axis lengths, voxel sizes and tilted
+    # B0 in one batch (grouped launches per length): each equals its own
+    # single-volume solve bit for bit, and one is checked against the oracle.
+    shapes = [((144, 176, 160), (0.7, 1.1, 0.9)), ((176, 144, 192), (1.0, 0.8, 0.6)),
+              ((160, 160, 144), (1.2, 1.2, 0.9))]
+    p = qsm.TvParams(lambda_=1e-3, rho=0.1, tol=0.0, max_iter=8)
+    keep, outs, vols, singles, phs = [], [], [], [], []
+    for v, (dims, vs) in enumerate(shapes):
+        b = phantom.tilted_bdir(2000 + v)
+        ph = phantom.make_phantom(dims, voxel_size=vs, bdir=b, n_ellipsoids=12, seed=2000 + v)
+        phs.append(ph)
+        singles.append(qsm.tv_admm(ph.local_field, ph.mask, qsm.Grid(dims, vs), b, p))
+        tf = qsm.DeviceArray.from_numpy(ph.local_field)
+        tm = qsm.DeviceArray.from_numpy(ph.mask)
+        tc = qsm.DeviceArray(ph.local_field.size, np.float64)
+        keep.append((tf, tm))
+        outs.append(tc)
+        vols.append((qsm.Grid(dims, vs), b, tf.ptr, tm.ptr, tc.ptr))
+    batch = qsm.TvBatch(vols, p)
+    batch.run()
+    for v in range(len(shapes)):
+        assert np.array_equal(outs[v].numpy(), singles[v]), v
+        assert batch.status(v)[0] == 8
+    batch.close()
+    ph = phs[1]
+    xo = oracle.tv_admm(ph.local_field, ph.mask, ph.dims, ph.voxel_size, ph.bdir, 1e-3, 0.1, 0.0, 8)
+    e = nrmse_dc(singles[1], xo, ph.mask)
+    print("uneven batch volume 1 nrmse(dc removed)", e, "raw", oracle.nrmse(singles[1], xo, ph.mask))
+    assert e <= 1e-3

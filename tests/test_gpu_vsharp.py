@@ -100,3 +100,40 @@ def test_vsharp_defaults_phantom(qsm, oracle, phantom):
     e = oracle.nrmse(local, lo, mo)
     print("vsharp 64^3 nrmse", e, "maxabs", np.max(np.abs(local - lo)))
     assert e < 1e-4
+
+
+def test_chain_vsharp_then_tv_device_resident(qsm, oracle, phantom):
+    # run_bg_removal -> run_dipole_inversion on device: a V-SHARP batch
+    # feeds its local field and eroded mask straight into a TV batch.  Checked
+    # against the host entry points (bit for bit) and against the oracle's
+    # vsharp -> tv_admm chain (masks identical, NRMSE <= 1e-3).
+    dims, vs, b = (128, 128, 64), (0.9, 0.9, 0.9), (0.0, 0.0, 1.0)
+    ph = phantom.add_background(phantom.make_phantom(dims, voxel_size=vs, n_ellipsoids=12,
+                                                     seed=4100), seed=4100)
+    radii = [4.5, 3.6, 2.7, 1.8]
+    p = qsm.TvParams(lambda_=1e-3, rho=0.1, tol=0.0, max_iter=15)
+    grid = qsm.Grid(dims, vs)
+    n = ph.mask.size
+    tf = qsm.DeviceArray.from_numpy(ph.total_field)
+    tm = qsm.DeviceArray.from_numpy(ph.mask)
+    tl = qsm.DeviceArray(n, np.float64)
+    te = qsm.DeviceArray(n, np.uint8)
+    tc = qsm.DeviceArray(n, np.float64)
+    vsb = qsm.VsharpBatch([(grid, tf.ptr, tm.ptr, tl.ptr, te.ptr)], radii, 0.05)
+    tvb = qsm.TvBatch([(grid, b, tl.ptr, te.ptr, tc.ptr)], p)
+    vsb.run()
+    tvb.run()
+    local, em, chi = tl.numpy(), te.numpy(), tc.numpy()
+    vsb.close()
+    tvb.close()
+    hl, hm = qsm.vsharp_with_radii(ph.total_field, ph.mask, grid, radii, 0.05)
+    hx = qsm.tv_admm(hl, hm, grid, b, p)
+    assert np.array_equal(em, hm) and np.array_equal(local, hl) and np.array_equal(chi, hx)
+    lo, mo = oracle.vsharp_with_radii(ph.total_field, ph.mask, dims, vs, radii, 0.05)
+    assert np.array_equal(em, mo)
+    err = _rel(local, lo, mo)
+    xo = oracle.tv_admm(lo, mo, dims, vs, b, 1e-3, 0.1, 0.0, 15)
+    e = oracle.nrmse(chi, xo, mo)
+    print("chain: eroded voxels", int(mo.sum()), "local max rel err", err, "chi nrmse", e)
+    assert err < 1e-4
+    assert e <= 1e-3
