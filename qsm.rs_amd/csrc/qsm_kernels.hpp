@@ -137,19 +137,26 @@ __device__ __forceinline__ double fftfreq_at(int i, int n, double d) {
 __device__ __forceinline__ int mirror_idx(int i, int n) { return i == 0 ? 0 : n - i; }
 
 // Binary sphere of src/kernels/smv.rs:19-67 (un-normalised: 1 inside).  The
-// squared distance is formed with explicit round-to-nearest operations, in the
-// reference's order and without fused multiply-adds, so boundary voxels are
+// squared distance is formed in FP64 in the reference's order with every
+// product and sum rounded (contraction off), so boundary voxels are
 // classified exactly as the reference (and the host voxel count) do.
 __device__ __forceinline__ float sphere_element(const VolDesc& V, int i, int r, int ridx) {
+    // The reference rounds every product and sum (no fused multiply-add);
+    // with FMA contraction voxels at exactly the radius (e.g. (4,3,0) *
+    // 0.9 mm at 4.5 mm) fall outside.  The pragma covers only operations
+    // written in this scope (the __dmul_rn / __dadd_rn helpers are still
+    // contracted after inlining), so the arithmetic is spelled out here.
+#pragma clang fp contract(off)
     const int k = (int)fastdiv((uint32_t)r, V.m_ny);
     const int j = r - k * V.ny;
     const double xi = (i <= V.nx / 2) ? (double)i : (double)(i - V.nx);
     const double yj = (j <= V.ny / 2) ? (double)j : (double)(j - V.ny);
     const double zk = (k <= V.nz / 2) ? (double)k : (double)(k - V.nz);
-    const double dx = __dmul_rn(xi, V.vsx), dy = __dmul_rn(yj, V.vsy), dz = __dmul_rn(zk, V.vsz);
-    const double d2 = __dadd_rn(__dadd_rn(__dmul_rn(dx, dx), __dmul_rn(dy, dy)), __dmul_rn(dz, dz));
+    const double dx = xi * V.vsx, dy = yj * V.vsy, dz = zk * V.vsz;
+    const double dxx = dx * dx, dyy = dy * dy, dzz = dz * dz;
+    const double d2 = (dxx + dyy) + dzz;
     const double rad = V.vs_radii[ridx];
-    return (d2 <= __dmul_rn(rad, rad)) ? 1.0f : 0.0f;
+    return (d2 <= rad * rad) ? 1.0f : 0.0f;
 }
 
 // ---------------------------------------------------------- tile helpers --

@@ -694,6 +694,7 @@ static int dipole_apply(const double* d_in, const qsm_grid* grid, const double b
 // sphere's bounding box only (same coordinates and the same f64 operation
 // order as the kernel, so the count equals the reference's `count`).
 static double sphere_count(const qsm_grid& g, double radius) {
+#pragma clang fp contract(off)
     const double r2 = radius * radius;
     auto coords = [&](size_t n, double vs, std::vector<double>& out) {
         const double lim = std::fabs(radius) / vs + 1.0;

@@ -27,6 +27,10 @@ def _rel(a, b, m=None):
     (16, (0.8, 1.0, 1.2), [4.0, 3.0, 2.0]),
     (24, (1.0, 1.0, 1.0), [5.0, 4.0, 3.0, 2.0, 1.0]),
     (20, (0.6, 0.6, 0.6), [3.0, 2.4, 1.8]),
+    # radii on exact voxel multiples where rounding decides the boundary:
+    # (4,3,0) * 0.9 mm is at exactly 4.5 mm only without fused multiply-add
+    (32, (0.9, 0.9, 0.9), [4.5, 3.6, 2.7, 1.8]),
+    (32, (0.9, 0.9, 0.9), [4.5]),
 ])
 def test_vsharp_matches_oracle(qsm, oracle, n, vs, radii):
     rng = np.random.default_rng(n)
