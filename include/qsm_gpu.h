@@ -85,7 +85,61 @@ typedef enum qsm_bg_removal_algorithm {
     QSM_BG_RESHARP, QSM_BG_HARPERELLA, QSM_BG_IHARPERELLA
 } qsm_bg_removal_algorithm;
 
+/* TkdParams, src/inversion/tkd.rs:21-33 (also the tsvd's parameters). */
+typedef struct qsm_tkd_params {
+    double threshold;  /* default 0.15 */
+} qsm_tkd_params;
+
+/* Regularization, src/inversion/tikhonov.rs:24-33 (same order). */
+typedef enum qsm_regularization { QSM_REG_IDENTITY = 0, QSM_REG_GRADIENT, QSM_REG_LAPLACIAN } qsm_regularization;
+
+/* TikhonovParams, src/inversion/tikhonov.rs:36-50. */
+typedef struct qsm_tikhonov_params {
+    double lambda;           /* default 0.01 */
+    qsm_regularization reg;  /* default Identity */
+} qsm_tikhonov_params;
+
+/* RtsParams, src/inversion/rts.rs:24-52. */
+typedef struct qsm_rts_params {
+    double delta;      /* 0.15 */
+    double mu;         /* 1e5 */
+    double rho;        /* 10 */
+    double tol;        /* 1e-2 */
+    size_t max_iter;   /* 20 */
+    size_t lsmr_iter;  /* 4 */
+} qsm_rts_params;
+
+/* SharpParams, src/bgremove/sharp.rs:27-45. */
+typedef struct qsm_sharp_params {
+    double threshold;  /* 0.05 */
+    double radius;     /* 6 mm */
+} qsm_sharp_params;
+
+/* InversionConfig, src/pipeline/config.rs:233-264, for the arms served here. */
+typedef struct qsm_inversion_config {
+    qsm_inversion_algorithm algorithm;  /* default Rts */
+    qsm_tkd_params tkd;
+    qsm_tkd_params tsvd;
+    qsm_tikhonov_params tikhonov;
+    qsm_tv_params tv;
+    qsm_rts_params rts;
+} qsm_inversion_config;
+
+/* BgRemovalConfig, src/pipeline/config.rs:202-229, for the arms served here. */
+typedef struct qsm_bg_removal_config {
+    qsm_bg_removal_algorithm algorithm;  /* default Vsharp */
+    qsm_vsharp_params vsharp;
+    qsm_sharp_params sharp;
+} qsm_bg_removal_config;
+
 /* ---- defaults and diagnostics ------------------------------------------ */
+
+void qsm_tkd_params_default(qsm_tkd_params* p);
+void qsm_tikhonov_params_default(qsm_tikhonov_params* p);
+void qsm_rts_params_default(qsm_rts_params* p);
+void qsm_sharp_params_default(qsm_sharp_params* p);
+void qsm_inversion_config_default(qsm_inversion_config* c);
+void qsm_bg_removal_config_default(qsm_bg_removal_config* c);
 
 /* TvParams::default(), src/inversion/tv.rs:34-43 */
 void qsm_tv_params_default(qsm_tv_params* p);
@@ -129,17 +183,45 @@ int qsm_vsharp_with_radii(const double* field, const uint8_t* mask, const qsm_gr
 
 /* ---- pipeline entry points --------------------------------------------- */
 
-/* run_dipole_inversion, src/pipeline/inversion.rs:28-35.  Serves the Tv arm
- * (tv_params); other algorithms return QSM_ECONFIG.  Lengths are checked as
- * the reference does (DimensionMismatch -> QSM_EDIM). */
+/* run_dipole_inversion, src/pipeline/inversion.rs:28-35.  Serves the Tkd,
+ * Tsvd, Tikhonov, Tv and Rts arms (tv_params for Tv, defaults for the
+ * others; qsm_run_dipole_inversion_cfg takes every parameter set); Tgv and
+ * Qsmart, and arms without a GPU backend here, return QSM_ECONFIG.  Lengths
+ * are checked as the reference does (DimensionMismatch -> QSM_EDIM). */
 int qsm_run_dipole_inversion(const double* local_field_ppm, size_t field_len,
                              const uint8_t* mask, size_t mask_len,
                              const qsm_scan_metadata* metadata, qsm_inversion_algorithm algorithm,
                              const qsm_tv_params* tv_params, qsm_progress_fn progress,
                              void* user, double* chi_out);
 
-/* run_bg_removal, src/pipeline/bg_removal.rs:20-26.  Serves the Vsharp arm;
- * other algorithms return QSM_ECONFIG. */
+/* ---- closed-form inversions, RTS, SHARP -------------------------------- */
+/* tkd, src/inversion/tkd.rs:48-99; tsvd, the function after it. */
+int qsm_tkd(const double* local_field, const uint8_t* mask, const qsm_grid* grid, const double bdir[3],
+            const qsm_tkd_params* params, double* chi_out);
+int qsm_tsvd(const double* local_field, const uint8_t* mask, const qsm_grid* grid, const double bdir[3],
+             const qsm_tkd_params* params, double* chi_out);
+/* tikhonov, src/inversion/tikhonov.rs:62-131. */
+int qsm_tikhonov(const double* local_field, const uint8_t* mask, const qsm_grid* grid,
+                 const double bdir[3], const qsm_tikhonov_params* params, double* chi_out);
+/* rts, src/inversion/rts.rs:71-212 (progress as tv_admm's). */
+int qsm_rts(const double* local_field, const uint8_t* mask, const qsm_grid* grid, const double bdir[3],
+            const qsm_rts_params* params, qsm_progress_fn progress, void* user, double* chi_out);
+/* sharp, src/bgremove/sharp.rs:48-56. */
+int qsm_sharp(const double* field, const uint8_t* mask, const qsm_grid* grid,
+              const qsm_sharp_params* params, double* local_out, uint8_t* mask_out);
+/* run_dipole_inversion / run_bg_removal with a full config (the arms above). */
+int qsm_run_dipole_inversion_cfg(const double* local_field_ppm, size_t field_len, const uint8_t* mask,
+                                 size_t mask_len, const qsm_scan_metadata* metadata,
+                                 const qsm_inversion_config* config, qsm_progress_fn progress,
+                                 void* user, double* chi_out);
+int qsm_run_bg_removal_cfg(const double* field_ppm, size_t field_len, const uint8_t* mask,
+                           size_t mask_len, const qsm_scan_metadata* metadata,
+                           const qsm_bg_removal_config* config, qsm_progress_fn progress,
+                           void* user, double* local_out, uint8_t* mask_out);
+
+/* run_bg_removal, src/pipeline/bg_removal.rs:20-26.  Serves the Vsharp arm
+ * (vsharp_params) and the Sharp arm (default SharpParams; see
+ * qsm_run_bg_removal_cfg); other algorithms return QSM_ECONFIG. */
 int qsm_run_bg_removal(const double* field_ppm, size_t field_len, const uint8_t* mask,
                        size_t mask_len, const qsm_scan_metadata* metadata,
                        qsm_bg_removal_algorithm algorithm,
@@ -164,6 +246,10 @@ typedef struct qsm_tv_batch qsm_tv_batch;
  * the device workspace once. */
 int qsm_tv_batch_create(const qsm_tv_volume* vols, size_t n_vols, const qsm_tv_params* params,
                         void* stream, qsm_tv_batch** out);
+/* The same batch running RTS (src/inversion/rts.rs:71-212) instead of TV:
+ * qsm_tv_batch_run / _status / _destroy apply unchanged. */
+int qsm_rts_batch_create(const qsm_tv_volume* vols, size_t n_vols, const qsm_rts_params* params,
+                         void* stream, qsm_tv_batch** out);
 /* Run the whole solve (spectral setup, iterations, masking) for every volume
  * in one launch per kernel.  Progress: (iter+1, max_iter) per iteration of
  * the batch, (k, k) once every volume has converged.  Blocks until done. */

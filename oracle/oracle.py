@@ -59,6 +59,10 @@ def lib():
             "orc_tv_admm": (None, [_D, _U8] + [_SZ] * 3 + [_F] * 9 + [_SZ, PROGRESS_FN,
                                    C.c_void_p, _D, C.POINTER(_SZ), _D]),
             "orc_tikhonov_gradient": (None, [_D, _U8] + [_SZ] * 3 + [_F] * 7 + [_D]),
+            "orc_tkd": (None, [_D, _U8] + [_SZ] * 3 + [_F] * 7 + [C.c_int, _D]),
+            "orc_tikhonov": (None, [_D, _U8] + [_SZ] * 3 + [_F] * 7 + [C.c_int, _D]),
+            "orc_rts": (None, [_D, _U8] + [_SZ] * 3 + [_F] * 10 + [_SZ, _SZ, PROGRESS_FN,
+                                                                  C.c_void_p, _D, C.POINTER(_SZ)]),
             "orc_tv_unrolled_vjp": (None, [_D, _U8] + [_SZ] * 3 + [_F] * 8 + [_SZ, _D, _D, _D,
                                                                             C.POINTER(_F)]),
             "orc_vsharp_radii": (_SZ, [_F, _F, _D, _SZ]),
@@ -236,6 +240,34 @@ def tv_unrolled_vjp(local_field, mask, dims, vs, bdir, lam, rho, K, chi_bar):
     lib().orc_tv_unrolled_vjp(_d(f), _u8(m), *dims, *vs, *bdir, lam, rho, K, _d(cb), _d(chi),
                               _d(fb), C.byref(lb))
     return chi, fb, float(lb.value)
+
+
+def tkd(local_field, mask, dims, vs, bdir, threshold, tsvd=False):
+    """src/inversion/tkd.rs:48-99 (tsvd=True: the tsvd variant after it)"""
+    f, m = _f64(local_field), _mask(mask)
+    out = np.empty(f.size)
+    lib().orc_tkd(_d(f), _u8(m), *dims, *vs, *bdir, threshold, 1 if tsvd else 0, _d(out))
+    return out
+
+
+def tikhonov(local_field, mask, dims, vs, bdir, lam, reg):
+    """src/inversion/tikhonov.rs:62-131; reg 0 Identity, 1 Gradient, 2 Laplacian"""
+    f, m = _f64(local_field), _mask(mask)
+    out = np.empty(f.size)
+    lib().orc_tikhonov(_d(f), _u8(m), *dims, *vs, *bdir, lam, int(reg), _d(out))
+    return out
+
+
+def rts(local_field, mask, dims, vs, bdir, delta=0.15, mu=1e5, rho=10.0, tol=1e-2, max_iter=20,
+        lsmr_iter=4, progress=None, return_info=False):
+    """src/inversion/rts.rs:71-212"""
+    f, m = _f64(local_field), _mask(mask)
+    out = np.empty(f.size)
+    iters = _SZ(0)
+    cb = _cb(progress)
+    lib().orc_rts(_d(f), _u8(m), *dims, *vs, *bdir, delta, mu, rho, tol, max_iter, lsmr_iter, cb,
+                  None, _d(out), C.byref(iters))
+    return (out, int(iters.value)) if return_info else out
 
 
 def tikhonov_gradient(local_field, mask, dims, vs, bdir, lam):

@@ -544,6 +544,133 @@ void orc_tv_unrolled_vjp(const double* local_field, const uint8_t* mask, size_t 
     if (lambda_bar) *lambda_bar = tbar / rho;
 }
 
+// src/inversion/tkd.rs:48-99: 1/D, or sign(D)/thr where |D| <= thr (tkd);
+// 0 there (tsvd, the function after it).
+void orc_tkd(const double* local_field, const uint8_t* mask, size_t nx, size_t ny, size_t nz,
+             double vsx, double vsy, double vsz, double bx, double by, double bz,
+             double threshold, int tsvd, double* chi_out) {
+    const size_t n = nx * ny * nz;
+    std::vector<double> d(n);
+    orc_dipole_kernel(nx, ny, nz, vsx, vsy, vsz, bx, by, bz, d.data());
+    const double inv_t = 1.0 / threshold;
+    auto fc = to_complex(local_field, n);
+    fft3d(fc.data(), nx, ny, nz, false);
+    for (size_t i = 0; i < n; ++i) {
+        double m;
+        if (std::fabs(d[i]) <= threshold) m = tsvd ? 0.0 : (d[i] >= 0.0 ? inv_t : -inv_t);
+        else m = 1.0 / d[i];
+        fc[i] *= m;
+    }
+    fft3d(fc.data(), nx, ny, nz, true);
+    for (size_t i = 0; i < n; ++i) chi_out[i] = fc[i].real();
+    orc_apply_mask_zero(chi_out, mask, n);
+}
+
+// src/inversion/tikhonov.rs:62-131: D / (D^2 + lambda Gamma), Gamma = 1,
+// FFT of the negative Laplacian stencil, or the square of the FFT of the
+// Laplacian stencil.
+void orc_tikhonov(const double* local_field, const uint8_t* mask, size_t nx, size_t ny,
+                  size_t nz, double vsx, double vsy, double vsz, double bx, double by,
+                  double bz, double lambda, int reg, double* chi_out) {
+    const size_t n = nx * ny * nz;
+    std::vector<double> d(n), gamma(n, 1.0);
+    orc_dipole_kernel(nx, ny, nz, vsx, vsy, vsz, bx, by, bz, d.data());
+    if (reg == 1 || reg == 2) {
+        std::vector<double> l(n);
+        orc_laplacian_kernel(nx, ny, nz, vsx, vsy, vsz, reg == 1 ? 1 : 0, l.data());
+        auto lc = to_complex(l.data(), n);
+        fft3d(lc.data(), nx, ny, nz, false);
+        for (size_t i = 0; i < n; ++i)
+            gamma[i] = reg == 1 ? lc[i].real() : lc[i].real() * lc[i].real();
+    }
+    auto fc = to_complex(local_field, n);
+    fft3d(fc.data(), nx, ny, nz, false);
+    for (size_t i = 0; i < n; ++i) {
+        const double denom = d[i] * d[i] + lambda * gamma[i];
+        fc[i] *= (std::fabs(denom) > 1e-20) ? d[i] / denom : 0.0;
+    }
+    fft3d(fc.data(), nx, ny, nz, true);
+    for (size_t i = 0; i < n; ++i) chi_out[i] = fc[i].real();
+    orc_apply_mask_zero(chi_out, mask, n);
+}
+
+// src/inversion/rts.rs:71-212.  Step 1: per-frequency refinement in the
+// well-conditioned cone |D| > delta; x0 = masked Re IFFT.  Step 2: ADMM
+// (admm_step with rho = 1 and inv_a = rho / (M + rho L^), M = mu on the
+// cone), shrink threshold 1/rho, starting from x0 with u = g = 0.
+void orc_rts(const double* local_field, const uint8_t* mask, size_t nx, size_t ny, size_t nz,
+             double vsx, double vsy, double vsz, double bx, double by, double bz,
+             double delta, double mu, double rho, double tol, size_t max_iter,
+             size_t lsmr_iter, orc_progress_fn progress, void* user, double* chi_out,
+             size_t* iters_out) {
+    const size_t n = nx * ny * nz;
+    std::vector<double> dk(n), lk(n);
+    orc_dipole_kernel(nx, ny, nz, vsx, vsy, vsz, bx, by, bz, dk.data());
+    orc_laplacian_kernel(nx, ny, nz, vsx, vsy, vsz, 1, lk.data());
+    std::vector<cd> work = to_complex(lk.data(), n);
+    fft3d(work.data(), nx, ny, nz, false);
+    std::vector<double> m_mask(n, 0.0), inv_a(n, 0.0);
+    for (size_t i = 0; i < n; ++i) {
+        const double l_fft = work[i].real();
+        if (std::fabs(dk[i]) > delta) m_mask[i] = mu;
+        const double a = m_mask[i] + rho * l_fft;
+        if (std::fabs(a) > 1e-20) inv_a[i] = rho / a;
+    }
+    for (size_t i = 0; i < n; ++i) work[i] = cd(local_field[i], 0.0);
+    fft3d(work.data(), nx, ny, nz, false);
+    const std::vector<cd> field_fft = work;
+    for (size_t i = 0; i < n; ++i) {
+        const double d = dk[i];
+        work[i] = (std::fabs(d) > delta) ? field_fft[i] * d / (d * d + 1e-6) : cd(0.0, 0.0);
+    }
+    std::vector<cd> residual(n);
+    for (size_t it = 0; it < lsmr_iter; ++it) {
+        for (size_t i = 0; i < n; ++i) residual[i] = field_fft[i] - work[i] * dk[i];
+        for (size_t i = 0; i < n; ++i) {
+            const double d = dk[i];
+            if (std::fabs(d) > delta) work[i] += residual[i] * d / (d * d + 1e-6);
+        }
+    }
+    fft3d(work.data(), nx, ny, nz, true);
+    std::vector<double> x(n);
+    for (size_t i = 0; i < n; ++i) x[i] = mask[i] ? work[i].real() : 0.0;
+    for (size_t i = 0; i < n; ++i) work[i] = cd(x[i], 0.0);
+    fft3d(work.data(), nx, ny, nz, false);
+    std::vector<cd> f_hat(n, cd(0.0, 0.0));
+    for (size_t i = 0; i < n; ++i)
+        if (std::fabs(m_mask[i]) > 1e-20 && std::fabs(inv_a[i]) > 1e-20)
+            f_hat[i] = work[i] * (m_mask[i] / rho) * inv_a[i];
+    std::vector<double> x_prev(n, 0.0), ux(n, 0.0), uy(n, 0.0), uz(n, 0.0), gx(n, 0.0),
+        gy(n, 0.0), gz(n, 0.0), div(n, 0.0);
+    const double t = 1.0 / rho;
+    size_t iters = 0;
+    for (size_t iter = 0; iter < max_iter; ++iter) {
+        if (progress) progress(iter + 1, max_iter, user);
+        iters = iter + 1;
+        std::swap(x, x_prev);
+        orc_bdiv(div.data(), gx.data(), gy.data(), gz.data(), nx, ny, nz, vsx, vsy, vsz);
+        for (size_t i = 0; i < n; ++i) work[i] = cd(div[i], 0.0);
+        fft3d(work.data(), nx, ny, nz, false);
+        for (size_t i = 0; i < n; ++i) work[i] = f_hat[i] - 1.0 * work[i] * inv_a[i];
+        fft3d(work.data(), nx, ny, nz, true);
+        for (size_t i = 0; i < n; ++i) x[i] = work[i].real();
+        if (orc_relative_change(x.data(), x_prev.data(), n) < tol) {
+            if (progress) progress(iter + 1, iter + 1, user);
+            break;
+        }
+        orc_fgrad(gx.data(), gy.data(), gz.data(), x.data(), nx, ny, nz, vsx, vsy, vsz);
+        for (size_t i = 0; i < n; ++i) {
+            double vx = gx[i] + ux[i], vy = gy[i] + uy[i], vz = gz[i] + uz[i];
+            double zx = orc_shrink(vx, t), zy = orc_shrink(vy, t), zz = orc_shrink(vz, t);
+            ux[i] = vx - zx; uy[i] = vy - zy; uz[i] = vz - zz;
+            gx[i] = 2.0 * zx - vx; gy[i] = 2.0 * zy - vy; gz[i] = 2.0 * zz - vz;
+        }
+    }
+    orc_apply_mask_zero(x.data(), mask, n);
+    std::memcpy(chi_out, x.data(), n * sizeof(double));
+    if (iters_out) *iters_out = iters;
+}
+
 // src/inversion/tikhonov.rs:62-131 (Regularization::Gradient arm)
 void orc_tikhonov_gradient(const double* local_field, const uint8_t* mask,
                            size_t nx, size_t ny, size_t nz, double vsx,

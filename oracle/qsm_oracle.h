@@ -87,6 +87,21 @@ void orc_tv_unrolled_vjp(const double* local_field, const uint8_t* mask, size_t 
                          size_t K, const double* chi_bar, double* chi_out,
                          double* field_bar, double* lambda_bar);
 
+/* src/inversion/tkd.rs:48-99 (tkd) and the tsvd that follows it. */
+void orc_tkd(const double* local_field, const uint8_t* mask, size_t nx, size_t ny, size_t nz,
+             double vsx, double vsy, double vsz, double bx, double by, double bz,
+             double threshold, int tsvd, double* chi_out);
+/* src/inversion/tikhonov.rs:62-131; reg 0 Identity, 1 Gradient, 2 Laplacian. */
+void orc_tikhonov(const double* local_field, const uint8_t* mask, size_t nx, size_t ny,
+                  size_t nz, double vsx, double vsy, double vsz, double bx, double by,
+                  double bz, double lambda, int reg, double* chi_out);
+/* src/inversion/rts.rs:71-212 (with admm_step, src/inversion/admm.rs:76-145). */
+void orc_rts(const double* local_field, const uint8_t* mask, size_t nx, size_t ny, size_t nz,
+             double vsx, double vsy, double vsz, double bx, double by, double bz,
+             double delta, double mu, double rho, double tol, size_t max_iter,
+             size_t lsmr_iter, orc_progress_fn progress, void* user, double* chi_out,
+             size_t* iters_out);
+
 /* src/inversion/tikhonov.rs:62-131 with Regularization::Gradient. */
 void orc_tikhonov_gradient(const double* local_field, const uint8_t* mask,
                            size_t nx, size_t ny, size_t nz, double vsx,

@@ -64,6 +64,32 @@ class _CVsharpParams(C.Structure):
                 ("min_radius", C.c_double)]
 
 
+class _CTkdParams(C.Structure):
+    _fields_ = [("threshold", C.c_double)]
+
+
+class _CTikhonovParams(C.Structure):
+    _fields_ = [("lambda_", C.c_double), ("reg", C.c_int)]
+
+
+class _CRtsParams(C.Structure):
+    _fields_ = [("delta", C.c_double), ("mu", C.c_double), ("rho", C.c_double), ("tol", C.c_double),
+                ("max_iter", C.c_size_t), ("lsmr_iter", C.c_size_t)]
+
+
+class _CSharpParams(C.Structure):
+    _fields_ = [("threshold", C.c_double), ("radius", C.c_double)]
+
+
+class _CInversionConfig(C.Structure):
+    _fields_ = [("algorithm", C.c_int), ("tkd", _CTkdParams), ("tsvd", _CTkdParams),
+                ("tikhonov", _CTikhonovParams), ("tv", _CTvParams), ("rts", _CRtsParams)]
+
+
+class _CBgRemovalConfig(C.Structure):
+    _fields_ = [("algorithm", C.c_int), ("vsharp", _CVsharpParams), ("sharp", _CSharpParams)]
+
+
 class _CScanMetadata(C.Structure):
     _fields_ = [("nx", C.c_size_t), ("ny", C.c_size_t), ("nz", C.c_size_t),
                 ("vsx", C.c_double), ("vsy", C.c_double), ("vsz", C.c_double),
@@ -109,6 +135,26 @@ _SIGS = {
     "qsm_tv_batch_status": (C.c_int, [C.c_void_p, _SZ, C.POINTER(_SZ), _D]),
     "qsm_tv_batch_destroy": (None, [C.c_void_p]),
     "qsm_tv_batch_profile": (C.c_int, [C.c_void_p, _SZ, _D, _SZ]),
+    "qsm_tkd_params_default": (None, [C.POINTER(_CTkdParams)]),
+    "qsm_tikhonov_params_default": (None, [C.POINTER(_CTikhonovParams)]),
+    "qsm_rts_params_default": (None, [C.POINTER(_CRtsParams)]),
+    "qsm_sharp_params_default": (None, [C.POINTER(_CSharpParams)]),
+    "qsm_inversion_config_default": (None, [C.POINTER(_CInversionConfig)]),
+    "qsm_bg_removal_config_default": (None, [C.POINTER(_CBgRemovalConfig)]),
+    "qsm_tkd": (C.c_int, [_D, _U8, C.POINTER(_CGrid), _D, C.POINTER(_CTkdParams), _D]),
+    "qsm_tsvd": (C.c_int, [_D, _U8, C.POINTER(_CGrid), _D, C.POINTER(_CTkdParams), _D]),
+    "qsm_tikhonov": (C.c_int, [_D, _U8, C.POINTER(_CGrid), _D, C.POINTER(_CTikhonovParams), _D]),
+    "qsm_rts": (C.c_int, [_D, _U8, C.POINTER(_CGrid), _D, C.POINTER(_CRtsParams), PROGRESS_FN,
+                          C.c_void_p, _D]),
+    "qsm_rts_batch_create": (C.c_int, [C.POINTER(_CTvVolume), _SZ, C.POINTER(_CRtsParams),
+                                       C.c_void_p, C.POINTER(C.c_void_p)]),
+    "qsm_sharp": (C.c_int, [_D, _U8, C.POINTER(_CGrid), C.POINTER(_CSharpParams), _D, _U8]),
+    "qsm_run_dipole_inversion_cfg": (C.c_int, [_D, _SZ, _U8, _SZ, C.POINTER(_CScanMetadata),
+                                               C.POINTER(_CInversionConfig), PROGRESS_FN,
+                                               C.c_void_p, _D]),
+    "qsm_run_bg_removal_cfg": (C.c_int, [_D, _SZ, _U8, _SZ, C.POINTER(_CScanMetadata),
+                                         C.POINTER(_CBgRemovalConfig), PROGRESS_FN, C.c_void_p,
+                                         _D, _U8]),
     "qsm_vsharp_batch_create": (C.c_int, [C.POINTER(_CVsharpVolume), _SZ, _D, _SZ, C.c_double,
                                           C.c_void_p, C.POINTER(C.c_void_p)]),
     "qsm_vsharp_batch_run": (C.c_int, [C.c_void_p, PROGRESS_FN, C.c_void_p]),
@@ -223,6 +269,57 @@ class VsharpParams:
         return _CVsharpParams(float(self.threshold), float(self.max_radius), float(self.min_radius))
 
 
+@dataclass
+class TkdParams:
+    """src/inversion/tkd.rs:21-33 (also the tsvd's parameters)"""
+    threshold: float = 0.15
+
+    def _c(self):
+        return _CTkdParams(float(self.threshold))
+
+
+class Regularization(enum.IntEnum):
+    """src/inversion/tikhonov.rs:24-33"""
+    Identity = 0
+    Gradient = 1
+    Laplacian = 2
+
+
+@dataclass
+class TikhonovParams:
+    """src/inversion/tikhonov.rs:36-50 (``lambda`` is spelled ``lambda_``)"""
+    lambda_: float = 0.01
+    reg: Regularization = Regularization.Identity
+
+    def _c(self):
+        return _CTikhonovParams(float(self.lambda_), int(self.reg))
+
+
+@dataclass
+class RtsParams:
+    """src/inversion/rts.rs:24-52"""
+    delta: float = 0.15
+    mu: float = 1e5
+    rho: float = 10.0
+    tol: float = 1e-2
+    max_iter: int = 20
+    lsmr_iter: int = 4
+
+    def _c(self):
+        return _CRtsParams(float(self.delta), float(self.mu), float(self.rho), float(self.tol),
+                           int(self.max_iter), int(self.lsmr_iter))
+
+
+@dataclass
+class SharpParams:
+    """src/bgremove/sharp.rs:27-45"""
+    threshold: float = 0.05
+    radius: float = 6.0
+
+    def _c(self):
+        return _CSharpParams(float(self.threshold), float(self.radius))
+
+
 def _flat64(a) -> np.ndarray:
     a = np.asarray(a)
     if a.ndim == 3:
@@ -310,6 +407,61 @@ def vsharp_with_radii(field, mask, grid: Grid, radii: Sequence[float], threshold
     return local, mout
 
 
+def _filter(fn, local_field, mask, grid, bdir, cparams):
+    f, m = _flat64(local_field), _flatu8(mask)
+    _check_len(grid, f, m)
+    out = np.empty(f.size)
+    g = grid._c()
+    b = np.ascontiguousarray(np.asarray(bdir, dtype=np.float64))
+    _check(fn(f.ctypes.data_as(_D), m.ctypes.data_as(_U8), C.byref(g), b.ctypes.data_as(_D),
+              C.byref(cparams), out.ctypes.data_as(_D)))
+    return out
+
+
+def tkd(local_field, mask, grid: Grid, bdir, params: TkdParams = None):
+    """src/inversion/tkd.rs:48-99"""
+    return _filter(_lib.qsm_tkd, local_field, mask, grid, bdir, (params or TkdParams())._c())
+
+
+def tsvd(local_field, mask, grid: Grid, bdir, params: TkdParams = None):
+    """The tsvd of src/inversion/tkd.rs (zero where |D| <= threshold)."""
+    return _filter(_lib.qsm_tsvd, local_field, mask, grid, bdir, (params or TkdParams())._c())
+
+
+def tikhonov(local_field, mask, grid: Grid, bdir, params: TikhonovParams = None):
+    """src/inversion/tikhonov.rs:62-131"""
+    return _filter(_lib.qsm_tikhonov, local_field, mask, grid, bdir,
+                   (params or TikhonovParams())._c())
+
+
+def rts(local_field, mask, grid: Grid, bdir, params: RtsParams = None,
+        progress: Optional[Callable[[int, int], None]] = None):
+    """src/inversion/rts.rs:71-212"""
+    f, m = _flat64(local_field), _flatu8(mask)
+    _check_len(grid, f, m)
+    out = np.empty(f.size)
+    g = grid._c()
+    b = np.ascontiguousarray(np.asarray(bdir, dtype=np.float64))
+    p = (params or RtsParams())._c()
+    cb = _progress(progress)
+    _check(_lib.qsm_rts(f.ctypes.data_as(_D), m.ctypes.data_as(_U8), C.byref(g),
+                        b.ctypes.data_as(_D), C.byref(p), cb, None, out.ctypes.data_as(_D)))
+    return out
+
+
+def sharp(field, mask, grid: Grid, params: SharpParams = None):
+    """src/bgremove/sharp.rs:48-56 -> (local_field, eroded_mask)"""
+    f, m = _flat64(field), _flatu8(mask)
+    _check_len(grid, f, m)
+    local = np.empty(f.size)
+    mout = np.empty(f.size, dtype=np.uint8)
+    g = grid._c()
+    p = (params or SharpParams())._c()
+    _check(_lib.qsm_sharp(f.ctypes.data_as(_D), m.ctypes.data_as(_U8), C.byref(g), C.byref(p),
+                          local.ctypes.data_as(_D), mout.ctypes.data_as(_U8)))
+    return local, mout
+
+
 # --------------------------------------------------------------- pipeline --
 class InversionAlgorithm(enum.IntEnum):
     """src/pipeline/config.rs:47-58"""
@@ -360,16 +512,30 @@ class ScanMetadata:
 
 @dataclass
 class InversionConfig:
-    """src/pipeline/config.rs:236-264 (the Tv arm is the one with a GPU backend)."""
+    """src/pipeline/config.rs:233-264 (the arms with a GPU backend: Tkd, Tsvd,
+    Tikhonov, Tv, Rts)."""
     algorithm: InversionAlgorithm = InversionAlgorithm.Rts
+    tkd: TkdParams = field(default_factory=TkdParams)
+    tsvd: TkdParams = field(default_factory=TkdParams)
+    tikhonov: TikhonovParams = field(default_factory=TikhonovParams)
     tv: TvParams = field(default_factory=TvParams)
+    rts: RtsParams = field(default_factory=RtsParams)
+
+    def _c(self):
+        return _CInversionConfig(int(self.algorithm), self.tkd._c(), self.tsvd._c(),
+                                 self.tikhonov._c(), self.tv._c(), self.rts._c())
 
 
 @dataclass
 class BgRemovalConfig:
-    """src/pipeline/config.rs:202-230 (the Vsharp arm is the one with a GPU backend)."""
+    """src/pipeline/config.rs:202-230 (the arms with a GPU backend: Vsharp,
+    Sharp)."""
     algorithm: BgRemovalAlgorithm = BgRemovalAlgorithm.Vsharp
     vsharp: VsharpParams = field(default_factory=VsharpParams)
+    sharp: SharpParams = field(default_factory=SharpParams)
+
+    def _c(self):
+        return _CBgRemovalConfig(int(self.algorithm), self.vsharp._c(), self.sharp._c())
 
 
 @dataclass
@@ -398,17 +564,18 @@ def _pipeline_error(e: QsmError, expected=None, got=None) -> PipelineError:
 
 def run_dipole_inversion(local_field_ppm, mask, metadata: ScanMetadata, config: InversionConfig,
                          magnitude=None, progress: Optional[Callable[[int, int], None]] = None):
-    """src/pipeline/inversion.rs:28-114 -- length check, then the Tv arm on the
-    GPU; other arms raise PipelineError('InvalidConfig')."""
+    """src/pipeline/inversion.rs:28-114 -- length check, then the Tkd / Tsvd /
+    Tikhonov / Tv / Rts arm on the GPU; Tgv, Qsmart and arms without a GPU
+    backend raise PipelineError('InvalidConfig')."""
     f = _flat64(local_field_ppm)
     m = _flatu8(mask)
     md, _keep = metadata._c()
-    p = config.tv._c()
+    c = config._c()
     out = np.empty(f.size)
     cb = _progress(progress)
-    rc = _lib.qsm_run_dipole_inversion(f.ctypes.data_as(_D), f.size, m.ctypes.data_as(_U8), m.size,
-                                       C.byref(md), int(config.algorithm), C.byref(p), cb, None,
-                                       out.ctypes.data_as(_D))
+    rc = _lib.qsm_run_dipole_inversion_cfg(f.ctypes.data_as(_D), f.size, m.ctypes.data_as(_U8),
+                                           m.size, C.byref(md), C.byref(c), cb, None,
+                                           out.ctypes.data_as(_D))
     if rc != QSM_OK:
         n = int(np.prod(metadata.dims))
         raise _pipeline_error(QsmError(rc, _lib.qsm_last_error_message().decode()), n, f.size)
@@ -417,18 +584,18 @@ def run_dipole_inversion(local_field_ppm, mask, metadata: ScanMetadata, config: 
 
 def run_bg_removal(field_ppm, mask, metadata: ScanMetadata, config: BgRemovalConfig,
                    progress: Optional[Callable[[int, int], None]] = None) -> BgRemovalResult:
-    """src/pipeline/bg_removal.rs:20-92 -- length check, then the Vsharp arm on
-    the GPU; other arms raise PipelineError('InvalidConfig')."""
+    """src/pipeline/bg_removal.rs:20-92 -- length check, then the Vsharp or
+    Sharp arm on the GPU; other arms raise PipelineError('InvalidConfig')."""
     f = _flat64(field_ppm)
     m = _flatu8(mask)
     md, _keep = metadata._c()
-    p = config.vsharp._c()
+    c = config._c()
     local = np.empty(f.size)
     mout = np.empty(f.size, dtype=np.uint8)
     cb = _progress(progress)
-    rc = _lib.qsm_run_bg_removal(f.ctypes.data_as(_D), f.size, m.ctypes.data_as(_U8), m.size,
-                                 C.byref(md), int(config.algorithm), C.byref(p), cb, None,
-                                 local.ctypes.data_as(_D), mout.ctypes.data_as(_U8))
+    rc = _lib.qsm_run_bg_removal_cfg(f.ctypes.data_as(_D), f.size, m.ctypes.data_as(_U8), m.size,
+                                     C.byref(md), C.byref(c), cb, None,
+                                     local.ctypes.data_as(_D), mout.ctypes.data_as(_U8))
     if rc != QSM_OK:
         n = int(np.prod(metadata.dims))
         raise _pipeline_error(QsmError(rc, _lib.qsm_last_error_message().decode()), n, f.size)
@@ -495,7 +662,8 @@ class TvBatch:
     with raw device addresses (f64 field, u8 mask, f64 output); the caller
     keeps the memory alive.  ``stream`` is a raw hipStream_t or None."""
 
-    def __init__(self, volumes, params: TvParams, stream: Optional[int] = None):
+    def __init__(self, volumes, params, stream: Optional[int] = None):
+        """params: TvParams (TV-ADMM) or RtsParams (the same batch running RTS)."""
         arr = (_CTvVolume * len(volumes))()
         for i, (grid, bdir, pf, pm, pc) in enumerate(volumes):
             arr[i].grid = grid._c()
@@ -503,8 +671,9 @@ class TvBatch:
             arr[i].d_field, arr[i].d_mask, arr[i].d_chi = int(pf), int(pm), int(pc)
         self._p = params._c()
         h = C.c_void_p()
-        _check(_lib.qsm_tv_batch_create(arr, len(volumes), C.byref(self._p),
-                                        C.c_void_p(stream) if stream else None, C.byref(h)))
+        create = _lib.qsm_rts_batch_create if isinstance(params, RtsParams) else _lib.qsm_tv_batch_create
+        _check(create(arr, len(volumes), C.byref(self._p),
+                      C.c_void_p(stream) if stream else None, C.byref(h)))
         self._h = h
         self.n = len(volumes)
 
@@ -699,4 +868,6 @@ __all__ = [
     "run_bg_removal", "TvBatch", "QsmError", "backend_info", "fft_length_supported",
     "c_library", "LIB_PATH", "DeviceArray", "synchronize", "VsharpBatch", "vsharp_radii",
     "TvUnrolled", "tv_unrolled_vjp", "UnrolledTvFunction", "dipole_forward",
+    "TkdParams", "Regularization", "TikhonovParams", "RtsParams", "SharpParams", "tkd", "tsvd",
+    "tikhonov", "rts", "sharp",
 ]

@@ -78,6 +78,11 @@ struct VolDesc {
     const double* adj_chibar;    // dL/dchi (device, f64)
     double* adj_out;             // f64 output of the C2R F64 mode (field-bar, dipole output)
     double* adj_lbar;            // dL/dlambda (device, one double)
+    // closed-form inversions and RTS
+    int filt;                    // Filter kind (ZOP_FILTER)
+    int lsmr_iter;               // RTS step-1 refinement rounds
+    double filt_a;               // threshold (TKD/TSVD), lambda (Tikhonov), delta (RTS)
+    double filt_b;               // mu (RTS)
 };
 
 __device__ __host__ inline int find_vol(const LaunchTable& lt, int bid) {
@@ -113,6 +118,8 @@ enum PassKind : int {
     P_R2C_TVDIV_REC, P_R2C_TVADJ_FIRST, P_R2C_TVADJ, P_R2C_ACC,
     P_Z_TV_ADJ, P_Z_TV_APPLYA, P_Z_DIPOLE,
     P_C2R_ADJ, P_C2R_F64OUT,
+    // closed-form inversions and RTS
+    P_Z_FILTER, P_Z_RTS_SETUP, P_Z_TV_SOLVE0, P_C2R_F64_MASKED, P_C2R_X0, P_R2C_X0,
     P_KIND_COUNT
 };
 

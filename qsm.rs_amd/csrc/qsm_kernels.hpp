@@ -200,7 +200,8 @@ enum SrcKind : int {
     SRC_TVDIV_REC = 5,    // SRC_TVDIV that also records the shrink pattern
     SRC_TVADJ_FIRST = 6,  // adjoint source of the last iteration: x-bar = mask * chi-bar
     SRC_TVADJ = 7,        // adjoint source of an earlier iteration
-    SRC_ACC = 8           // the adjoint accumulator
+    SRC_ACC = 8,          // the adjoint accumulator
+    SRC_X0 = 9            // x[0] (the RTS starting point)
 };
 
 template <int SRC>
@@ -209,6 +210,7 @@ __device__ __forceinline__ float simple_src(const VolDesc& V, int i, int r, int 
     else if constexpr (SRC == SRC_U8) return (float)glb(V.mask)[i + (size_t)r * V.nx];
     else if constexpr (SRC == SRC_F32) return glb(V.vs_local)[i + (size_t)r * V.nx];
     else if constexpr (SRC == SRC_ACC) return glb(V.adj_acc)[i + (size_t)r * V.nx];
+    else if constexpr (SRC == SRC_X0) return glb(V.x[0])[i + (size_t)r * V.nx];
     else return sphere_element(V, i, r, aux);
 }
 
@@ -574,42 +576,106 @@ __global__ __launch_bounds__(kThreads) void k_y_pass(const VolDesc* __restrict__
 }
 
 // ---------------------------------------------------- TV spectral passes --
-// D * inv_a and inv_a at one index (src/kernels/dipole.rs:24-72,
-// src/inversion/admm.rs:157-168), with the Laplacian spectrum in closed form
-// sum_a (2 - 2 cos(2 pi k_a / n_a)) / vs_a^2 (the DFT of the stencil of
-// src/kernels/laplacian.rs:20-62, exactly 0 at DC).
-__device__ inline void tv_multipliers(const VolDesc& V, int kx, int ky, int kz, double& dinv,
-                                      double& inv) {
+// Spectral terms at one index, in FP64 with every operation rounded (no
+// contraction) so threshold decisions match the reference's f64 arithmetic:
+// D = 1/3 - (k.b)^2/|k|^2, 0 where |k|^2 <= 1e-20 (src/kernels/dipole.rs:
+// 24-72), and the DFT of the negative Laplacian stencil of
+// src/kernels/laplacian.rs:20-62 in closed form,
+// L = sum_a (2 - 2 cos(2 pi k_a / n_a)) / vs_a^2 (exactly 0 at DC).
+__device__ inline void spec_terms(const VolDesc& V, int kx, int ky, int kz, double& D, double& L) {
+#pragma clang fp contract(off)
     const double fx = fftfreq_at(kx, V.nx, V.vsx);
     const double fy = fftfreq_at(ky, V.ny, V.vsy);
     const double fz = fftfreq_at(kz, V.nz, V.vsz);
     const double kdb = fx * V.bx + fy * V.by + fz * V.bz;
     const double k2 = fx * fx + fy * fy + fz * fz;
-    const double D = (k2 > 1e-20) ? (1.0 / 3.0) - (kdb * kdb) / k2 : 0.0;
+    D = (k2 > 1e-20) ? (1.0 / 3.0) - (kdb * kdb) / k2 : 0.0;
     const double two_pi = 6.283185307179586476925286766559;
-    const double L = (2.0 - 2.0 * cos(two_pi * kx / V.nx)) / (V.vsx * V.vsx) +
-                     (2.0 - 2.0 * cos(two_pi * ky / V.ny)) / (V.vsy * V.vsy) +
-                     (2.0 - 2.0 * cos(two_pi * kz / V.nz)) / (V.vsz * V.vsz);
+    L = (2.0 - 2.0 * cos(two_pi * kx / V.nx)) / (V.vsx * V.vsx) +
+        (2.0 - 2.0 * cos(two_pi * ky / V.ny)) / (V.vsy * V.vsy) +
+        (2.0 - 2.0 * cos(two_pi * kz / V.nz)) / (V.vsz * V.vsz);
+}
+
+// TV: D inv_a and inv_a, inv_a = 1/(D^2 + rho L) or 0 where |.| <= 1e-20
+// (src/inversion/admm.rs:157-168).
+__device__ inline void tv_multipliers(const VolDesc& V, int kx, int ky, int kz, double& dinv,
+                                      double& inv) {
+#pragma clang fp contract(off)
+    double D, L;
+    spec_terms(V, kx, ky, kz, D, L);
     const double a = D * D + V.rho * L;
     inv = (fabs(a) > 1e-20) ? 1.0 / a : 0.0;
     dinv = D * inv;
 }
 
+// RTS (src/inversion/rts.rs:102-183): M = mu on the cone |D| > delta, else 0;
+// inv_a = rho / (M + rho L) or 0; f^ multiplier (M / rho) inv_a where both
+// are non-zero; the ADMM step uses rho = 1, so its multiplier is inv_a.
+__device__ inline void rts_multipliers(const VolDesc& V, int kx, int ky, int kz, double& fm,
+                                       double& inv) {
+#pragma clang fp contract(off)
+    double D, L;
+    spec_terms(V, kx, ky, kz, D, L);
+    const double M = (fabs(D) > V.filt_a) ? V.filt_b : 0.0;
+    const double a = M + V.rho * L;
+    inv = (fabs(a) > 1e-20) ? V.rho / a : 0.0;
+    fm = (fabs(M) > 1e-20 && fabs(inv) > 1e-20) ? (M / V.rho) * inv : 0.0;
+}
+
 // The dipole kernel alone (src/kernels/dipole.rs:24-72).
 __device__ inline double dipole_at(const VolDesc& V, int kx, int ky, int kz) {
-    const double fx = fftfreq_at(kx, V.nx, V.vsx);
-    const double fy = fftfreq_at(ky, V.ny, V.vsy);
-    const double fz = fftfreq_at(kz, V.nz, V.vsz);
-    const double kdb = fx * V.bx + fy * V.by + fz * V.bz;
-    const double k2 = fx * fx + fy * fy + fz * fz;
-    return (k2 > 1e-20) ? (1.0 / 3.0) - (kdb * kdb) / k2 : 0.0;
+    double D, L;
+    spec_terms(V, kx, ky, kz, D, L);
+    return D;
+}
+
+// Closed-form spectral inversions, one multiplier per frequency:
+//   TKD  1/D, or sign(D)/thr where |D| <= thr      (src/inversion/tkd.rs:48-99)
+//   TSVD 1/D, or 0 where |D| <= thr                 (the tsvd after it)
+//   Tikhonov D / (D^2 + lambda Gamma), Gamma = 1, L or L^2, 0 where the
+//     denominator is <= 1e-20                        (src/inversion/tikhonov.rs:62-131)
+//   RTS step 1: lsmr_iter rounds of w += (1 - w D) D/(D^2 + 1e-6) from
+//     w = D/(D^2 + 1e-6) on the cone |D| > delta, 0 off it (the per-frequency
+//     refinement of src/inversion/rts.rs:117-150, applied to FFT(f))
+enum Filter : int { FILT_TKD = 0, FILT_TSVD = 1, FILT_TIK_ID = 2, FILT_TIK_GRAD = 3, FILT_TIK_LAP = 4,
+                    FILT_RTS_X0 = 5 };
+
+__device__ inline double filter_mult(const VolDesc& V, int kx, int ky, int kz) {
+#pragma clang fp contract(off)
+    double D, L;
+    spec_terms(V, kx, ky, kz, D, L);
+    const double p = V.filt_a;
+    switch (V.filt) {
+        case FILT_TKD: {
+            const double inv_t = 1.0 / p;
+            return fabs(D) <= p ? (D >= 0.0 ? inv_t : -inv_t) : 1.0 / D;
+        }
+        case FILT_TSVD: return fabs(D) <= p ? 0.0 : 1.0 / D;
+        case FILT_TIK_ID:
+        case FILT_TIK_GRAD:
+        case FILT_TIK_LAP: {
+            const double g = V.filt == FILT_TIK_ID ? 1.0 : (V.filt == FILT_TIK_GRAD ? L : L * L);
+            const double denom = D * D + p * g;
+            return fabs(denom) > 1e-20 ? D / denom : 0.0;
+        }
+        default: {  // FILT_RTS_X0
+            if (!(fabs(D) > p)) return 0.0;
+            const double gk = D / (D * D + 1e-6);
+            double w = gk;
+            for (int it = 0; it < V.lsmr_iter; ++it) w += (1.0 - w * D) * gk;
+            return w;
+        }
+    }
 }
 
 enum ZOp : int {
     ZOP_TV_SETUP = 0, ZOP_TV_SOLVE = 1, ZOP_VS_FWD = 2, ZOP_VS_RADIUS = 3, ZOP_VS_DECONV = 4,
     ZOP_TV_ADJ = 5,     // c-bar^ = -rho inv_a x-bar^ / N (the adjoint of the x-update in c)
     ZOP_TV_APPLYA = 6,  // field-bar^ = D inv_a acc^ / N  (the adjoint in f, A is symmetric)
-    ZOP_DIPOLE = 7      // D chi^ / N  (the dipole forward operator, self-adjoint)
+    ZOP_DIPOLE = 7,     // D chi^ / N  (the dipole forward operator, self-adjoint)
+    ZOP_FILTER = 8,     // closed-form inversion multiplier / N (filter_mult)
+    ZOP_RTS_SETUP = 9,  // as TV_SETUP with the RTS multipliers
+    ZOP_TV_SOLVE0 = 10  // first ADMM step from g = 0: x^ = f^, inverse z only
 };
 
 // TV z pass.  SETUP: forward z of FFT(f) (after x, y), then
@@ -643,6 +709,14 @@ __global__ __launch_bounds__(kThreads) void k_z_tv(const VolDesc* __restrict__ v
     const int total = nz * kTile;
     QSM_GLOBAL float2* buf = glb(V.spec[SPEC_WORK]) + base;
     load_twiddles(twl, tw + V.pz.tw_off, nz);
+    if constexpr (OP == ZOP_TV_SOLVE0) {
+        // g = 0 before the first step (AdmmBuffers::new), so div = 0 and x^ = f^
+        col_load<EMAX>(lds, glb(V.spec[SPEC_A]) + base, zs, nz, kx0, nxh);
+        __syncthreads();
+        run_fft<N, EMAX, true, kLog2Tile, true>(lds, V.pz, kTile, kTile, 1, twl);
+        col_store<EMAX>(lds, buf, zs, nz, kx0, nxh);
+        return;
+    }
     col_load<EMAX>(lds, buf, zs, nz, kx0, nxh);
     // x-update operands, in flight during the forward FFT
     constexpr bool USE_Q = OP == ZOP_TV_SOLVE || OP == ZOP_TV_ADJ;
@@ -663,7 +737,7 @@ __global__ __launch_bounds__(kThreads) void k_z_tv(const VolDesc* __restrict__ v
     }
     __syncthreads();
     run_fft<N, EMAX, true, kLog2Tile, false>(lds, V.pz, kTile, kTile, 1, twl);
-    if constexpr (OP == ZOP_TV_SETUP) {
+    if constexpr (OP == ZOP_TV_SETUP || OP == ZOP_RTS_SETUP) {
         const double invN = 1.0 / ((double)V.nx * V.ny * V.nz);
         QSM_GLOBAL float2* F = glb(V.spec[SPEC_A]) + base;
         QSM_GLOBAL float* Q = glb(V.q) + base;
@@ -672,18 +746,26 @@ __global__ __launch_bounds__(kThreads) void k_z_tv(const VolDesc* __restrict__ v
             const int kx = kx0 + c;
             if (kx < nxh) {
                 double d1, i1, d2, i2;
-                tv_multipliers(V, kx, ky, kz, d1, i1);
-                tv_multipliers(V, mirror_idx(kx, V.nx), mirror_idx(ky, V.ny), mirror_idx(kz, nz),
-                               d2, i2);
-                const double m = 0.5 * (d1 + d2) * invN;
-                const double qv = V.rho * 0.5 * (i1 + i2) * invN;
+                const int mx = mirror_idx(kx, V.nx), my = mirror_idx(ky, V.ny), mz = mirror_idx(kz, nz);
+                double m, qv;
+                if constexpr (OP == ZOP_TV_SETUP) {
+                    tv_multipliers(V, kx, ky, kz, d1, i1);
+                    tv_multipliers(V, mx, my, mz, d2, i2);
+                    m = 0.5 * (d1 + d2) * invN;
+                    qv = V.rho * 0.5 * (i1 + i2) * invN;
+                } else {
+                    rts_multipliers(V, kx, ky, kz, d1, i1);
+                    rts_multipliers(V, mx, my, mz, d2, i2);
+                    m = 0.5 * (d1 + d2) * invN;
+                    qv = 0.5 * (i1 + i2) * invN;
+                }
                 const float2 sv = lds[e];
                 stg2(F + kx + kz * zs, make_float2((float)((double)sv.x * m), (float)((double)sv.y * m)));
                 Q[kx + kz * zs] = (float)qv;
             }
         }
     } else {
-        if constexpr (OP == ZOP_TV_APPLYA || OP == ZOP_DIPOLE) {
+        if constexpr (OP == ZOP_TV_APPLYA || OP == ZOP_DIPOLE || OP == ZOP_FILTER) {
             const double invN = 1.0 / ((double)V.nx * V.ny * V.nz);
             for (int e = threadIdx.x; e < total; e += kThreads) {
                 const int kz = e >> kLog2Tile, c = e & (kTile - 1);
@@ -696,8 +778,10 @@ __global__ __launch_bounds__(kThreads) void k_z_tv(const VolDesc* __restrict__ v
                         tv_multipliers(V, kx, ky, kz, d1, i1);
                         tv_multipliers(V, mx, my, mz, d2, i2);
                         m = 0.5 * (d1 + d2) * invN;
-                    } else {
+                    } else if constexpr (OP == ZOP_DIPOLE) {
                         m = 0.5 * (dipole_at(V, kx, ky, kz) + dipole_at(V, mx, my, mz)) * invN;
+                    } else {
+                        m = 0.5 * (filter_mult(V, kx, ky, kz) + filter_mult(V, mx, my, mz)) * invN;
                     }
                 }
                 const float2 sv = lds[e];
@@ -762,7 +846,9 @@ __device__ __forceinline__ void c2r_load(float2* lds, int nx, int nxh, int RB,
 // reduces them.
 // MODE C2R_TV: as above.  C2R_ADJ: Re IFFT to adj_c (c-bar), nothing
 // else.  C2R_F64: Re IFFT widened to f64 into adj_out.
-enum C2RMode : int { C2R_TV = 0, C2R_ADJ = 1, C2R_F64 = 2 };
+// C2R_F64_MASKED: f64 into out, 0 where mask == 0 (src/utils/mask.rs:47-53).
+// C2R_X0: FP32 into x[0], 0 where mask == 0 (the RTS starting point).
+enum C2RMode : int { C2R_TV = 0, C2R_ADJ = 1, C2R_F64 = 2, C2R_F64_MASKED = 3, C2R_X0 = 4 };
 
 template <int N, int EG, int MODE>
 __global__ __launch_bounds__(kThreads) void k_c2r_tv(const VolDesc* __restrict__ vols, LaunchTable lt,
@@ -802,6 +888,18 @@ __global__ __launch_bounds__(kThreads) void k_c2r_tv(const VolDesc* __restrict__
                     QSM_GLOBAL float* __restrict__ o = glb(V.adj_c);
                     if (ra < nrows) o[i + (size_t)ra * nx] = z.x;
                     if (rb < nrows) o[i + (size_t)rb * nx] = z.y;
+                } else if constexpr (MODE == C2R_F64_MASKED || MODE == C2R_X0) {
+                    const QSM_GLOBAL uint8_t* __restrict__ m = glb(V.mask);
+                    const size_t ia = i + (size_t)ra * nx, ib = i + (size_t)rb * nx;
+                    if constexpr (MODE == C2R_F64_MASKED) {
+                        QSM_GLOBAL double* __restrict__ o = glb(V.out);
+                        if (ra < nrows) o[ia] = m[ia] ? (double)z.x : 0.0;
+                        if (rb < nrows) o[ib] = m[ib] ? (double)z.y : 0.0;
+                    } else {
+                        QSM_GLOBAL float* __restrict__ o = glb(V.x[0]);
+                        if (ra < nrows) o[ia] = m[ia] ? z.x : 0.0f;
+                        if (rb < nrows) o[ib] = m[ib] ? z.y : 0.0f;
+                    }
                 } else {
                     QSM_GLOBAL double* __restrict__ o = glb(V.adj_out);
                     if (ra < nrows) o[i + (size_t)ra * nx] = (double)z.x;

@@ -13,7 +13,8 @@ static bool is_row_kind(int kind) {
     return kind == P_R2C_TVDIV || kind == P_R2C_F64 || kind == P_R2C_U8 || kind == P_R2C_F32 ||
            kind == P_R2C_SPHERE || kind == P_C2R_TV || kind == P_C2R_VS_SELECT || kind == P_C2R_VS_OUT ||
            kind == P_R2C_TVDIV_REC || kind == P_R2C_TVADJ_FIRST || kind == P_R2C_TVADJ ||
-           kind == P_R2C_ACC || kind == P_C2R_ADJ || kind == P_C2R_F64OUT;
+           kind == P_R2C_ACC || kind == P_C2R_ADJ || kind == P_C2R_F64OUT || kind == P_C2R_F64_MASKED ||
+           kind == P_C2R_X0 || kind == P_R2C_X0;
 }
 
 template <int N, int EG>
@@ -45,6 +46,9 @@ static void launch_rows(int kind, const LaunchTable& lt, const PassArgs& a) {
         case P_C2R_TV: k_c2r_tv<N, EG, C2R_TV><<<g, b, 0, st>>>(a.vols, lt, a.tw, a.parity, a.status); break;
         case P_C2R_ADJ: k_c2r_tv<N, EG, C2R_ADJ><<<g, b, 0, st>>>(a.vols, lt, a.tw, 0, nullptr); break;
         case P_C2R_F64OUT: k_c2r_tv<N, EG, C2R_F64><<<g, b, 0, st>>>(a.vols, lt, a.tw, 0, nullptr); break;
+        case P_C2R_F64_MASKED: k_c2r_tv<N, EG, C2R_F64_MASKED><<<g, b, 0, st>>>(a.vols, lt, a.tw, 0, nullptr); break;
+        case P_C2R_X0: k_c2r_tv<N, EG, C2R_X0><<<g, b, 0, st>>>(a.vols, lt, a.tw, 0, nullptr); break;
+        case P_R2C_X0: k_r2c_x<N, EG, SRC_X0><<<g, b, 0, st>>>(a.vols, lt, a.tw, a.sel, 0, nullptr, nullptr, nullptr, 0); break;
         case P_C2R_VS_SELECT: k_c2r_vs_select<N, EG><<<g, b, 0, st>>>(a.vols, lt, a.tw, a.ridx, a.delta); break;
         case P_C2R_VS_OUT: k_c2r_vs_out<N, EG><<<g, b, 0, st>>>(a.vols, lt, a.tw); break;
         default: break;
@@ -63,6 +67,9 @@ static void launch_cols(int kind, const LaunchTable& lt, const PassArgs& a) {
         case P_Z_TV_ADJ: k_z_tv<N, EG, ZOP_TV_ADJ><<<g, b, 0, st>>>(a.vols, lt, a.tw, nullptr); break;
         case P_Z_TV_APPLYA: k_z_tv<N, EG, ZOP_TV_APPLYA><<<g, b, 0, st>>>(a.vols, lt, a.tw, nullptr); break;
         case P_Z_DIPOLE: k_z_tv<N, EG, ZOP_DIPOLE><<<g, b, 0, st>>>(a.vols, lt, a.tw, nullptr); break;
+        case P_Z_FILTER: k_z_tv<N, EG, ZOP_FILTER><<<g, b, 0, st>>>(a.vols, lt, a.tw, nullptr); break;
+        case P_Z_RTS_SETUP: k_z_tv<N, EG, ZOP_RTS_SETUP><<<g, b, 0, st>>>(a.vols, lt, a.tw, nullptr); break;
+        case P_Z_TV_SOLVE0: k_z_tv<N, EG, ZOP_TV_SOLVE0><<<g, b, 0, st>>>(a.vols, lt, a.tw, a.status); break;
         case P_Z_VS_FWD: k_z_vs<N, EG, ZOP_VS_FWD><<<g, b, 0, st>>>(a.vols, lt, a.tw, a.sel, 0, 0); break;
         case P_Z_VS_RADIUS: k_z_vs<N, EG, ZOP_VS_RADIUS><<<g, b, 0, st>>>(a.vols, lt, a.tw, 0, a.ridx, a.first); break;
         case P_Z_VS_DECONV: k_z_vs<N, EG, ZOP_VS_DECONV><<<g, b, 0, st>>>(a.vols, lt, a.tw, 0, 0, a.divide); break;

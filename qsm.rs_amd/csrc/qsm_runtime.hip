@@ -199,6 +199,7 @@ using namespace qsm;
 // ============================================================ TV batch ====
 struct qsm_tv_batch {
     int nvol = 0;
+    int method = 0;  // 0 TV-ADMM (src/inversion/tv.rs), 1 RTS (src/inversion/rts.rs)
     qsm_tv_params params{};
     std::vector<VolDesc> h;
     VolDesc* d_vols = nullptr;
@@ -238,8 +239,21 @@ static void batch_free(qsm_tv_batch* b) {
     delete b;
 }
 
-static int tv_batch_create_impl(const qsm_tv_volume* vols, size_t n_vols, const qsm_tv_params* params,
-                                void* stream, qsm_tv_batch** out) {
+// rts != null selects RTS: its rho, tol, max_iter replace params' (params
+// may then be null).
+static int tv_batch_create_impl(const qsm_tv_volume* vols, size_t n_vols, const qsm_tv_params* params_in,
+                                void* stream, qsm_tv_batch** out, const qsm_rts_params* rts = nullptr) {
+    qsm_tv_params rts_as_tv{};
+    if (rts) {
+        if (!(rts->rho > 0) || std::isnan(rts->delta) || std::isnan(rts->mu))
+            return fail(QSM_EPARAM, "RTS needs rho > 0 and finite delta, mu");
+        rts_as_tv.lambda = 1.0;  // threshold 1/rho is set per volume below
+        rts_as_tv.rho = rts->rho;
+        rts_as_tv.tol = rts->tol;
+        rts_as_tv.max_iter = rts->max_iter;
+        params_in = &rts_as_tv;
+    }
+    const qsm_tv_params* params = params_in;
     if (!vols || !params || !out) return fail(QSM_EPARAM, "null argument");
     if (n_vols == 0 || n_vols > (size_t)kMaxVolumes)
         return fail(QSM_EPARAM, "batch size %zu outside 1..%d", n_vols, kMaxVolumes);
@@ -256,6 +270,7 @@ static int tv_batch_create_impl(const qsm_tv_volume* vols, size_t n_vols, const 
     }
     qsm_tv_batch* B = new qsm_tv_batch();
     B->nvol = (int)n_vols;
+    B->method = rts ? 1 : 0;
     B->params = *params;
     B->h.resize(n_vols);
     TwiddleBank tb;
@@ -271,6 +286,13 @@ static int tv_batch_create_impl(const qsm_tv_volume* vols, size_t n_vols, const 
         V.rho = params->rho;
         V.tol = params->tol;
         V.thr = (float)(params->lambda / params->rho);          // src/inversion/tv.rs:78
+        if (rts) {
+            V.thr = (float)(1.0 / rts->rho);                     // src/inversion/rts.rs:191-201
+            V.filt = 5;                                          // FILT_RTS_X0
+            V.filt_a = rts->delta;
+            V.filt_b = rts->mu;
+            V.lsmr_iter = (int)rts->lsmr_iter;
+        }
         V.field = vols[v].d_field;
         V.mask = vols[v].d_mask;
         V.out = vols[v].d_chi;
@@ -345,6 +367,15 @@ static int tv_batch_create_impl(const qsm_tv_volume* vols, size_t n_vols, const 
 static int enqueue_iteration(qsm_tv_batch* B, size_t it, hipEvent_t closed, int rec = -1) {
     PassArgs a = B->args(B->d_status);
     a.parity = (int)((it - 1) & 1);
+    if (B->method == 1 && it == 1) {
+        // RTS starts from x0 != 0 with g = u = 0: the first step is x^ = f^
+        // (no z/u update and no divergence).
+        run_groups(P_Z_TV_SOLVE0, B->gz, a);
+        run_groups(P_Y_INV, B->gy, a);
+        run_groups(P_C2R_TV, B->gxc, a);
+        QSM_CHECK_LAUNCH();
+        return QSM_OK;
+    }
     if (it >= 2) {
         a.status_rw = B->d_status;
         a.mirror = B->h_status_dev;
@@ -366,6 +397,24 @@ static int enqueue_iteration(qsm_tv_batch* B, size_t it, hipEvent_t closed, int 
 // f^ and inv_a (prepare_admm_spectral, src/inversion/admm.rs:150-178).
 static int enqueue_setup(qsm_tv_batch* B) {
     hipStream_t st = B->stream;
+    if (B->method == 1) {
+        // RTS (src/inversion/rts.rs:93-183): x0 = mask * Re IFFT(FFT(f) w(D)),
+        // then f^ = FFT(x0) (M/rho) inv_a and inv_a.  u[0] is what the second
+        // step reads as the previous dual variable.
+        for (auto& V : B->h) launch_fill_zero(st, V.u[0], 3 * (size_t)V.nx * V.ny * V.nz);
+        launch_status_reset(st, B->d_status, B->nvol);
+        PassArgs a = B->args(nullptr);
+        run_groups(P_R2C_F64, B->gx, a);
+        run_groups(P_Y_FWD, B->gy, a);
+        run_groups(P_Z_FILTER, B->gz, a);
+        run_groups(P_Y_INV, B->gy, a);
+        run_groups(P_C2R_X0, B->gxc, a);
+        run_groups(P_R2C_X0, B->gx, a);
+        run_groups(P_Y_FWD, B->gy, a);
+        run_groups(P_Z_RTS_SETUP, B->gz, a);
+        QSM_CHECK_LAUNCH();
+        return QSM_OK;
+    }
     for (auto& V : B->h) {
         const size_t N = (size_t)V.nx * V.ny * V.nz;
         launch_fill_zero(st, V.x[0], N);
@@ -626,9 +675,12 @@ static int unrolled_backward_impl(qsm_tv_unrolled* U, const double* const* d_chi
 // src/kernels/dipole.rs:24-72), device pointers, f64 in and out, FP32
 // transforms.  D is real and even after taking its Hermitian part, so the
 // operator is self-adjoint and the adjoint entry point runs the same passes.
-static int dipole_apply(const double* d_in, const qsm_grid* grid, const double bdir[3], double* d_out,
-                        void* stream) {
-    if (!d_in || !grid || !bdir || !d_out) return fail(QSM_EPARAM, "null argument");
+// zkind P_Z_DIPOLE (no mask, f64 out) or P_Z_FILTER (filter `filt` with
+// parameter `fa`, output masked by d_mask).
+static int spectral_apply(int zkind, int filt, double fa, const double* d_in, const uint8_t* d_mask,
+                          const qsm_grid* grid, const double bdir[3], double* d_out, void* stream) {
+    if (!d_in || !grid || !bdir || !d_out || (zkind == P_Z_FILTER && !d_mask))
+        return fail(QSM_EPARAM, "null argument");
     int rc = validate_grid(grid);
     if (rc) return rc;
     const double bn = std::sqrt(bdir[0] * bdir[0] + bdir[1] * bdir[1] + bdir[2] * bdir[2]);
@@ -640,6 +692,10 @@ static int dipole_apply(const double* d_in, const qsm_grid* grid, const double b
     V.bx = bdir[0] / bn; V.by = bdir[1] / bn; V.bz = bdir[2] / bn;
     V.field = d_in;
     V.adj_out = d_out;
+    V.out = d_out;
+    V.mask = d_mask;
+    V.filt = filt;
+    V.filt_a = fa;
     const size_t Nh = (size_t)V.nxh * V.ny * V.nz;
     std::vector<void*> allocs;
     float2* d_tw = nullptr;
@@ -679,9 +735,9 @@ static int dipole_apply(const double* d_in, const qsm_grid* grid, const double b
     a.sel2 = -1;
     run_groups(P_R2C_F64, gx, a);
     run_groups(P_Y_FWD, gy, a);
-    run_groups(P_Z_DIPOLE, gz, a);
+    run_groups(zkind, gz, a);
     run_groups(P_Y_INV, gy, a);
-    run_groups(P_C2R_F64OUT, gxc, a);
+    run_groups(zkind == P_Z_FILTER ? P_C2R_F64_MASKED : P_C2R_F64OUT, gxc, a);
     hipError_t le = hipGetLastError();
     if (le != hipSuccess) return done(fail(QSM_EHIP, "kernel launch: %s", hipGetErrorString(le)));
     hipError_t se = hipStreamSynchronize(st);
@@ -1077,13 +1133,12 @@ int qsm_run_dipole_inversion(const double* local_field_ppm, size_t field_len, co
     if (algorithm == QSM_INV_TGV || algorithm == QSM_INV_QSMART)
         return fail(QSM_ECONFIG, "invalid config: %s should use run_tgv or run_qsmart",
                     algorithm == QSM_INV_TGV ? "Tgv" : "Qsmart");
-    if (algorithm != QSM_INV_TV)
-        return fail(QSM_ECONFIG, "invalid config: algorithm %d has no GPU backend here (Tv only)",
-                    (int)algorithm);
-    qsm_grid g{md->nx, md->ny, md->nz, md->vsx, md->vsy, md->vsz};
-    qsm_tv_params p;
-    if (tv_params) p = *tv_params; else qsm_tv_params_default(&p);
-    return qsm_tv_admm(local_field_ppm, mask, &g, md->b0_direction, &p, progress, user, chi_out);
+    qsm_inversion_config cfg;
+    qsm_inversion_config_default(&cfg);
+    cfg.algorithm = algorithm;
+    if (tv_params) cfg.tv = *tv_params;
+    return qsm_run_dipole_inversion_cfg(local_field_ppm, field_len, mask, mask_len, md, &cfg, progress,
+                                        user, chi_out);
 }
 
 int qsm_tv_unrolled_create(const qsm_tv_volume* vols, size_t n_vols, const qsm_tv_params* params,
@@ -1108,13 +1163,175 @@ void qsm_tv_unrolled_destroy(qsm_tv_unrolled* h) { unrolled_free(h); }
 int qsm_dipole_forward(const double* d_chi, const qsm_grid* grid, const double bdir[3], double* d_field,
                        void* stream) {
     g_err.clear();
-    return dipole_apply(d_chi, grid, bdir, d_field, stream);
+    return spectral_apply(P_Z_DIPOLE, 0, 0.0, d_chi, nullptr, grid, bdir, d_field, stream);
 }
 
 int qsm_dipole_adjoint(const double* d_field, const qsm_grid* grid, const double bdir[3], double* d_chi,
                        void* stream) {
     g_err.clear();
-    return dipole_apply(d_field, grid, bdir, d_chi, stream);
+    return spectral_apply(P_Z_DIPOLE, 0, 0.0, d_field, nullptr, grid, bdir, d_chi, stream);
+}
+
+// Host-array wrapper of a closed-form inversion (filter kind, parameter).
+static int filter_host(int filt, double fa, const double* field, const uint8_t* mask,
+                       const qsm_grid* grid, const double bdir[3], double* chi_out) {
+    if (!field || !mask || !grid || !bdir || !chi_out) return fail(QSM_EPARAM, "null argument");
+    int rc = validate_grid(grid);
+    if (rc) return rc;
+    const size_t N = grid->nx * grid->ny * grid->nz;
+    double *d_f = nullptr, *d_o = nullptr;
+    uint8_t* d_m = nullptr;
+    auto done = [&](int code) {
+        if (d_f) (void)hipFree(d_f);
+        if (d_o) (void)hipFree(d_o);
+        if (d_m) (void)hipFree(d_m);
+        return code;
+    };
+    if ((rc = dev_alloc(&d_f, N)) || (rc = dev_alloc(&d_o, N)) || (rc = dev_alloc(&d_m, N))) return done(rc);
+    hipError_t e = hipMemcpy(d_f, field, N * sizeof(double), hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(d_m, mask, N, hipMemcpyHostToDevice);
+    if (e != hipSuccess) return done(fail(QSM_EHIP, "hipMemcpy H2D: %s", hipGetErrorString(e)));
+    if ((rc = spectral_apply(P_Z_FILTER, filt, fa, d_f, d_m, grid, bdir, d_o, nullptr))) return done(rc);
+    e = hipMemcpy(chi_out, d_o, N * sizeof(double), hipMemcpyDeviceToHost);
+    if (e != hipSuccess) return done(fail(QSM_EHIP, "hipMemcpy D2H: %s", hipGetErrorString(e)));
+    return done(QSM_OK);
+}
+
+void qsm_tkd_params_default(qsm_tkd_params* p) {
+    if (p) p->threshold = 0.15;  // src/inversion/tkd.rs:29-33
+}
+
+void qsm_tikhonov_params_default(qsm_tikhonov_params* p) {
+    if (!p) return;
+    p->lambda = 0.01;  // src/inversion/tikhonov.rs:45-49
+    p->reg = QSM_REG_IDENTITY;
+}
+
+void qsm_rts_params_default(qsm_rts_params* p) {
+    if (!p) return;  // src/inversion/rts.rs:41-52
+    p->delta = 0.15;
+    p->mu = 1e5;
+    p->rho = 10.0;
+    p->tol = 1e-2;
+    p->max_iter = 20;
+    p->lsmr_iter = 4;
+}
+
+void qsm_sharp_params_default(qsm_sharp_params* p) {
+    if (!p) return;  // src/bgremove/sharp.rs:38-45
+    p->threshold = 0.05;
+    p->radius = 6.0;
+}
+
+int qsm_tkd(const double* local_field, const uint8_t* mask, const qsm_grid* grid, const double bdir[3],
+            const qsm_tkd_params* params, double* chi_out) {
+    g_err.clear();
+    if (!params || !(params->threshold > 0)) return fail(QSM_EPARAM, "TKD threshold must be > 0");
+    return filter_host(0, params->threshold, local_field, mask, grid, bdir, chi_out);
+}
+
+int qsm_tsvd(const double* local_field, const uint8_t* mask, const qsm_grid* grid, const double bdir[3],
+             const qsm_tkd_params* params, double* chi_out) {
+    g_err.clear();
+    if (!params || std::isnan(params->threshold)) return fail(QSM_EPARAM, "TSVD threshold is NaN");
+    return filter_host(1, params->threshold, local_field, mask, grid, bdir, chi_out);
+}
+
+int qsm_tikhonov(const double* local_field, const uint8_t* mask, const qsm_grid* grid,
+                 const double bdir[3], const qsm_tikhonov_params* params, double* chi_out) {
+    g_err.clear();
+    if (!params || params->reg < QSM_REG_IDENTITY || params->reg > QSM_REG_LAPLACIAN)
+        return fail(QSM_EPARAM, "bad Tikhonov parameters");
+    return filter_host(2 + (int)params->reg, params->lambda, local_field, mask, grid, bdir, chi_out);
+}
+
+int qsm_rts(const double* local_field, const uint8_t* mask, const qsm_grid* grid, const double bdir[3],
+            const qsm_rts_params* params, qsm_progress_fn progress, void* user, double* chi_out) {
+    g_err.clear();
+    if (!local_field || !mask || !grid || !bdir || !params || !chi_out)
+        return fail(QSM_EPARAM, "null argument");
+    int rc = validate_grid(grid);
+    if (rc) return rc;
+    const size_t N = grid->nx * grid->ny * grid->nz;
+    double *d_f = nullptr, *d_chi = nullptr;
+    uint8_t* d_m = nullptr;
+    qsm_tv_batch* B = nullptr;
+    auto done = [&](int code) {
+        if (B) batch_free(B);
+        if (d_f) (void)hipFree(d_f);
+        if (d_chi) (void)hipFree(d_chi);
+        if (d_m) (void)hipFree(d_m);
+        return code;
+    };
+    if ((rc = dev_alloc(&d_f, N)) || (rc = dev_alloc(&d_chi, N)) || (rc = dev_alloc(&d_m, N)))
+        return done(rc);
+    hipError_t e = hipMemcpy(d_f, local_field, N * sizeof(double), hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(d_m, mask, N, hipMemcpyHostToDevice);
+    if (e != hipSuccess) return done(fail(QSM_EHIP, "hipMemcpy H2D: %s", hipGetErrorString(e)));
+    qsm_tv_volume vol;
+    vol.grid = *grid;
+    vol.bdir[0] = bdir[0]; vol.bdir[1] = bdir[1]; vol.bdir[2] = bdir[2];
+    vol.d_field = d_f; vol.d_mask = d_m; vol.d_chi = d_chi;
+    if ((rc = tv_batch_create_impl(&vol, 1, nullptr, nullptr, &B, params))) return done(rc);
+    if ((rc = tv_batch_run_impl(B, progress, user))) return done(rc);
+    e = hipMemcpy(chi_out, d_chi, N * sizeof(double), hipMemcpyDeviceToHost);
+    if (e != hipSuccess) return done(fail(QSM_EHIP, "hipMemcpy D2H: %s", hipGetErrorString(e)));
+    return done(QSM_OK);
+}
+
+int qsm_rts_batch_create(const qsm_tv_volume* vols, size_t n_vols, const qsm_rts_params* params,
+                         void* stream, qsm_tv_batch** out) {
+    g_err.clear();
+    if (!params) return fail(QSM_EPARAM, "null argument");
+    return tv_batch_create_impl(vols, n_vols, nullptr, stream, out, params);
+}
+
+int qsm_sharp(const double* field, const uint8_t* mask, const qsm_grid* grid,
+              const qsm_sharp_params* params, double* local_out, uint8_t* mask_out) {
+    g_err.clear();
+    if (!params) return fail(QSM_EPARAM, "null argument");
+    const double r = params->radius;
+    return vsharp_host(field, mask, grid, &r, 1, params->threshold, nullptr, nullptr, local_out,
+                       mask_out);
+}
+
+void qsm_inversion_config_default(qsm_inversion_config* c) {
+    if (!c) return;  // src/pipeline/config.rs:248-264
+    c->algorithm = QSM_INV_RTS;
+    qsm_tkd_params_default(&c->tkd);
+    qsm_tkd_params_default(&c->tsvd);
+    qsm_tikhonov_params_default(&c->tikhonov);
+    qsm_tv_params_default(&c->tv);
+    qsm_rts_params_default(&c->rts);
+}
+
+// run_dipole_inversion, src/pipeline/inversion.rs:28-114, for the arms with
+// a GPU backend here (Tkd, Tsvd, Tikhonov, Tv, Rts).
+int qsm_run_dipole_inversion_cfg(const double* local_field_ppm, size_t field_len, const uint8_t* mask,
+                                 size_t mask_len, const qsm_scan_metadata* md,
+                                 const qsm_inversion_config* cfg, qsm_progress_fn progress, void* user,
+                                 double* chi_out) {
+    g_err.clear();
+    if (!md || !cfg) return fail(QSM_EPARAM, "metadata or config is null");
+    const size_t n = md->nx * md->ny * md->nz;
+    if (field_len != n) return fail(QSM_EDIM, "dimension mismatch: expected %zu, got %zu", n, field_len);
+    if (mask_len != n) return fail(QSM_EDIM, "dimension mismatch: expected %zu, got %zu", n, mask_len);
+    qsm_grid g{md->nx, md->ny, md->nz, md->vsx, md->vsy, md->vsz};
+    const double* b = md->b0_direction;
+    switch (cfg->algorithm) {
+        case QSM_INV_TKD: return qsm_tkd(local_field_ppm, mask, &g, b, &cfg->tkd, chi_out);
+        case QSM_INV_TSVD: return qsm_tsvd(local_field_ppm, mask, &g, b, &cfg->tsvd, chi_out);
+        case QSM_INV_TIKHONOV: return qsm_tikhonov(local_field_ppm, mask, &g, b, &cfg->tikhonov, chi_out);
+        case QSM_INV_TV: return qsm_tv_admm(local_field_ppm, mask, &g, b, &cfg->tv, progress, user, chi_out);
+        case QSM_INV_RTS: return qsm_rts(local_field_ppm, mask, &g, b, &cfg->rts, progress, user, chi_out);
+        case QSM_INV_TGV:
+        case QSM_INV_QSMART:
+            return fail(QSM_ECONFIG, "invalid config: %s should use run_tgv or run_qsmart",
+                        cfg->algorithm == QSM_INV_TGV ? "Tgv" : "Qsmart");
+        default:
+            return fail(QSM_ECONFIG, "invalid config: algorithm %d has no GPU backend here",
+                        (int)cfg->algorithm);
+    }
 }
 
 int qsm_vsharp_batch_create(const qsm_vsharp_volume* vols, size_t n_vols, const double* radii,
@@ -1161,13 +1378,42 @@ int qsm_run_bg_removal(const double* field_ppm, size_t field_len, const uint8_t*
     // src/pipeline/bg_removal.rs:30-35
     if (field_len != n) return fail(QSM_EDIM, "dimension mismatch: expected %zu, got %zu", n, field_len);
     if (mask_len != n) return fail(QSM_EDIM, "dimension mismatch: expected %zu, got %zu", n, mask_len);
-    if (algorithm != QSM_BG_VSHARP)
-        return fail(QSM_ECONFIG, "invalid config: algorithm %d has no GPU backend here (Vsharp only)",
-                    (int)algorithm);
+    qsm_bg_removal_config cfg;
+    qsm_bg_removal_config_default(&cfg);
+    cfg.algorithm = algorithm;
+    if (vsharp_params) cfg.vsharp = *vsharp_params;
+    return qsm_run_bg_removal_cfg(field_ppm, field_len, mask, mask_len, md, &cfg, progress, user,
+                                  local_out, mask_out);
+}
+
+void qsm_bg_removal_config_default(qsm_bg_removal_config* c) {
+    if (!c) return;  // src/pipeline/config.rs:215-229
+    c->algorithm = QSM_BG_VSHARP;
+    qsm_vsharp_params_default(&c->vsharp);
+    qsm_sharp_params_default(&c->sharp);
+}
+
+// run_bg_removal, src/pipeline/bg_removal.rs:20-92, for the Vsharp and Sharp
+// arms.
+int qsm_run_bg_removal_cfg(const double* field_ppm, size_t field_len, const uint8_t* mask,
+                           size_t mask_len, const qsm_scan_metadata* md,
+                           const qsm_bg_removal_config* cfg, qsm_progress_fn progress, void* user,
+                           double* local_out, uint8_t* mask_out) {
+    g_err.clear();
+    if (!md || !cfg) return fail(QSM_EPARAM, "metadata or config is null");
+    const size_t n = md->nx * md->ny * md->nz;
+    if (field_len != n) return fail(QSM_EDIM, "dimension mismatch: expected %zu, got %zu", n, field_len);
+    if (mask_len != n) return fail(QSM_EDIM, "dimension mismatch: expected %zu, got %zu", n, mask_len);
     qsm_grid g{md->nx, md->ny, md->nz, md->vsx, md->vsy, md->vsz};
-    qsm_vsharp_params p;
-    if (vsharp_params) p = *vsharp_params; else qsm_vsharp_params_default(&p);
-    return qsm_vsharp(field_ppm, mask, &g, &p, progress, user, local_out, mask_out);
+    switch (cfg->algorithm) {
+        case QSM_BG_VSHARP:
+            return qsm_vsharp(field_ppm, mask, &g, &cfg->vsharp, progress, user, local_out, mask_out);
+        case QSM_BG_SHARP:
+            return qsm_sharp(field_ppm, mask, &g, &cfg->sharp, local_out, mask_out);
+        default:
+            return fail(QSM_ECONFIG, "invalid config: algorithm %d has no GPU backend here",
+                        (int)cfg->algorithm);
+    }
 }
 
 int qsm_device_alloc(size_t bytes, void** out) {

@@ -7,6 +7,10 @@
 //   qsm::inversion::tv_admm         inversion::tv_admm    src/inversion/tv.rs:63-97
 //   qsm::bgremove::VsharpParams     bgremove::VsharpParams src/bgremove/vsharp.rs:20-39
 //   qsm::bgremove::vsharp           bgremove::vsharp      src/bgremove/vsharp.rs:55-64
+//   qsm::inversion::tkd / tsvd      src/inversion/tkd.rs:48-99
+//   qsm::inversion::tikhonov        src/inversion/tikhonov.rs:62-131
+//   qsm::inversion::rts             src/inversion/rts.rs:71-212
+//   qsm::bgremove::sharp            src/bgremove/sharp.rs:48-56
 //   qsm::pipeline::run_dipole_inversion / run_bg_removal  src/pipeline/*.rs
 //
 // Algorithm functions throw qsm::Error on a non-zero status (the Rust
@@ -89,9 +93,92 @@ inline std::vector<double> tv_admm(const std::vector<double>& local_field,
     return chi;
 }
 
+struct TkdParams {
+    double threshold = 0.15;
+    qsm_tkd_params c() const { return qsm_tkd_params{threshold}; }
+};
+
+enum class Regularization { Identity = QSM_REG_IDENTITY, Gradient = QSM_REG_GRADIENT, Laplacian = QSM_REG_LAPLACIAN };
+
+struct TikhonovParams {
+    double lambda = 0.01;
+    Regularization reg = Regularization::Identity;
+    qsm_tikhonov_params c() const { return qsm_tikhonov_params{lambda, (qsm_regularization)reg}; }
+};
+
+struct RtsParams {
+    double delta = 0.15, mu = 1e5, rho = 10.0, tol = 1e-2;
+    size_t max_iter = 20, lsmr_iter = 4;
+    qsm_rts_params c() const { return qsm_rts_params{delta, mu, rho, tol, max_iter, lsmr_iter}; }
+};
+
+namespace detail {
+template <class F, class P>
+inline std::vector<double> closed_form(F fn, const std::vector<double>& f, const std::vector<uint8_t>& m,
+                                       const Grid& grid, std::tuple<double, double, double> bdir,
+                                       const P& p) {
+    if (f.size() != grid.n_total() || m.size() != grid.n_total())
+        throw Error(QSM_EDIM, "array length does not match the grid");
+    std::vector<double> chi(grid.n_total());
+    const qsm_grid g = grid.c();
+    const double b[3] = {std::get<0>(bdir), std::get<1>(bdir), std::get<2>(bdir)};
+    const auto pc = p.c();
+    check(fn(f.data(), m.data(), &g, b, &pc, chi.data()));
+    return chi;
+}
+}  // namespace detail
+
+// src/inversion/tkd.rs:48-99 and the tsvd after it
+inline std::vector<double> tkd(const std::vector<double>& f, const std::vector<uint8_t>& m, const Grid& g,
+                               std::tuple<double, double, double> bdir, const TkdParams& p) {
+    return detail::closed_form(qsm_tkd, f, m, g, bdir, p);
+}
+inline std::vector<double> tsvd(const std::vector<double>& f, const std::vector<uint8_t>& m, const Grid& g,
+                                std::tuple<double, double, double> bdir, const TkdParams& p) {
+    return detail::closed_form(qsm_tsvd, f, m, g, bdir, p);
+}
+// src/inversion/tikhonov.rs:62-131
+inline std::vector<double> tikhonov(const std::vector<double>& f, const std::vector<uint8_t>& m,
+                                    const Grid& g, std::tuple<double, double, double> bdir,
+                                    const TikhonovParams& p) {
+    return detail::closed_form(qsm_tikhonov, f, m, g, bdir, p);
+}
+// src/inversion/rts.rs:71-212
+inline std::vector<double> rts(const std::vector<double>& f, const std::vector<uint8_t>& m, const Grid& grid,
+                               std::tuple<double, double, double> bdir, const RtsParams& params,
+                               Progress progress = nullptr) {
+    if (f.size() != grid.n_total() || m.size() != grid.n_total())
+        throw Error(QSM_EDIM, "array length does not match the grid");
+    std::vector<double> chi(grid.n_total());
+    const qsm_grid g = grid.c();
+    const double b[3] = {std::get<0>(bdir), std::get<1>(bdir), std::get<2>(bdir)};
+    const qsm_rts_params p = params.c();
+    check(qsm_rts(f.data(), m.data(), &g, b, &p, progress ? qsm::detail::trampoline : nullptr,
+                  progress ? &progress : nullptr, chi.data()));
+    return chi;
+}
+
 }  // namespace inversion
 
 namespace bgremove {
+
+struct SharpParams {
+    double threshold = 0.05, radius = 6.0;
+};
+
+// src/bgremove/sharp.rs:48-56
+inline std::pair<std::vector<double>, std::vector<uint8_t>> sharp(const std::vector<double>& field,
+                                                                   const std::vector<uint8_t>& mask,
+                                                                   const Grid& grid, const SharpParams& p) {
+    if (field.size() != grid.n_total() || mask.size() != grid.n_total())
+        throw Error(QSM_EDIM, "array length does not match the grid");
+    std::vector<double> local(grid.n_total());
+    std::vector<uint8_t> eroded(grid.n_total());
+    const qsm_grid g = grid.c();
+    const qsm_sharp_params pc{p.threshold, p.radius};
+    check(qsm_sharp(field.data(), mask.data(), &g, &pc, local.data(), eroded.data()));
+    return {std::move(local), std::move(eroded)};
+}
 
 struct VsharpParams {
     double threshold, max_radius, min_radius;
@@ -172,12 +259,23 @@ struct ScanMetadata {
 
 struct InversionConfig {
     qsm_inversion_algorithm algorithm = QSM_INV_RTS;
+    inversion::TkdParams tkd;
+    inversion::TkdParams tsvd;
+    inversion::TikhonovParams tikhonov;
     inversion::TvParams tv = inversion::TvParams::default_params();
+    inversion::RtsParams rts;
+    qsm_inversion_config c() const {
+        return qsm_inversion_config{algorithm, tkd.c(), tsvd.c(), tikhonov.c(), tv.c(), rts.c()};
+    }
 };
 
 struct BgRemovalConfig {
     qsm_bg_removal_algorithm algorithm = QSM_BG_VSHARP;
     bgremove::VsharpParams vsharp = bgremove::VsharpParams::default_params();
+    bgremove::SharpParams sharp;
+    qsm_bg_removal_config c() const {
+        return qsm_bg_removal_config{algorithm, vsharp.c(), qsm_sharp_params{sharp.threshold, sharp.radius}};
+    }
 };
 
 struct BgRemovalResult {
@@ -200,11 +298,10 @@ inline Result<std::vector<double>> run_dipole_inversion(const std::vector<double
                                                         Progress progress = nullptr) {
     Result<std::vector<double>> r{false, std::vector<double>(local_field_ppm.size()), {}};
     const qsm_scan_metadata m = md.c();
-    const qsm_tv_params p = config.tv.c();
-    const int rc = qsm_run_dipole_inversion(
-        local_field_ppm.data(), local_field_ppm.size(), mask.data(), mask.size(), &m,
-        config.algorithm, &p, progress ? detail::trampoline : nullptr,
-        progress ? &progress : nullptr, r.value.data());
+    const qsm_inversion_config c = config.c();
+    const int rc = qsm_run_dipole_inversion_cfg(
+        local_field_ppm.data(), local_field_ppm.size(), mask.data(), mask.size(), &m, &c,
+        progress ? detail::trampoline : nullptr, progress ? &progress : nullptr, r.value.data());
     if (rc != QSM_OK) { r.error = to_error(rc); r.value.clear(); return r; }
     r.ok = true;
     return r;
@@ -217,12 +314,11 @@ inline Result<BgRemovalResult> run_bg_removal(const std::vector<double>& field_p
     Result<BgRemovalResult> r{false, {std::vector<double>(field_ppm.size()),
                                       std::vector<uint8_t>(field_ppm.size())}, {}};
     const qsm_scan_metadata m = md.c();
-    const qsm_vsharp_params p = config.vsharp.c();
-    const int rc = qsm_run_bg_removal(field_ppm.data(), field_ppm.size(), mask.data(), mask.size(),
-                                      &m, config.algorithm, &p,
-                                      progress ? detail::trampoline : nullptr,
-                                      progress ? &progress : nullptr,
-                                      r.value.local_field_ppm.data(), r.value.eroded_mask.data());
+    const qsm_bg_removal_config c = config.c();
+    const int rc = qsm_run_bg_removal_cfg(field_ppm.data(), field_ppm.size(), mask.data(), mask.size(),
+                                          &m, &c, progress ? detail::trampoline : nullptr,
+                                          progress ? &progress : nullptr,
+                                          r.value.local_field_ppm.data(), r.value.eroded_mask.data());
     if (rc != QSM_OK) { r.error = to_error(rc); r.value = {}; return r; }
     r.ok = true;
     return r;

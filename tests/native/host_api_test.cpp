@@ -188,12 +188,83 @@ static void test_inversion_tv_and_rejects_tgv() {
     EXPECT(!t.ok && t.error.kind == pipeline::PipelineErrorKind::InvalidConfig, "Tgv rejected");
 }
 
+// src/inversion/tkd.rs:160-231 (zero field, mask, finite, tsvd differs)
+static void test_tkd_reference_tests() {
+    const size_t n = 8;
+    Grid g = Grid::make(n, n, n, 1, 1, 1);
+    std::vector<uint8_t> m(n * n * n, 1);
+    auto z = inversion::tkd(std::vector<double>(n * n * n, 0.0), m, g, {0, 0, 1}, {0.15});
+    for (double v : z) EXPECT(std::fabs(v) < 1e-10, "zero field -> zero chi");
+    std::vector<uint8_t> m2 = m;
+    m2[0] = m2[1] = 0;
+    auto c = inversion::tkd(std::vector<double>(n * n * n, 1.0), m2, g, {0, 0, 1}, {0.15});
+    EXPECT(c[0] == 0.0 && c[1] == 0.0, "outside the mask is 0");
+    std::vector<double> f16(16 * 16 * 16);
+    for (size_t i = 0; i < f16.size(); ++i) f16[i] = std::sin(i * 0.1);
+    Grid g16 = Grid::make(16, 16, 16, 1, 1, 1);
+    std::vector<uint8_t> m16(f16.size(), 1);
+    auto a = inversion::tkd(f16, m16, g16, {0, 0, 1}, {0.15});
+    auto b = inversion::tsvd(f16, m16, g16, {0, 0, 1}, {0.15});
+    double d = 0;
+    for (size_t i = 0; i < a.size(); ++i) d += std::fabs(a[i] - b[i]);
+    EXPECT(d > 1e-10, "tkd and tsvd differ");
+}
+
+// src/inversion/tikhonov.rs:173-199 (regularisation types differ)
+static void test_tikhonov_regularization_types() {
+    const size_t n = 8;
+    Grid g = Grid::make(n, n, n, 1, 1, 1);
+    std::vector<double> f(n * n * n);
+    for (size_t i = 0; i < f.size(); ++i) f[i] = std::sin(i * 0.1);
+    std::vector<uint8_t> m(f.size(), 1);
+    using R = inversion::Regularization;
+    auto i0 = inversion::tikhonov(f, m, g, {0, 0, 1}, {0.01, R::Identity});
+    auto i1 = inversion::tikhonov(f, m, g, {0, 0, 1}, {0.01, R::Gradient});
+    auto i2 = inversion::tikhonov(f, m, g, {0, 0, 1}, {0.01, R::Laplacian});
+    double d01 = 0, d12 = 0;
+    for (size_t i = 0; i < f.size(); ++i) { d01 += std::fabs(i0[i] - i1[i]); d12 += std::fabs(i1[i] - i2[i]); }
+    EXPECT(d01 > 1e-10 && d12 > 1e-10, "Identity, Gradient and Laplacian differ");
+}
+
+// src/inversion/rts.rs:219-265 (zero field, finite, mask)
+static void test_rts_reference_tests() {
+    const size_t n = 8;
+    Grid g = Grid::make(n, n, n, 1, 1, 1);
+    inversion::RtsParams p;
+    p.max_iter = 5;
+    p.lsmr_iter = 2;
+    std::vector<uint8_t> m(n * n * n, 1);
+    auto z = inversion::rts(std::vector<double>(n * n * n, 0.0), m, g, {0, 0, 1}, p);
+    for (double v : z) EXPECT(std::fabs(v) < 1e-6, "zero field -> near-zero chi");
+    auto r = inversion::rts(ramp(n * n * n, 0.001), m, g, {0, 0, 1}, p);
+    for (double v : r) EXPECT(std::isfinite(v), "finite");
+    std::vector<uint8_t> m2 = m;
+    m2[0] = m2[10] = 0;
+    auto c = inversion::rts(ramp(n * n * n, 0.001), m2, g, {0, 0, 1}, p);
+    EXPECT(c[0] == 0.0 && c[10] == 0.0, "masked voxels are 0");
+}
+
+// src/pipeline/inversion.rs: the default config (Rts) runs; Sharp arm runs
+static void test_pipeline_default_rts_and_sharp_arm() {
+    pipeline::ScanMetadata md{{8, 8, 8}, {1, 1, 1}, {0.005}, 3.0, {0, 0, 1}};
+    auto r = pipeline::run_dipole_inversion(std::vector<double>(512, 0.01), std::vector<uint8_t>(512, 1),
+                                            md, pipeline::InversionConfig{});
+    EXPECT(r.ok && r.value.size() == 512, "default (Rts) arm runs");
+    pipeline::BgRemovalConfig bc;
+    bc.algorithm = QSM_BG_SHARP;
+    bc.sharp.radius = 3.0;
+    auto s = pipeline::run_bg_removal(ramp(512, 0.001), std::vector<uint8_t>(512, 1), md, bc);
+    EXPECT(s.ok && s.value.eroded_mask.size() == 512, "Sharp arm runs");
+}
+
 int main() {
     void (*tests[])() = {test_tv_admm_zero_field, test_tv_admm_finite, test_tv_admm_smoother_than_tkd,
                          test_tv_params_default, test_vsharp_zero_field, test_vsharp_nonuniform_voxels,
                          test_vsharp_empty_radii, test_vsharp_larger_volume, test_vsharp_with_progress,
                          test_vsharp_unsorted_radii, test_bg_removal_dispatches_vsharp,
-                         test_bg_removal_validates_dims, test_inversion_tv_and_rejects_tgv};
+                         test_bg_removal_validates_dims, test_inversion_tv_and_rejects_tgv,
+                         test_tkd_reference_tests, test_tikhonov_regularization_types,
+                         test_rts_reference_tests, test_pipeline_default_rts_and_sharp_arm};
     int i = 0;
     for (auto t : tests) {
         const int before = g_fail;

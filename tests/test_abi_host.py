@@ -49,6 +49,23 @@ def test_param_defaults():
     assert (v.threshold, v.max_radius, v.min_radius) == (0.05, 12.0, 1.0)
     g = qsm.Grid.new(64, 64, 32, 1.0, 1.0, 2.0)
     assert (g.nx(), g.ny(), g.nz(), g.n_total(), g.vsz()) == (64, 64, 32, 64 * 64 * 32, 2.0)
+    # the C defaults (tkd.rs:29-33, tikhonov.rs:45-49, rts.rs:41-52,
+    # sharp.rs:38-45, config.rs:248-264) agree with the Python mirror's
+    import ctypes as C
+    lib = qsm.c_library()
+    for cls, fn, py in [(qsm._CTkdParams, "qsm_tkd_params_default", qsm.TkdParams()),
+                        (qsm._CTikhonovParams, "qsm_tikhonov_params_default", qsm.TikhonovParams()),
+                        (qsm._CRtsParams, "qsm_rts_params_default", qsm.RtsParams()),
+                        (qsm._CSharpParams, "qsm_sharp_params_default", qsm.SharpParams())]:
+        c = cls()
+        getattr(lib, fn)(C.byref(c))
+        pc = py._c()
+        assert [getattr(c, f[0]) for f in cls._fields_] == [getattr(pc, f[0]) for f in cls._fields_], fn
+    assert (qsm.RtsParams().delta, qsm.RtsParams().mu, qsm.RtsParams().rho) == (0.15, 1e5, 10.0)
+    cfg = qsm._CInversionConfig()
+    lib.qsm_inversion_config_default(C.byref(cfg))
+    assert cfg.algorithm == int(qsm.InversionAlgorithm.Rts) and cfg.tkd.threshold == 0.15
+    assert cfg.tv.max_iter == 250 and cfg.rts.max_iter == 20 and cfg.tikhonov.reg == 0
 
 
 def test_pipeline_dimension_mismatch_and_config_errors():
@@ -62,8 +79,13 @@ def test_pipeline_dimension_mismatch_and_config_errors():
         qsm.run_dipole_inversion(np.zeros(64), np.ones(64, np.uint8), meta,
                                  qsm.InversionConfig(algorithm=qsm.InversionAlgorithm.Tgv))
     assert e.value.kind == "InvalidConfig"
-    with pytest.raises(qsm.PipelineError) as e:  # default algorithm Rts has no GPU arm here
-        qsm.run_dipole_inversion(np.zeros(64), np.ones(64, np.uint8), meta, qsm.InversionConfig())
+    with pytest.raises(qsm.PipelineError) as e:  # Medi has no GPU arm here
+        qsm.run_dipole_inversion(np.zeros(64), np.ones(64, np.uint8), meta,
+                                 qsm.InversionConfig(algorithm=qsm.InversionAlgorithm.Medi))
+    assert e.value.kind == "InvalidConfig"
+    with pytest.raises(qsm.PipelineError) as e:
+        qsm.run_bg_removal(np.zeros(64), np.ones(64, np.uint8), meta,
+                           qsm.BgRemovalConfig(algorithm=qsm.BgRemovalAlgorithm.Pdf))
     assert e.value.kind == "InvalidConfig"
 
 

@@ -1,0 +1,124 @@
+"""GPU closed-form inversions (tkd / tsvd / tikhonov), RTS, SHARP and the
+pipeline arms that dispatch to them, against the f64 oracle.  FP32
+transforms: relative L2 error <= 1e-5 for the closed forms (one transform
+pair); RTS NRMSE <= 1e-3 (tests/common/mod.rs:34-51), tighter on small
+volumes."""
+import numpy as np
+import pytest
+
+from conftest import nrmse_dc
+
+pytestmark = pytest.mark.gpu
+
+
+def _rel_l2(a, b):
+    return float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300))
+
+
+def _case(seed, dims, frac=0.8):
+    rng = np.random.default_rng(seed)
+    n = int(np.prod(dims))
+    return rng.standard_normal(n) * 0.02, (rng.random(n) < frac).astype(np.uint8)
+
+
+SHAPES = [((16, 16, 16), (1.0, 1.0, 1.0), (0.0, 0.0, 1.0)),
+          ((20, 18, 22), (0.9, 1.1, 1.3), (0.2, -0.1, 0.97)),
+          ((64, 64, 64), (1.0, 1.0, 1.0), (0.1, 0.2, 1.0)),
+          ((128, 64, 64), (0.6, 0.6, 0.6), (0.0, 0.0, 1.0))]
+
+
+@pytest.mark.parametrize("dims,vs,b", SHAPES)
+def test_closed_forms_match_oracle(qsm, oracle, dims, vs, b):
+    f, m = _case(3, dims)
+    g = qsm.Grid(dims, vs)
+    got = {
+        "tkd": qsm.tkd(f, m, g, b, qsm.TkdParams(0.15)),
+        "tsvd": qsm.tsvd(f, m, g, b, qsm.TkdParams(0.15)),
+        "tik_id": qsm.tikhonov(f, m, g, b, qsm.TikhonovParams(0.01, qsm.Regularization.Identity)),
+        "tik_grad": qsm.tikhonov(f, m, g, b, qsm.TikhonovParams(0.01, qsm.Regularization.Gradient)),
+        "tik_lap": qsm.tikhonov(f, m, g, b, qsm.TikhonovParams(0.01, qsm.Regularization.Laplacian)),
+    }
+    ref = {
+        "tkd": oracle.tkd(f, m, dims, vs, b, 0.15),
+        "tsvd": oracle.tkd(f, m, dims, vs, b, 0.15, tsvd=True),
+        "tik_id": oracle.tikhonov(f, m, dims, vs, b, 0.01, 0),
+        "tik_grad": oracle.tikhonov(f, m, dims, vs, b, 0.01, 1),
+        "tik_lap": oracle.tikhonov(f, m, dims, vs, b, 0.01, 2),
+    }
+    for k in got:
+        e = _rel_l2(got[k], ref[k])
+        print(dims, k, "rel L2", e)
+        assert e <= 1e-5, k
+        assert np.all(got[k][m == 0] == 0.0)
+
+
+@pytest.mark.parametrize("dims,vs,b", SHAPES[:3])
+def test_rts_matches_oracle(qsm, oracle, dims, vs, b):
+    f, m = _case(4, dims)
+    g = qsm.Grid(dims, vs)
+    for tol, iters in [(0.0, 10), (1e-2, 20)]:
+        p = qsm.RtsParams(tol=tol, max_iter=iters)
+        calls, ocalls = [], []
+        x = qsm.rts(f, m, g, b, p, progress=lambda a, c: calls.append((a, c)))
+        xo, oit = oracle.rts(f, m, dims, vs, b, 0.15, 1e5, 10.0, tol, iters, 4,
+                             progress=lambda a, c: ocalls.append((a, c)), return_info=True)
+        # anisotropic voxels: the reference's DC term (FFT of the stencil leaves
+        # ~1e-16 at k = 0) is compared apart, as for TV
+        e = nrmse_dc(x, xo, m)
+        print(dims, "tol", tol, "gpu calls", len(calls), "oracle iters", oit, "nrmse(dc)", e,
+              "raw", oracle.nrmse(x, xo, m))
+        assert e <= (2e-5 if tol == 0.0 else 1e-3)
+        if tol == 0.0:
+            assert calls == ocalls
+        assert np.all(x[m == 0] == 0.0)
+
+
+def test_rts_default_pipeline_arm(qsm, oracle, phantom):
+    # InversionConfig::default() is Rts (src/pipeline/config.rs:251)
+    ph = phantom.make_phantom((64, 64, 64), n_ellipsoids=8, seed=1006)
+    meta = qsm.ScanMetadata(dims=ph.dims, voxel_size=ph.voxel_size, b0_direction=ph.bdir)
+    x = qsm.run_dipole_inversion(ph.local_field, ph.mask, meta, qsm.InversionConfig())
+    xo = oracle.rts(ph.local_field, ph.mask, ph.dims, ph.voxel_size, ph.bdir)
+    e = oracle.nrmse(x, xo, ph.mask)
+    print("pipeline rts 64^3 nrmse", e)
+    assert e <= 1e-3
+    cfg = qsm.InversionConfig(algorithm=qsm.InversionAlgorithm.Tkd)
+    t = qsm.run_dipole_inversion(ph.local_field, ph.mask, meta, cfg)
+    assert np.array_equal(t, qsm.tkd(ph.local_field, ph.mask, qsm.Grid(ph.dims, ph.voxel_size),
+                                     ph.bdir, qsm.TkdParams()))
+
+
+def test_rts_batch_equals_single(qsm):
+    shapes = [((16, 16, 16), (1.0, 1.0, 1.0), (0, 0, 1)), ((20, 18, 22), (1.0, 1.0, 1.0), (0.1, 0.2, 0.95))]
+    p = qsm.RtsParams(tol=0.0, max_iter=8)
+    keep, vols, singles, outs = [], [], [], []
+    for s, (dims, vs, b) in enumerate(shapes):
+        f, m = _case(10 + s, dims)
+        singles.append(qsm.rts(f, m, qsm.Grid(dims, vs), b, p))
+        tf, tm = qsm.DeviceArray.from_numpy(f), qsm.DeviceArray.from_numpy(m)
+        tc = qsm.DeviceArray(f.size, np.float64)
+        keep.append((tf, tm))
+        outs.append(tc)
+        vols.append((qsm.Grid(dims, vs), b, tf.ptr, tm.ptr, tc.ptr))
+    batch = qsm.TvBatch(vols, p)
+    batch.run()
+    for s in range(len(shapes)):
+        assert np.array_equal(outs[s].numpy(), singles[s])
+    batch.close()
+
+
+def test_sharp_matches_oracle_and_pipeline(qsm, oracle, phantom):
+    ph = phantom.add_background(phantom.make_phantom((64, 64, 64), n_ellipsoids=8, seed=1007),
+                                seed=1007)
+    g = qsm.Grid(ph.dims, ph.voxel_size)
+    p = qsm.SharpParams(0.05, 4.0)
+    local, em = qsm.sharp(ph.total_field, ph.mask, g, p)
+    lo, mo = oracle.sharp_core(ph.total_field, ph.mask, ph.dims, ph.voxel_size, 0.05, 4.0)
+    assert np.array_equal(em, mo)
+    e = _rel_l2(local, lo)
+    print("sharp 64^3 rel L2", e)
+    assert e <= 1e-4
+    meta = qsm.ScanMetadata(dims=ph.dims, voxel_size=ph.voxel_size)
+    r = qsm.run_bg_removal(ph.total_field, ph.mask, meta,
+                           qsm.BgRemovalConfig(algorithm=qsm.BgRemovalAlgorithm.Sharp, sharp=p))
+    assert np.array_equal(r.local_field_ppm, local) and np.array_equal(r.eroded_mask, em)
