@@ -206,6 +206,9 @@ int qsm_tikhonov(const double* local_field, const uint8_t* mask, const qsm_grid*
 /* rts, src/inversion/rts.rs:71-212 (progress as tv_admm's). */
 int qsm_rts(const double* local_field, const uint8_t* mask, const qsm_grid* grid, const double bdir[3],
             const qsm_rts_params* params, qsm_progress_fn progress, void* user, double* chi_out);
+/* laplacian_unwrap, src/unwrap/laplacian.rs:135-161 (wrapped phase in
+ * radians, f64; output masked). */
+int qsm_laplacian_unwrap(const double* phase, const uint8_t* mask, const qsm_grid* grid, double* out);
 /* sharp, src/bgremove/sharp.rs:48-56. */
 int qsm_sharp(const double* field, const uint8_t* mask, const qsm_grid* grid,
               const qsm_sharp_params* params, double* local_out, uint8_t* mask_out);

@@ -59,6 +59,8 @@ def lib():
             "orc_tv_admm": (None, [_D, _U8] + [_SZ] * 3 + [_F] * 9 + [_SZ, PROGRESS_FN,
                                    C.c_void_p, _D, C.POINTER(_SZ), _D]),
             "orc_tikhonov_gradient": (None, [_D, _U8] + [_SZ] * 3 + [_F] * 7 + [_D]),
+            "orc_wrap": (_F, [_F]),
+            "orc_laplacian_unwrap": (None, [_D, _U8] + [_SZ] * 3 + [_F] * 3 + [_D]),
             "orc_tkd": (None, [_D, _U8] + [_SZ] * 3 + [_F] * 7 + [C.c_int, _D]),
             "orc_tikhonov": (None, [_D, _U8] + [_SZ] * 3 + [_F] * 7 + [C.c_int, _D]),
             "orc_rts": (None, [_D, _U8] + [_SZ] * 3 + [_F] * 10 + [_SZ, _SZ, PROGRESS_FN,
@@ -240,6 +242,19 @@ def tv_unrolled_vjp(local_field, mask, dims, vs, bdir, lam, rho, K, chi_bar):
     lib().orc_tv_unrolled_vjp(_d(f), _u8(m), *dims, *vs, *bdir, lam, rho, K, _d(cb), _d(chi),
                               _d(fb), C.byref(lb))
     return chi, fb, float(lb.value)
+
+
+def wrap(x):
+    """src/unwrap/laplacian.rs:41-50"""
+    return float(lib().orc_wrap(float(x)))
+
+
+def laplacian_unwrap(phase, mask, dims, vs):
+    """src/unwrap/laplacian.rs:135-161"""
+    p, m = _f64(phase), _mask(mask)
+    out = np.empty(p.size)
+    lib().orc_laplacian_unwrap(_d(p), _u8(m), *dims, *vs, _d(out))
+    return out
 
 
 def tkd(local_field, mask, dims, vs, bdir, threshold, tsvd=False):

@@ -859,3 +859,59 @@ double orc_nrmse(const double* a, const double* b, const uint8_t* mask, size_t n
 }
 
 }  // extern "C"
+
+extern "C" {
+// src/unwrap/laplacian.rs:41-50
+double orc_wrap(double x) {
+    double y = std::fmod(x, 2.0 * M_PI);
+    if (y > M_PI) y -= 2.0 * M_PI;
+    else if (y < -M_PI) y += 2.0 * M_PI;
+    return y;
+}
+
+// src/unwrap/laplacian.rs:135-161 with wrapped_laplacian_periodic (:67-122)
+// and solve_poisson_fft (:93-133).
+void orc_laplacian_unwrap(const double* phase, const uint8_t* mask, size_t nx, size_t ny, size_t nz,
+                          double vsx, double vsy, double vsz, double* out) {
+    const size_t n = nx * ny * nz, nxy = nx * ny;
+    const double dx2 = 1.0 / (vsx * vsx), dy2 = 1.0 / (vsy * vsy), dz2 = 1.0 / (vsz * vsz);
+    std::vector<cd> f(n);
+    for (size_t k = 0; k < nz; ++k) {
+        const size_t km1 = k == 0 ? nz - 1 : k - 1, kp1 = k + 1 >= nz ? 0 : k + 1;
+        for (size_t j = 0; j < ny; ++j) {
+            const size_t jm1 = j == 0 ? ny - 1 : j - 1, jp1 = j + 1 >= ny ? 0 : j + 1;
+            for (size_t i = 0; i < nx; ++i) {
+                const size_t im1 = i == 0 ? nx - 1 : i - 1, ip1 = i + 1 >= nx ? 0 : i + 1;
+                const size_t idx = i + j * nx + k * nxy;
+                const double u = phase[idx];
+                const double lx = (orc_wrap(phase[ip1 + j * nx + k * nxy] - u) -
+                                   orc_wrap(u - phase[im1 + j * nx + k * nxy])) * dx2;
+                const double ly = (orc_wrap(phase[i + jp1 * nx + k * nxy] - u) -
+                                   orc_wrap(u - phase[i + jm1 * nx + k * nxy])) * dy2;
+                const double lz = (orc_wrap(phase[i + j * nx + kp1 * nxy] - u) -
+                                   orc_wrap(u - phase[i + j * nx + km1 * nxy])) * dz2;
+                f[idx] = cd(mask[idx] != 0 ? lx + ly + lz : 0.0, 0.0);
+            }
+        }
+    }
+    fft3d(f.data(), nx, ny, nz, false);
+    for (size_t k = 0; k < nz; ++k) {
+        const double fk = k <= nz / 2 ? (double)k / nz : ((double)k - (double)nz) / nz;
+        const double lz = 2.0 * (std::cos(2.0 * M_PI * fk) - 1.0) * dz2;
+        for (size_t j = 0; j < ny; ++j) {
+            const double fj = j <= ny / 2 ? (double)j / ny : ((double)j - (double)ny) / ny;
+            const double ly = 2.0 * (std::cos(2.0 * M_PI * fj) - 1.0) * dy2;
+            for (size_t i = 0; i < nx; ++i) {
+                const double fi = i <= nx / 2 ? (double)i / nx : ((double)i - (double)nx) / nx;
+                const double lx = 2.0 * (std::cos(2.0 * M_PI * fi) - 1.0) * dx2;
+                const double lam = lx + ly + lz;
+                const size_t idx = i + j * nx + k * nxy;
+                if (std::fabs(lam) > 1e-20) f[idx] /= lam;
+                else f[idx] = cd(0.0, 0.0);
+            }
+        }
+    }
+    fft3d(f.data(), nx, ny, nz, true);
+    for (size_t i = 0; i < n; ++i) out[i] = mask[i] != 0 ? f[i].real() : 0.0;
+}
+}

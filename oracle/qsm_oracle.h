@@ -102,6 +102,11 @@ void orc_rts(const double* local_field, const uint8_t* mask, size_t nx, size_t n
              size_t lsmr_iter, orc_progress_fn progress, void* user, double* chi_out,
              size_t* iters_out);
 
+/* src/unwrap/laplacian.rs:41-50 and :135-161. */
+double orc_wrap(double x);
+void orc_laplacian_unwrap(const double* phase, const uint8_t* mask, size_t nx, size_t ny, size_t nz,
+                          double vsx, double vsy, double vsz, double* out);
+
 /* src/inversion/tikhonov.rs:62-131 with Regularization::Gradient. */
 void orc_tikhonov_gradient(const double* local_field, const uint8_t* mask,
                            size_t nx, size_t ny, size_t nz, double vsx,

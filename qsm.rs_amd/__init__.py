@@ -149,6 +149,7 @@ _SIGS = {
     "qsm_rts_batch_create": (C.c_int, [C.POINTER(_CTvVolume), _SZ, C.POINTER(_CRtsParams),
                                        C.c_void_p, C.POINTER(C.c_void_p)]),
     "qsm_sharp": (C.c_int, [_D, _U8, C.POINTER(_CGrid), C.POINTER(_CSharpParams), _D, _U8]),
+    "qsm_laplacian_unwrap": (C.c_int, [_D, _U8, C.POINTER(_CGrid), _D]),
     "qsm_run_dipole_inversion_cfg": (C.c_int, [_D, _SZ, _U8, _SZ, C.POINTER(_CScanMetadata),
                                                C.POINTER(_CInversionConfig), PROGRESS_FN,
                                                C.c_void_p, _D]),
@@ -446,6 +447,17 @@ def rts(local_field, mask, grid: Grid, bdir, params: RtsParams = None,
     cb = _progress(progress)
     _check(_lib.qsm_rts(f.ctypes.data_as(_D), m.ctypes.data_as(_U8), C.byref(g),
                         b.ctypes.data_as(_D), C.byref(p), cb, None, out.ctypes.data_as(_D)))
+    return out
+
+
+def laplacian_unwrap(phase, mask, grid: Grid):
+    """src/unwrap/laplacian.rs:135-161 (wrapped phase in radians)"""
+    p, m = _flat64(phase), _flatu8(mask)
+    _check_len(grid, p, m)
+    out = np.empty(p.size)
+    g = grid._c()
+    _check(_lib.qsm_laplacian_unwrap(p.ctypes.data_as(_D), m.ctypes.data_as(_U8), C.byref(g),
+                                     out.ctypes.data_as(_D)))
     return out
 
 
@@ -869,5 +881,5 @@ __all__ = [
     "c_library", "LIB_PATH", "DeviceArray", "synchronize", "VsharpBatch", "vsharp_radii",
     "TvUnrolled", "tv_unrolled_vjp", "UnrolledTvFunction", "dipole_forward",
     "TkdParams", "Regularization", "TikhonovParams", "RtsParams", "SharpParams", "tkd", "tsvd",
-    "tikhonov", "rts", "sharp",
+    "tikhonov", "rts", "sharp", "laplacian_unwrap",
 ]

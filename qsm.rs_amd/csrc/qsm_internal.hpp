@@ -120,6 +120,7 @@ enum PassKind : int {
     P_C2R_ADJ, P_C2R_F64OUT,
     // closed-form inversions and RTS
     P_Z_FILTER, P_Z_RTS_SETUP, P_Z_TV_SOLVE0, P_C2R_F64_MASKED, P_C2R_X0, P_R2C_X0,
+    P_R2C_WRAPLAP,
     P_KIND_COUNT
 };
 

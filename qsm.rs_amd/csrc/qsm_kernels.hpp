@@ -201,8 +201,41 @@ enum SrcKind : int {
     SRC_TVADJ_FIRST = 6,  // adjoint source of the last iteration: x-bar = mask * chi-bar
     SRC_TVADJ = 7,        // adjoint source of an earlier iteration
     SRC_ACC = 8,          // the adjoint accumulator
-    SRC_X0 = 9            // x[0] (the RTS starting point)
+    SRC_X0 = 9,           // x[0] (the RTS starting point)
+    SRC_WRAPLAP = 10      // masked wrapped Laplacian of the f64 phase in field
 };
+
+// wrap into [-pi, pi] as src/unwrap/laplacian.rs:41-50 (f64 remainder).
+__device__ __forceinline__ double wrap_pi(double x) {
+    const double two_pi = 6.283185307179586476925286766559;
+    const double pi = 3.14159265358979323846264338327950288;
+    double y = fmod(x, two_pi);
+    if (y > pi) y -= two_pi;
+    else if (y < -pi) y += two_pi;
+    return y;
+}
+
+// Wrapped periodic Laplacian at voxel (i, row r), zero outside the mask
+// (src/unwrap/laplacian.rs:67-122 and the masking of :146-150), in FP64.
+__device__ __forceinline__ float wraplap_at(const VolDesc& V, int i, int r) {
+#pragma clang fp contract(off)
+    const int nx = V.nx, ny = V.ny, nz = V.nz;
+    const size_t idx = i + (size_t)r * nx;
+    if (!glb(V.mask)[idx]) return 0.0f;
+    const int k = (int)fastdiv((uint32_t)r, V.m_ny);
+    const int j = r - k * ny;
+    const size_t nxy = (size_t)nx * ny;
+    auto at = [&](int ii, int jj, int kk) { return glb(V.field)[ii + (size_t)jj * nx + (size_t)kk * nxy]; };
+    const double u = at(i, j, k);
+    const double uxp = at(i + 1 < nx ? i + 1 : 0, j, k), uxm = at(i > 0 ? i - 1 : nx - 1, j, k);
+    const double uyp = at(i, j + 1 < ny ? j + 1 : 0, k), uym = at(i, j > 0 ? j - 1 : ny - 1, k);
+    const double uzp = at(i, j, k + 1 < nz ? k + 1 : 0), uzm = at(i, j, k > 0 ? k - 1 : nz - 1);
+    const double dx2 = 1.0 / (V.vsx * V.vsx), dy2 = 1.0 / (V.vsy * V.vsy), dz2 = 1.0 / (V.vsz * V.vsz);
+    const double lx = (wrap_pi(uxp - u) - wrap_pi(u - uxm)) * dx2;
+    const double ly = (wrap_pi(uyp - u) - wrap_pi(u - uym)) * dy2;
+    const double lz = (wrap_pi(uzp - u) - wrap_pi(u - uzm)) * dz2;
+    return (float)(lx + ly + lz);
+}
 
 template <int SRC>
 __device__ __forceinline__ float simple_src(const VolDesc& V, int i, int r, int aux) {
@@ -211,6 +244,7 @@ __device__ __forceinline__ float simple_src(const VolDesc& V, int i, int r, int 
     else if constexpr (SRC == SRC_F32) return glb(V.vs_local)[i + (size_t)r * V.nx];
     else if constexpr (SRC == SRC_ACC) return glb(V.adj_acc)[i + (size_t)r * V.nx];
     else if constexpr (SRC == SRC_X0) return glb(V.x[0])[i + (size_t)r * V.nx];
+    else if constexpr (SRC == SRC_WRAPLAP) return wraplap_at(V, i, r);
     else return sphere_element(V, i, r, aux);
 }
 
@@ -637,8 +671,10 @@ __device__ inline double dipole_at(const VolDesc& V, int kx, int ky, int kz) {
 //   RTS step 1: lsmr_iter rounds of w += (1 - w D) D/(D^2 + 1e-6) from
 //     w = D/(D^2 + 1e-6) on the cone |D| > delta, 0 off it (the per-frequency
 //     refinement of src/inversion/rts.rs:117-150, applied to FFT(f))
+//   Poisson: 1/lambda with lambda = -L the periodic Laplacian's eigenvalue, 0
+//     where |lambda| <= 1e-20 (src/unwrap/laplacian.rs:93-133)
 enum Filter : int { FILT_TKD = 0, FILT_TSVD = 1, FILT_TIK_ID = 2, FILT_TIK_GRAD = 3, FILT_TIK_LAP = 4,
-                    FILT_RTS_X0 = 5 };
+                    FILT_RTS_X0 = 5, FILT_POISSON = 6 };
 
 __device__ inline double filter_mult(const VolDesc& V, int kx, int ky, int kz) {
 #pragma clang fp contract(off)
@@ -657,6 +693,10 @@ __device__ inline double filter_mult(const VolDesc& V, int kx, int ky, int kz) {
             const double g = V.filt == FILT_TIK_ID ? 1.0 : (V.filt == FILT_TIK_GRAD ? L : L * L);
             const double denom = D * D + p * g;
             return fabs(denom) > 1e-20 ? D / denom : 0.0;
+        }
+        case FILT_POISSON: {
+            const double lam = -L;
+            return fabs(lam) > 1e-20 ? 1.0 / lam : 0.0;
         }
         default: {  // FILT_RTS_X0
             if (!(fabs(D) > p)) return 0.0;

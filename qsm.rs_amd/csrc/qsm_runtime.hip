@@ -677,9 +677,14 @@ static int unrolled_backward_impl(qsm_tv_unrolled* U, const double* const* d_chi
 // operator is self-adjoint and the adjoint entry point runs the same passes.
 // zkind P_Z_DIPOLE (no mask, f64 out) or P_Z_FILTER (filter `filt` with
 // parameter `fa`, output masked by d_mask).
+// src_kind P_R2C_F64 (d_in as is) or P_R2C_WRAPLAP (masked wrapped
+// Laplacian of d_in); bdir may be null for filters that do not use D.
 static int spectral_apply(int zkind, int filt, double fa, const double* d_in, const uint8_t* d_mask,
-                          const qsm_grid* grid, const double bdir[3], double* d_out, void* stream) {
-    if (!d_in || !grid || !bdir || !d_out || (zkind == P_Z_FILTER && !d_mask))
+                          const qsm_grid* grid, const double bdir_in[3], double* d_out, void* stream,
+                          int src_kind = P_R2C_F64) {
+    static const double z_axis[3] = {0.0, 0.0, 1.0};
+    const double* bdir = bdir_in ? bdir_in : z_axis;
+    if (!d_in || !grid || !d_out || (zkind == P_Z_FILTER && !d_mask))
         return fail(QSM_EPARAM, "null argument");
     int rc = validate_grid(grid);
     if (rc) return rc;
@@ -733,7 +738,7 @@ static int spectral_apply(int zkind, int filt, double fa, const double* d_in, co
     a.tw = d_tw;
     a.sel = SPEC_WORK;
     a.sel2 = -1;
-    run_groups(P_R2C_F64, gx, a);
+    run_groups(src_kind, gx, a);
     run_groups(P_Y_FWD, gy, a);
     run_groups(zkind, gz, a);
     run_groups(P_Y_INV, gy, a);
@@ -1174,8 +1179,10 @@ int qsm_dipole_adjoint(const double* d_field, const qsm_grid* grid, const double
 
 // Host-array wrapper of a closed-form inversion (filter kind, parameter).
 static int filter_host(int filt, double fa, const double* field, const uint8_t* mask,
-                       const qsm_grid* grid, const double bdir[3], double* chi_out) {
-    if (!field || !mask || !grid || !bdir || !chi_out) return fail(QSM_EPARAM, "null argument");
+                       const qsm_grid* grid, const double bdir[3], double* chi_out,
+                       int src_kind = P_R2C_F64) {
+    if (!field || !mask || !grid || (!bdir && src_kind == P_R2C_F64) || !chi_out)
+        return fail(QSM_EPARAM, "null argument");
     int rc = validate_grid(grid);
     if (rc) return rc;
     const size_t N = grid->nx * grid->ny * grid->nz;
@@ -1191,10 +1198,18 @@ static int filter_host(int filt, double fa, const double* field, const uint8_t* 
     hipError_t e = hipMemcpy(d_f, field, N * sizeof(double), hipMemcpyHostToDevice);
     if (e == hipSuccess) e = hipMemcpy(d_m, mask, N, hipMemcpyHostToDevice);
     if (e != hipSuccess) return done(fail(QSM_EHIP, "hipMemcpy H2D: %s", hipGetErrorString(e)));
-    if ((rc = spectral_apply(P_Z_FILTER, filt, fa, d_f, d_m, grid, bdir, d_o, nullptr))) return done(rc);
+    if ((rc = spectral_apply(P_Z_FILTER, filt, fa, d_f, d_m, grid, bdir, d_o, nullptr, src_kind)))
+        return done(rc);
     e = hipMemcpy(chi_out, d_o, N * sizeof(double), hipMemcpyDeviceToHost);
     if (e != hipSuccess) return done(fail(QSM_EHIP, "hipMemcpy D2H: %s", hipGetErrorString(e)));
     return done(QSM_OK);
+}
+
+// laplacian_unwrap, src/unwrap/laplacian.rs:135-161: Poisson solve of the
+// masked wrapped Laplacian, masked.
+int qsm_laplacian_unwrap(const double* phase, const uint8_t* mask, const qsm_grid* grid, double* out) {
+    g_err.clear();
+    return filter_host(6, 0.0, phase, mask, grid, nullptr, out, P_R2C_WRAPLAP);
 }
 
 void qsm_tkd_params_default(qsm_tkd_params* p) {

@@ -11,6 +11,7 @@
 //   qsm::inversion::tikhonov        src/inversion/tikhonov.rs:62-131
 //   qsm::inversion::rts             src/inversion/rts.rs:71-212
 //   qsm::bgremove::sharp            src/bgremove/sharp.rs:48-56
+//   qsm::unwrap::laplacian_unwrap   src/unwrap/laplacian.rs:135-161
 //   qsm::pipeline::run_dipole_inversion / run_bg_removal  src/pipeline/*.rs
 //
 // Algorithm functions throw qsm::Error on a non-zero status (the Rust
@@ -219,6 +220,19 @@ inline std::pair<std::vector<double>, std::vector<uint8_t>> vsharp_with_radii(
 }
 
 }  // namespace bgremove
+
+namespace unwrap {
+// src/unwrap/laplacian.rs:135-161
+inline std::vector<double> laplacian_unwrap(const std::vector<double>& phase, const std::vector<uint8_t>& mask,
+                                            const Grid& grid) {
+    if (phase.size() != grid.n_total() || mask.size() != grid.n_total())
+        throw Error(QSM_EDIM, "array length does not match the grid");
+    std::vector<double> out(grid.n_total());
+    const qsm_grid g = grid.c();
+    check(qsm_laplacian_unwrap(phase.data(), mask.data(), &g, out.data()));
+    return out;
+}
+}  // namespace unwrap
 
 namespace pipeline {
 

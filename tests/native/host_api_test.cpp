@@ -257,6 +257,26 @@ static void test_pipeline_default_rts_and_sharp_arm() {
     EXPECT(s.ok && s.value.eroded_mask.size() == 512, "Sharp arm runs");
 }
 
+// src/unwrap/laplacian.rs:181-230 (constant -> constant, smooth, finite)
+static void test_laplacian_unwrap_reference_tests() {
+    const size_t n = 8;
+    Grid g = Grid::make(n, n, n, 1, 1, 1);
+    std::vector<uint8_t> m(n * n * n, 1);
+    auto u = unwrap::laplacian_unwrap(std::vector<double>(n * n * n, 1.0), m, g);
+    double mean = 0;
+    for (double v : u) mean += v;
+    mean /= u.size();
+    for (double v : u) EXPECT(std::fabs(v - mean) < 1e-6, "constant phase -> constant");
+    const size_t n16 = 16;
+    Grid g16 = Grid::make(n16, n16, n16, 1, 1, 1);
+    std::vector<double> ph(n16 * n16 * n16);
+    for (size_t k = 0; k < n16; ++k)
+        for (size_t j = 0; j < n16; ++j)
+            for (size_t i = 0; i < n16; ++i) ph[i + j * n16 + k * n16 * n16] = 0.5 * std::sin(2 * M_PI * i / n16);
+    auto s = unwrap::laplacian_unwrap(ph, std::vector<uint8_t>(ph.size(), 1), g16);
+    for (size_t i = 0; i < ph.size(); ++i) EXPECT(std::isfinite(s[i]) && std::fabs(ph[i] - s[i]) < 1.0, "smooth");
+}
+
 int main() {
     void (*tests[])() = {test_tv_admm_zero_field, test_tv_admm_finite, test_tv_admm_smoother_than_tkd,
                          test_tv_params_default, test_vsharp_zero_field, test_vsharp_nonuniform_voxels,
@@ -264,7 +284,8 @@ int main() {
                          test_vsharp_unsorted_radii, test_bg_removal_dispatches_vsharp,
                          test_bg_removal_validates_dims, test_inversion_tv_and_rejects_tgv,
                          test_tkd_reference_tests, test_tikhonov_regularization_types,
-                         test_rts_reference_tests, test_pipeline_default_rts_and_sharp_arm};
+                         test_rts_reference_tests, test_pipeline_default_rts_and_sharp_arm,
+                         test_laplacian_unwrap_reference_tests};
     int i = 0;
     for (auto t : tests) {
         const int before = g_fail;

@@ -122,3 +122,28 @@ def test_sharp_matches_oracle_and_pipeline(qsm, oracle, phantom):
     r = qsm.run_bg_removal(ph.total_field, ph.mask, meta,
                            qsm.BgRemovalConfig(algorithm=qsm.BgRemovalAlgorithm.Sharp, sharp=p))
     assert np.array_equal(r.local_field_ppm, local) and np.array_equal(r.eroded_mask, em)
+
+
+@pytest.mark.parametrize("dims,vs", [((16, 12, 10), (1.0, 0.8, 1.2)), ((64, 64, 64), (1.0, 1.0, 1.0)),
+                                     ((128, 128, 64), (0.9, 0.9, 0.9))])
+def test_laplacian_unwrap_matches_oracle(qsm, oracle, dims, vs):
+    # wrapped smooth phase with mask: GPU vs oracle; with an all-ones mask the
+    # unwrapped phase is the true phase minus its mean
+    i, j, k = [np.arange(q) for q in dims]
+    true = (6.0 * np.sin(2 * np.pi * i[:, None, None] / dims[0])
+            + 4.0 * np.cos(2 * np.pi * j[None, :, None] / dims[1])
+            + 3.0 * np.sin(2 * np.pi * k[None, None, :] / dims[2])).ravel(order="F")
+    wrapped = np.mod(true + np.pi, 2 * np.pi) - np.pi
+    rng = np.random.default_rng(1)
+    m = (rng.random(true.size) < 0.9).astype(np.uint8)
+    g = qsm.Grid(dims, vs)
+    got = qsm.laplacian_unwrap(wrapped, m, g)
+    ref = oracle.laplacian_unwrap(wrapped, m, dims, vs)
+    e = _rel_l2(got, ref)
+    ones = np.ones(true.size, np.uint8)
+    full = qsm.laplacian_unwrap(wrapped, ones, g)
+    ef = float(np.max(np.abs(full - (true - true.mean()))))
+    print(dims, "unwrap rel L2 vs oracle", e, "max err vs truth", ef)
+    assert e <= 1e-5
+    assert ef <= 1e-4 * np.max(np.abs(true))
+    assert np.all(got[m == 0] == 0.0)

@@ -131,3 +131,25 @@ def test_rts_reference_unit_tests(oracle):
         oracle.set_num_threads(one)
     b = oracle.rts(f16, np.ones(16 ** 3, np.uint8), (16,) * 3, (1, 1, 1), (0, 0, 1), **kw16)
     assert np.max(np.abs(a - b)) < 1e-10
+
+
+def test_laplacian_unwrap_reference_tests_and_closed_form(oracle):
+    # src/unwrap/laplacian.rs:170-230
+    pi = np.pi
+    for x, w in [(0.0, 0.0), (pi, pi), (-pi, -pi), (2 * pi, 0.0), (3 * pi, pi), (-3 * pi, -pi)]:
+        assert abs(oracle.wrap(x) - w) < 1e-10
+    n = 8
+    ones = np.ones(n ** 3, np.uint8)
+    u = oracle.laplacian_unwrap(np.ones(n ** 3), ones, (n, n, n), (1, 1, 1))
+    assert np.max(np.abs(u - u.mean())) < 1e-6
+    # closed form: a smooth periodic phase whose neighbour differences stay
+    # below pi, wrapped into [-pi, pi], unwraps to itself minus its mean
+    dims, vs = (16, 12, 10), (1.0, 0.8, 1.2)
+    i, j, k = [np.arange(q) for q in dims]
+    true = (6.0 * np.sin(2 * pi * i[:, None, None] / dims[0])
+            + 4.0 * np.cos(2 * pi * j[None, :, None] / dims[1])
+            + 3.0 * np.sin(2 * pi * k[None, None, :] / dims[2])).ravel(order="F")
+    wrapped = np.array([oracle.wrap(v) for v in true])
+    ones = np.ones(true.size, np.uint8)
+    got = oracle.laplacian_unwrap(wrapped, ones, dims, vs)
+    assert np.max(np.abs(got - (true - true.mean()))) < 1e-10
