@@ -109,6 +109,15 @@ typedef struct qsm_rts_params {
     size_t lsmr_iter;  /* 4 */
 } qsm_rts_params;
 
+/* NltvParams, src/inversion/nltv.rs:24-49. */
+typedef struct qsm_nltv_params {
+    double lambda;       /* 1e-3 (rho = 100 lambda) */
+    double mu;           /* 1 */
+    double tol;          /* 1e-3 */
+    size_t max_iter;     /* 250 ADMM steps per Newton round */
+    size_t newton_iter;  /* 10 */
+} qsm_nltv_params;
+
 /* SharpParams, src/bgremove/sharp.rs:27-45. */
 typedef struct qsm_sharp_params {
     double threshold;  /* 0.05 */
@@ -123,6 +132,7 @@ typedef struct qsm_inversion_config {
     qsm_tikhonov_params tikhonov;
     qsm_tv_params tv;
     qsm_rts_params rts;
+    qsm_nltv_params nltv;
 } qsm_inversion_config;
 
 /* BgRemovalConfig, src/pipeline/config.rs:202-229, for the arms served here. */
@@ -138,6 +148,7 @@ void qsm_tkd_params_default(qsm_tkd_params* p);
 void qsm_tikhonov_params_default(qsm_tikhonov_params* p);
 void qsm_rts_params_default(qsm_rts_params* p);
 void qsm_sharp_params_default(qsm_sharp_params* p);
+void qsm_nltv_params_default(qsm_nltv_params* p);
 void qsm_inversion_config_default(qsm_inversion_config* c);
 void qsm_bg_removal_config_default(qsm_bg_removal_config* c);
 
@@ -184,7 +195,7 @@ int qsm_vsharp_with_radii(const double* field, const uint8_t* mask, const qsm_gr
 /* ---- pipeline entry points --------------------------------------------- */
 
 /* run_dipole_inversion, src/pipeline/inversion.rs:28-35.  Serves the Tkd,
- * Tsvd, Tikhonov, Tv and Rts arms (tv_params for Tv, defaults for the
+ * Tsvd, Tikhonov, Tv, Rts and Nltv arms (tv_params for Tv, defaults for the
  * others; qsm_run_dipole_inversion_cfg takes every parameter set); Tgv and
  * Qsmart, and arms without a GPU backend here, return QSM_ECONFIG.  Lengths
  * are checked as the reference does (DimensionMismatch -> QSM_EDIM). */
@@ -206,6 +217,10 @@ int qsm_tikhonov(const double* local_field, const uint8_t* mask, const qsm_grid*
 /* rts, src/inversion/rts.rs:71-212 (progress as tv_admm's). */
 int qsm_rts(const double* local_field, const uint8_t* mask, const qsm_grid* grid, const double bdir[3],
             const qsm_rts_params* params, qsm_progress_fn progress, void* user, double* chi_out);
+/* nltv, src/inversion/nltv.rs:63-133 (progress (current, total) before every
+ * step, total = max_iter * newton_iter; no final call on convergence). */
+int qsm_nltv(const double* local_field, const uint8_t* mask, const qsm_grid* grid, const double bdir[3],
+             const qsm_nltv_params* params, qsm_progress_fn progress, void* user, double* chi_out);
 /* laplacian_unwrap, src/unwrap/laplacian.rs:135-161 (wrapped phase in
  * radians, f64; output masked). */
 int qsm_laplacian_unwrap(const double* phase, const uint8_t* mask, const qsm_grid* grid, double* out);

@@ -61,6 +61,8 @@ def lib():
             "orc_tikhonov_gradient": (None, [_D, _U8] + [_SZ] * 3 + [_F] * 7 + [_D]),
             "orc_wrap": (_F, [_F]),
             "orc_laplacian_unwrap": (None, [_D, _U8] + [_SZ] * 3 + [_F] * 3 + [_D]),
+            "orc_nltv": (None, [_D, _U8] + [_SZ] * 3 + [_F] * 9 + [_SZ, _SZ, PROGRESS_FN,
+                                                                 C.c_void_p, _D]),
             "orc_tkd": (None, [_D, _U8] + [_SZ] * 3 + [_F] * 7 + [C.c_int, _D]),
             "orc_tikhonov": (None, [_D, _U8] + [_SZ] * 3 + [_F] * 7 + [C.c_int, _D]),
             "orc_rts": (None, [_D, _U8] + [_SZ] * 3 + [_F] * 10 + [_SZ, _SZ, PROGRESS_FN,
@@ -254,6 +256,17 @@ def laplacian_unwrap(phase, mask, dims, vs):
     p, m = _f64(phase), _mask(mask)
     out = np.empty(p.size)
     lib().orc_laplacian_unwrap(_d(p), _u8(m), *dims, *vs, _d(out))
+    return out
+
+
+def nltv(local_field, mask, dims, vs, bdir, lam=1e-3, mu=1.0, tol=1e-3, max_iter=250,
+         newton_iter=10, progress=None):
+    """src/inversion/nltv.rs:63-133"""
+    f, m = _f64(local_field), _mask(mask)
+    out = np.empty(f.size)
+    cb = _cb(progress)
+    lib().orc_nltv(_d(f), _u8(m), *dims, *vs, *bdir, lam, mu, tol, max_iter, newton_iter, cb, None,
+                   _d(out))
     return out
 
 

@@ -915,3 +915,62 @@ void orc_laplacian_unwrap(const double* phase, const uint8_t* mask, size_t nx, s
     for (size_t i = 0; i < n; ++i) out[i] = mask[i] != 0 ? f[i].real() : 0.0;
 }
 }
+
+extern "C" {
+// src/inversion/nltv.rs:63-133: rho = 100 lambda; inner ADMM loops
+// (admm_step with weighted_shrink, src/utils/mod.rs:55-59) that stop at
+// convergence; after each, g is overwritten with grad(x) (which the next
+// inner loop's first step then uses) and the weights become
+// 1/(|grad x| + mu 1e-6), divided by their maximum when that exceeds 1.
+void orc_nltv(const double* local_field, const uint8_t* mask, size_t nx, size_t ny, size_t nz,
+              double vsx, double vsy, double vsz, double bx, double by, double bz, double lambda,
+              double mu, double tol, size_t max_iter, size_t newton_iter, orc_progress_fn progress,
+              void* user, double* chi_out) {
+    const size_t n = nx * ny * nz;
+    const double eps = 1e-6;
+    const double rho = 100.0 * lambda;
+    std::vector<double> inv_a(n);
+    std::vector<cd> f_hat(n);
+    orc_prepare_admm_spectral(local_field, nx, ny, nz, vsx, vsy, vsz, bx, by, bz, rho, inv_a.data(),
+                              reinterpret_cast<double*>(f_hat.data()));
+    std::vector<double> x(n, 0.0), x_prev(n, 0.0), ux(n, 0.0), uy(n, 0.0), uz(n, 0.0), gx(n, 0.0),
+        gy(n, 0.0), gz(n, 0.0), div(n, 0.0), w(n, 1.0);
+    std::vector<cd> work(n);
+    const size_t total = max_iter * newton_iter;
+    size_t current = 0;
+    const double t = lambda / rho;
+    for (size_t newton = 0; newton < newton_iter; ++newton) {
+        for (size_t it = 0; it < max_iter; ++it) {
+            ++current;
+            if (progress) progress(current, total, user);
+            std::swap(x, x_prev);
+            orc_bdiv(div.data(), gx.data(), gy.data(), gz.data(), nx, ny, nz, vsx, vsy, vsz);
+            for (size_t i = 0; i < n; ++i) work[i] = cd(div[i], 0.0);
+            fft3d(work.data(), nx, ny, nz, false);
+            for (size_t i = 0; i < n; ++i) work[i] = f_hat[i] - rho * work[i] * inv_a[i];
+            fft3d(work.data(), nx, ny, nz, true);
+            for (size_t i = 0; i < n; ++i) x[i] = work[i].real();
+            if (orc_relative_change(x.data(), x_prev.data(), n) < tol) break;
+            orc_fgrad(gx.data(), gy.data(), gz.data(), x.data(), nx, ny, nz, vsx, vsy, vsz);
+            for (size_t i = 0; i < n; ++i) {
+                const double ti = t * w[i];
+                double vx = gx[i] + ux[i], vy = gy[i] + uy[i], vz = gz[i] + uz[i];
+                double zx = orc_shrink(vx, ti), zy = orc_shrink(vy, ti), zz = orc_shrink(vz, ti);
+                ux[i] = vx - zx; uy[i] = vy - zy; uz[i] = vz - zz;
+                gx[i] = 2.0 * zx - vx; gy[i] = 2.0 * zy - vy; gz[i] = 2.0 * zz - vz;
+            }
+        }
+        orc_fgrad(gx.data(), gy.data(), gz.data(), x.data(), nx, ny, nz, vsx, vsy, vsz);
+        double wmax = 0.0;
+        for (size_t i = 0; i < n; ++i) {
+            const double gm = std::sqrt(gx[i] * gx[i] + gy[i] * gy[i] + gz[i] * gz[i]);
+            w[i] = 1.0 / (gm + mu * eps);
+            wmax = std::max(wmax, w[i]);
+        }
+        if (wmax > 1.0)
+            for (size_t i = 0; i < n; ++i) w[i] /= wmax;
+    }
+    orc_apply_mask_zero(x.data(), mask, n);
+    std::memcpy(chi_out, x.data(), n * sizeof(double));
+}
+}

@@ -107,6 +107,12 @@ double orc_wrap(double x);
 void orc_laplacian_unwrap(const double* phase, const uint8_t* mask, size_t nx, size_t ny, size_t nz,
                           double vsx, double vsy, double vsz, double* out);
 
+/* src/inversion/nltv.rs:63-133 */
+void orc_nltv(const double* local_field, const uint8_t* mask, size_t nx, size_t ny, size_t nz,
+              double vsx, double vsy, double vsz, double bx, double by, double bz, double lambda,
+              double mu, double tol, size_t max_iter, size_t newton_iter, orc_progress_fn progress,
+              void* user, double* chi_out);
+
 /* src/inversion/tikhonov.rs:62-131 with Regularization::Gradient. */
 void orc_tikhonov_gradient(const double* local_field, const uint8_t* mask,
                            size_t nx, size_t ny, size_t nz, double vsx,

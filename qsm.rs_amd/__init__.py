@@ -77,13 +77,19 @@ class _CRtsParams(C.Structure):
                 ("max_iter", C.c_size_t), ("lsmr_iter", C.c_size_t)]
 
 
+class _CNltvParams(C.Structure):
+    _fields_ = [("lambda_", C.c_double), ("mu", C.c_double), ("tol", C.c_double),
+                ("max_iter", C.c_size_t), ("newton_iter", C.c_size_t)]
+
+
 class _CSharpParams(C.Structure):
     _fields_ = [("threshold", C.c_double), ("radius", C.c_double)]
 
 
 class _CInversionConfig(C.Structure):
     _fields_ = [("algorithm", C.c_int), ("tkd", _CTkdParams), ("tsvd", _CTkdParams),
-                ("tikhonov", _CTikhonovParams), ("tv", _CTvParams), ("rts", _CRtsParams)]
+                ("tikhonov", _CTikhonovParams), ("tv", _CTvParams), ("rts", _CRtsParams),
+                ("nltv", _CNltvParams)]
 
 
 class _CBgRemovalConfig(C.Structure):
@@ -150,6 +156,9 @@ _SIGS = {
                                        C.c_void_p, C.POINTER(C.c_void_p)]),
     "qsm_sharp": (C.c_int, [_D, _U8, C.POINTER(_CGrid), C.POINTER(_CSharpParams), _D, _U8]),
     "qsm_laplacian_unwrap": (C.c_int, [_D, _U8, C.POINTER(_CGrid), _D]),
+    "qsm_nltv_params_default": (None, [C.POINTER(_CNltvParams)]),
+    "qsm_nltv": (C.c_int, [_D, _U8, C.POINTER(_CGrid), _D, C.POINTER(_CNltvParams), PROGRESS_FN,
+                           C.c_void_p, _D]),
     "qsm_run_dipole_inversion_cfg": (C.c_int, [_D, _SZ, _U8, _SZ, C.POINTER(_CScanMetadata),
                                                C.POINTER(_CInversionConfig), PROGRESS_FN,
                                                C.c_void_p, _D]),
@@ -312,6 +321,20 @@ class RtsParams:
 
 
 @dataclass
+class NltvParams:
+    """src/inversion/nltv.rs:24-49 (``lambda`` is spelled ``lambda_``)"""
+    lambda_: float = 1e-3
+    mu: float = 1.0
+    tol: float = 1e-3
+    max_iter: int = 250
+    newton_iter: int = 10
+
+    def _c(self):
+        return _CNltvParams(float(self.lambda_), float(self.mu), float(self.tol), int(self.max_iter),
+                            int(self.newton_iter))
+
+
+@dataclass
 class SharpParams:
     """src/bgremove/sharp.rs:27-45"""
     threshold: float = 0.05
@@ -461,6 +484,21 @@ def laplacian_unwrap(phase, mask, grid: Grid):
     return out
 
 
+def nltv(local_field, mask, grid: Grid, bdir, params: NltvParams = None,
+         progress: Optional[Callable[[int, int], None]] = None):
+    """src/inversion/nltv.rs:63-133"""
+    f, m = _flat64(local_field), _flatu8(mask)
+    _check_len(grid, f, m)
+    out = np.empty(f.size)
+    g = grid._c()
+    b = np.ascontiguousarray(np.asarray(bdir, dtype=np.float64))
+    p = (params or NltvParams())._c()
+    cb = _progress(progress)
+    _check(_lib.qsm_nltv(f.ctypes.data_as(_D), m.ctypes.data_as(_U8), C.byref(g),
+                         b.ctypes.data_as(_D), C.byref(p), cb, None, out.ctypes.data_as(_D)))
+    return out
+
+
 def sharp(field, mask, grid: Grid, params: SharpParams = None):
     """src/bgremove/sharp.rs:48-56 -> (local_field, eroded_mask)"""
     f, m = _flat64(field), _flatu8(mask)
@@ -525,17 +563,18 @@ class ScanMetadata:
 @dataclass
 class InversionConfig:
     """src/pipeline/config.rs:233-264 (the arms with a GPU backend: Tkd, Tsvd,
-    Tikhonov, Tv, Rts)."""
+    Tikhonov, Tv, Rts, Nltv)."""
     algorithm: InversionAlgorithm = InversionAlgorithm.Rts
     tkd: TkdParams = field(default_factory=TkdParams)
     tsvd: TkdParams = field(default_factory=TkdParams)
     tikhonov: TikhonovParams = field(default_factory=TikhonovParams)
     tv: TvParams = field(default_factory=TvParams)
     rts: RtsParams = field(default_factory=RtsParams)
+    nltv: NltvParams = field(default_factory=NltvParams)
 
     def _c(self):
         return _CInversionConfig(int(self.algorithm), self.tkd._c(), self.tsvd._c(),
-                                 self.tikhonov._c(), self.tv._c(), self.rts._c())
+                                 self.tikhonov._c(), self.tv._c(), self.rts._c(), self.nltv._c())
 
 
 @dataclass
@@ -881,5 +920,5 @@ __all__ = [
     "c_library", "LIB_PATH", "DeviceArray", "synchronize", "VsharpBatch", "vsharp_radii",
     "TvUnrolled", "tv_unrolled_vjp", "UnrolledTvFunction", "dipole_forward",
     "TkdParams", "Regularization", "TikhonovParams", "RtsParams", "SharpParams", "tkd", "tsvd",
-    "tikhonov", "rts", "sharp", "laplacian_unwrap",
+    "tikhonov", "rts", "sharp", "laplacian_unwrap", "NltvParams", "nltv",
 ]

@@ -82,7 +82,11 @@ struct VolDesc {
     int filt;                    // Filter kind (ZOP_FILTER)
     int lsmr_iter;               // RTS step-1 refinement rounds
     double filt_a;               // threshold (TKD/TSVD), lambda (Tikhonov), delta (RTS)
-    double filt_b;               // mu (RTS)
+    double filt_b;               // mu (RTS, NLTV)
+    // NLTV shrink weights (ping-pong) and their normalisations
+    float* nltv_w[2];
+    double* nltv_wnorm;          // 2 doubles (device)
+    double* nltv_wpart;          // per-block maxima of the weight pass
 };
 
 __device__ __host__ inline int find_vol(const LaunchTable& lt, int bid) {
@@ -120,7 +124,7 @@ enum PassKind : int {
     P_C2R_ADJ, P_C2R_F64OUT,
     // closed-form inversions and RTS
     P_Z_FILTER, P_Z_RTS_SETUP, P_Z_TV_SOLVE0, P_C2R_F64_MASKED, P_C2R_X0, P_R2C_X0,
-    P_R2C_WRAPLAP,
+    P_R2C_WRAPLAP, P_R2C_TVDIV_W, P_R2C_NLTV_BND,
     P_KIND_COUNT
 };
 

@@ -202,7 +202,9 @@ enum SrcKind : int {
     SRC_TVADJ = 7,        // adjoint source of an earlier iteration
     SRC_ACC = 8,          // the adjoint accumulator
     SRC_X0 = 9,           // x[0] (the RTS starting point)
-    SRC_WRAPLAP = 10      // masked wrapped Laplacian of the f64 phase in field
+    SRC_WRAPLAP = 10,     // masked wrapped Laplacian of the f64 phase in field
+    SRC_TVDIV_W = 11,     // SRC_TVDIV with NLTV weights (buffer ck)
+    SRC_NLTV_BND = 12     // first step of a later NLTV Newton round (previous weights ck)
 };
 
 // wrap into [-pi, pi] as src/unwrap/laplacian.rs:41-50 (f64 remainder).
@@ -280,18 +282,22 @@ __device__ __forceinline__ void fill_simple(float2* lds, const VolDesc& V, int n
 // arithmetic; restrict-qualified so the u' stores of one round do not order
 // the next round's loads.
 // The operands of CH elements (2*CH voxels) of the divergence.
-template <int CH>
+// W: per-voxel shrink weights (NLTV, weighted_shrink of src/utils/mod.rs:
+// 55-59): wv = w at i, i-e_x, i-e_y, i-e_z.
+template <int CH, bool W = false>
 struct TvDivRound {
     float xv[CH][2][7], uv[CH][2][6];
+    float wv[CH][2][W ? 4 : 1];
     int idx[CH][2];
     bool ok[CH][2];
 };
 
-template <int CH>
-__device__ __forceinline__ void tvdiv_load(TvDivRound<CH>& R, const VolDesc& V, int nx, int t0, int EMAX,
+template <int CH, bool W = false>
+__device__ __forceinline__ void tvdiv_load(TvDivRound<CH, W>& R, const VolDesc& V, int nx, int t0, int EMAX,
                                            int total, int row0, int nrows,
                                            const QSM_GLOBAL float* __restrict__ x,
-                                           const QSM_GLOBAL float* __restrict__ u) {
+                                           const QSM_GLOBAL float* __restrict__ u,
+                                           const QSM_GLOBAL float* __restrict__ w = nullptr) {
     const int ny = V.ny, nz = V.nz;
     const int nxy = nx * ny, N = nxy * nz;
 #pragma unroll
@@ -329,6 +335,12 @@ __device__ __forceinline__ void tvdiv_load(TvDivRound<CH>& R, const VolDesc& V, 
             R.uv[c][h][3] = u[idx + dxm];
             R.uv[c][h][4] = u[N + idx + dym];
             R.uv[c][h][5] = u[2 * N + idx + dzm];
+            if constexpr (W) {
+                R.wv[c][h][0] = w[idx];
+                R.wv[c][h][1] = w[idx + dxm];
+                R.wv[c][h][2] = w[idx + dym];
+                R.wv[c][h][3] = w[idx + dzm];
+            }
         }
     }
 }
@@ -337,11 +349,11 @@ __device__ __forceinline__ int8_t shrink_sign(float v, float t) {
     return v > t ? (int8_t)1 : (v < -t ? (int8_t)-1 : (int8_t)0);
 }
 
-template <int CH, bool REC>
-__device__ __forceinline__ void tvdiv_compute(const TvDivRound<CH>& R, float2* lds, const VolDesc& V,
+template <int CH, bool REC, bool W = false>
+__device__ __forceinline__ void tvdiv_compute(const TvDivRound<CH, W>& R, float2* lds, const VolDesc& V,
                                               int t0, int EMAX, int total,
                                               QSM_GLOBAL float* __restrict__ uo,
-                                              QSM_GLOBAL int8_t* __restrict__ rec) {
+                                              QSM_GLOBAL int8_t* __restrict__ rec, float wnorm = 1.0f) {
     const int N = V.nx * V.ny * V.nz;
     const float t = V.thr, hx = V.hx, hy = V.hy, hz = V.hz;
 #pragma unroll
@@ -357,7 +369,16 @@ __device__ __forceinline__ void tvdiv_compute(const TvDivRound<CH>& R, float2* l
             const float vxm = (xc - R.xv[c][h][4]) * hx + R.uv[c][h][3];
             const float vym = (xc - R.xv[c][h][5]) * hy + R.uv[c][h][4];
             const float vzm = (xc - R.xv[c][h][6]) * hz + R.uv[c][h][5];
-            const float zx = shrinkf(vx, t), zy = shrinkf(vy, t), zz = shrinkf(vz, t);
+            // thresholds lambda/rho, times the weight (divided by its
+            // normalisation) of the voxel each component belongs to
+            float ti = t, txm = t, tym = t, tzm = t;
+            if constexpr (W) {
+                ti = t * (R.wv[c][h][0] / wnorm);
+                txm = t * (R.wv[c][h][1] / wnorm);
+                tym = t * (R.wv[c][h][2] / wnorm);
+                tzm = t * (R.wv[c][h][3] / wnorm);
+            }
+            const float zx = shrinkf(vx, ti), zy = shrinkf(vy, ti), zz = shrinkf(vz, ti);
             if (R.ok[c][h]) {
                 const int idx = R.idx[c][h];
                 uo[idx] = vx - zx;
@@ -371,9 +392,9 @@ __device__ __forceinline__ void tvdiv_compute(const TvDivRound<CH>& R, float2* l
                 }
             }
             const float gx = 2.0f * zx - vx, gy = 2.0f * zy - vy, gz = 2.0f * zz - vz;
-            const float gxm = 2.0f * shrinkf(vxm, t) - vxm;
-            const float gym = 2.0f * shrinkf(vym, t) - vym;
-            const float gzm = 2.0f * shrinkf(vzm, t) - vzm;
+            const float gxm = 2.0f * shrinkf(vxm, txm) - vxm;
+            const float gym = 2.0f * shrinkf(vym, tym) - vym;
+            const float gzm = 2.0f * shrinkf(vzm, tzm) - vzm;
             d[h] = R.ok[c][h] ? (gx - gxm) * hx + (gy - gym) * hy + (gz - gzm) * hz : 0.0f;
         }
         if (t0 + c < EMAX && e < total) lds[e] = make_float2(d[0], d[1]);
@@ -388,25 +409,88 @@ __device__ __forceinline__ void tvdiv_compute(const TvDivRound<CH>& R, float2* l
 // from x and u at i-1 (likewise y and z), so g never goes to memory.
 // Rounds of CH elements (2*CH voxels) are software-pipelined: the loads of
 // round r+1 are issued before round r is computed and stored.
-template <int EMAX, bool REC>
+template <int EMAX, bool REC, bool W = false>
 __device__ __forceinline__ void fill_tvdiv(float2* lds, const VolDesc& V, int nx, int RB,
                                            const QSM_GLOBAL float* __restrict__ x,
                                            const QSM_GLOBAL float* __restrict__ u,
                                            QSM_GLOBAL float* __restrict__ uo, int row0, int nrows,
-                                           QSM_GLOBAL int8_t* __restrict__ rec) {
-    constexpr int CH = 2;
+                                           QSM_GLOBAL int8_t* __restrict__ rec,
+                                           const QSM_GLOBAL float* __restrict__ w = nullptr,
+                                           float wnorm = 1.0f) {
+    constexpr int CH = W ? 1 : 2;
     constexpr int NR = (EMAX + CH - 1) / CH;
     const int total = RB * nx;
-    TvDivRound<CH> A, B;
-    tvdiv_load<CH>(A, V, nx, 0, EMAX, total, row0, nrows, x, u);
+    TvDivRound<CH, W> A, B;
+    tvdiv_load<CH, W>(A, V, nx, 0, EMAX, total, row0, nrows, x, u, w);
 #pragma unroll
     for (int r = 0; r < NR; r += 2) {
-        if (r + 1 < NR) tvdiv_load<CH>(B, V, nx, (r + 1) * CH, EMAX, total, row0, nrows, x, u);
-        tvdiv_compute<CH, REC>(A, lds, V, r * CH, EMAX, total, uo, rec);
+        if (r + 1 < NR) tvdiv_load<CH, W>(B, V, nx, (r + 1) * CH, EMAX, total, row0, nrows, x, u, w);
+        tvdiv_compute<CH, REC, W>(A, lds, V, r * CH, EMAX, total, uo, rec, wnorm);
         if (r + 1 < NR) {
-            if (r + 2 < NR) tvdiv_load<CH>(A, V, nx, (r + 2) * CH, EMAX, total, row0, nrows, x, u);
-            tvdiv_compute<CH, REC>(B, lds, V, (r + 1) * CH, EMAX, total, uo, rec);
+            if (r + 2 < NR) tvdiv_load<CH, W>(A, V, nx, (r + 2) * CH, EMAX, total, row0, nrows, x, u, w);
+            tvdiv_compute<CH, REC, W>(B, lds, V, (r + 1) * CH, EMAX, total, uo, rec, wnorm);
         }
+    }
+}
+
+// First step of an NLTV Newton round after the first (src/inversion/
+// nltv.rs:86-120): the previous round ended either at convergence (no z/u
+// update) or after max_iter steps (a z/u update with the previous weights is
+// still due); then g was overwritten with grad(x).  So: u' = u or
+// v - shrink(v, t w_old) with v = grad x + u, and the divergence source is
+// bdiv(grad x).
+template <int EMAX>
+__device__ __forceinline__ void fill_nltv_boundary(float2* lds, const VolDesc& V, int nx, int RB,
+                                                   const QSM_GLOBAL float* __restrict__ x,
+                                                   const QSM_GLOBAL float* __restrict__ u,
+                                                   QSM_GLOBAL float* __restrict__ uo, int row0,
+                                                   int nrows, bool converged,
+                                                   const QSM_GLOBAL float* __restrict__ w, float wnorm) {
+    const int ny = V.ny, nz = V.nz;
+    const int nxy = nx * ny, N = nxy * nz;
+    const float t = V.thr, hx = V.hx, hy = V.hy, hz = V.hz;
+    const int total = RB * nx;
+#pragma unroll
+    for (int tt = 0; tt < EMAX; ++tt) {
+        const int e = threadIdx.x + tt * kThreads;
+        const int p = e / nx;
+        const int i = e - p * nx;
+        float d[2];
+#pragma unroll
+        for (int h = 0; h < 2; ++h) {
+            const int r = row0 + 2 * p + h;
+            const bool ok = (e < total) && (r < nrows);
+            const int rr = ok ? r : 0;
+            const int ii = ok ? i : 0;
+            const int k = (int)fastdiv((uint32_t)rr, V.m_ny);
+            const int j = rr - k * ny;
+            const int idx = ii + rr * nx;
+            const int dxp = (ii + 1 < nx) ? 1 : 1 - nx;
+            const int dxm = (ii > 0) ? -1 : nx - 1;
+            const int dyp = (j + 1 < ny) ? nx : nx - nxy;
+            const int dym = (j > 0) ? -nx : nxy - nx;
+            const int dzp = (k + 1 < nz) ? nxy : nxy - N;
+            const int dzm = (k > 0) ? -nxy : N - nxy;
+            const float xc = x[idx];
+            const float gx = (x[idx + dxp] - xc) * hx, gy = (x[idx + dyp] - xc) * hy, gz = (x[idx + dzp] - xc) * hz;
+            const float gxm = (xc - x[idx + dxm]) * hx, gym = (xc - x[idx + dym]) * hy, gzm = (xc - x[idx + dzm]) * hz;
+            const float ux = u[idx], uy = u[N + idx], uz = u[2 * N + idx];
+            if (ok) {
+                if (converged) {
+                    uo[idx] = ux;
+                    uo[N + idx] = uy;
+                    uo[2 * N + idx] = uz;
+                } else {
+                    const float ti = t * (w[idx] / wnorm);
+                    const float vx = gx + ux, vy = gy + uy, vz = gz + uz;
+                    uo[idx] = vx - shrinkf(vx, ti);
+                    uo[N + idx] = vy - shrinkf(vy, ti);
+                    uo[2 * N + idx] = vz - shrinkf(vz, ti);
+                }
+            }
+            d[h] = ok ? (gx - gxm) * hx + (gy - gym) * hy + (gz - gzm) * hz : 0.0f;
+        }
+        if (e < total) lds[e] = make_float2(d[0], d[1]);
     }
 }
 
@@ -546,6 +630,14 @@ __global__ __launch_bounds__(kThreads) void k_r2c_x(const VolDesc* __restrict__ 
         const size_t N3 = (size_t)3 * V.nx * V.ny * V.nz;
         fill_tvdiv<EMAX, true>(lds, V, nx, RB, glb(V.x[aux]), glb(V.u[aux ^ 1]), glb(V.u[aux]), row0,
                                nrows, glb(V.ckpt) + (size_t)ck * N3);
+    } else if constexpr (SRC == SRC_TVDIV_W) {
+        fill_tvdiv<EMAX, false, true>(lds, V, nx, RB, glb(V.x[aux]), glb(V.u[aux ^ 1]), glb(V.u[aux]),
+                                      row0, nrows, nullptr, glb(V.nltv_w[ck]), (float)V.nltv_wnorm[ck]);
+    } else if constexpr (SRC == SRC_NLTV_BND) {
+        // status_rw carries the convergence state of the round just ended
+        fill_nltv_boundary<EMAX>(lds, V, nx, RB, glb(V.x[aux]), glb(V.u[aux ^ 1]), glb(V.u[aux]), row0,
+                                 nrows, status_rw[gv].done != 0, glb(V.nltv_w[ck]),
+                                 (float)V.nltv_wnorm[ck]);
     } else if constexpr (SRC == SRC_TVADJ_FIRST || SRC == SRC_TVADJ) {
         __shared__ double sh[kThreads / 64];
         double ts;

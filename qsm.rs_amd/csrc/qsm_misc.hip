@@ -29,6 +29,89 @@ __global__ __launch_bounds__(kThreads) void k_tv_lbar(const VolDesc* __restrict_
     if (threadIdx.x == 0) V.adj_lbar[0] = t / V.rho;
 }
 
+// NLTV weights after a Newton round (src/inversion/nltv.rs:112-125):
+// w = 1/(|grad x| + mu 1e-6) from x[parity] (periodic forward differences,
+// src/utils/gradient.rs:21-58), in FP64, stored FP32 in nltv_w[sel]; per-block
+// maxima into nltv_wpart.  Blocks of kThreads*4 voxels.
+__global__ __launch_bounds__(kThreads) void k_nltv_weights(const VolDesc* __restrict__ vols, LaunchTable lt,
+                                                           int parity, int sel) {
+    __shared__ double sh[kThreads / 64];
+    int lb;
+    const int gv = locate(lt, blockIdx.x, lb);
+    const VolDesc& V = vols[gv];
+    const int nx = V.nx, ny = V.ny, nz = V.nz;
+    const size_t nxy = (size_t)nx * ny, N = nxy * nz;
+    const float* x = V.x[parity];
+    float* w = V.nltv_w[sel];
+    const size_t end = (size_t)(lb + 1) * kThreads * 4;
+    double mx = 0.0;
+    for (size_t q = (size_t)lb * kThreads * 4 + threadIdx.x; q < N && q < end; q += kThreads) {
+        const int i = (int)(q % nx);
+        const int j = (int)((q / nx) % ny);
+        const int k = (int)(q / nxy);
+        const long ox = (i + 1 < nx) ? 1L : -(long)(nx - 1);
+        const long oy = (j + 1 < ny) ? (long)nx : -(long)(ny - 1) * nx;
+        const long oz = (k + 1 < nz) ? (long)nxy : -(long)(nz - 1) * (long)nxy;
+        const long lq = (long)q;
+        const double xc = x[q];
+        const double gx = ((double)x[lq + ox] - xc) / V.vsx;
+        const double gy = ((double)x[lq + oy] - xc) / V.vsy;
+        const double gz = ((double)x[lq + oz] - xc) / V.vsz;
+        const double wv = 1.0 / (sqrt(gx * gx + gy * gy + gz * gz) + V.filt_b * 1e-6);
+        w[q] = (float)wv;
+        mx = fmax(mx, wv);
+    }
+    // block maximum
+    for (int o = 32; o > 0; o >>= 1) mx = fmax(mx, __shfl_xor(mx, o, 64));
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    if (lane == 0) sh[wave] = mx;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double m = 0.0;
+        for (int q = 0; q < kThreads / 64; ++q) m = fmax(m, sh[q]);
+        V.nltv_wpart[lb] = m;
+    }
+}
+
+// nltv_wnorm[sel] = max(w) if it exceeds 1, else 1 (src/inversion/nltv.rs:
+// 120-125; the weights are used as w / nltv_wnorm).  One block per volume.
+__global__ __launch_bounds__(kThreads) void k_nltv_wmax(const VolDesc* __restrict__ vols, int nvol, int nblk,
+                                                        int sel) {
+    __shared__ double sh[kThreads / 64];
+    const int v = blockIdx.x;
+    if (v >= nvol) return;
+    const VolDesc& V = vols[v];
+    const size_t N = (size_t)V.nx * V.ny * V.nz;
+    const int nb = (int)((N + kThreads * 4 - 1) / (kThreads * 4));
+    double mx = 0.0;
+    for (int q = threadIdx.x; q < nb && q < nblk; q += kThreads) mx = fmax(mx, V.nltv_wpart[q]);
+    for (int o = 32; o > 0; o >>= 1) mx = fmax(mx, __shfl_xor(mx, o, 64));
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    if (lane == 0) sh[wave] = mx;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double m = 0.0;
+        for (int q = 0; q < kThreads / 64; ++q) m = fmax(m, sh[q]);
+        V.nltv_wnorm[sel] = m > 1.0 ? m : 1.0;
+    }
+}
+
+// Weights of the first round: w = 1, normalisation 1.
+__global__ __launch_bounds__(kThreads) void k_nltv_init(const VolDesc* __restrict__ vols, LaunchTable lt) {
+    int lb;
+    const int gv = locate(lt, blockIdx.x, lb);
+    const VolDesc& V = vols[gv];
+    const size_t N = (size_t)V.nx * V.ny * V.nz;
+    const size_t end = (size_t)(lb + 1) * kThreads * 4;
+    for (size_t q = (size_t)lb * kThreads * 4 + threadIdx.x; q < N && q < end; q += kThreads) V.nltv_w[0][q] = 1.0f;
+    if (lb == 0 && threadIdx.x < 2) V.nltv_wnorm[threadIdx.x] = 1.0;
+}
+
+__global__ void k_status_clear_done(VolStatus* status, int nvol) {
+    const int v = blockIdx.x * blockDim.x + threadIdx.x;
+    if (v < nvol) status[v].done = 0;
+}
+
 // chi = x where mask != 0, else 0 (src/utils/mask.rs:47-53), widened to f64.
 __global__ __launch_bounds__(kThreads) void k_tv_output(const VolDesc* __restrict__ vols, LaunchTable lt,
                                                         const VolStatus* __restrict__ status) {
@@ -77,6 +160,18 @@ void launch_tv_reduce(hipStream_t st, const VolDesc* d, int nvol, VolStatus* sta
 }
 void launch_tv_lbar(hipStream_t st, const VolDesc* d, int nvol, int nrows) {
     k_tv_lbar<<<dim3(nvol), dim3(kThreads), 0, st>>>(d, nvol, nrows);
+}
+void launch_nltv_weights(hipStream_t st, const VolDesc* d, const LaunchTable& lt, int parity, int sel) {
+    k_nltv_weights<<<dim3(lt.start[lt.nvol]), dim3(kThreads), 0, st>>>(d, lt, parity, sel);
+}
+void launch_nltv_wmax(hipStream_t st, const VolDesc* d, int nvol, int nblk, int sel) {
+    k_nltv_wmax<<<dim3(nvol), dim3(kThreads), 0, st>>>(d, nvol, nblk, sel);
+}
+void launch_nltv_init(hipStream_t st, const VolDesc* d, const LaunchTable& lt) {
+    k_nltv_init<<<dim3(lt.start[lt.nvol]), dim3(kThreads), 0, st>>>(d, lt);
+}
+void launch_status_clear_done(hipStream_t st, VolStatus* s, int nvol) {
+    k_status_clear_done<<<dim3((nvol + 63) / 64), dim3(64), 0, st>>>(s, nvol);
 }
 void launch_tv_output(hipStream_t st, const VolDesc* d, const LaunchTable& lt, const VolStatus* status) {
     k_tv_output<<<dim3(lt.start[lt.nvol]), dim3(kThreads), 0, st>>>(d, lt, status);

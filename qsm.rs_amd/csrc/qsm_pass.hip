@@ -14,7 +14,8 @@ static bool is_row_kind(int kind) {
            kind == P_R2C_SPHERE || kind == P_C2R_TV || kind == P_C2R_VS_SELECT || kind == P_C2R_VS_OUT ||
            kind == P_R2C_TVDIV_REC || kind == P_R2C_TVADJ_FIRST || kind == P_R2C_TVADJ ||
            kind == P_R2C_ACC || kind == P_C2R_ADJ || kind == P_C2R_F64OUT || kind == P_C2R_F64_MASKED ||
-           kind == P_C2R_X0 || kind == P_R2C_X0 || kind == P_R2C_WRAPLAP;
+           kind == P_C2R_X0 || kind == P_R2C_X0 || kind == P_R2C_WRAPLAP || kind == P_R2C_TVDIV_W ||
+           kind == P_R2C_NLTV_BND;
 }
 
 template <int N, int EG>
@@ -49,6 +50,14 @@ static void launch_rows(int kind, const LaunchTable& lt, const PassArgs& a) {
         case P_C2R_F64_MASKED: k_c2r_tv<N, EG, C2R_F64_MASKED><<<g, b, 0, st>>>(a.vols, lt, a.tw, 0, nullptr); break;
         case P_C2R_X0: k_c2r_tv<N, EG, C2R_X0><<<g, b, 0, st>>>(a.vols, lt, a.tw, 0, nullptr); break;
         case P_R2C_X0: k_r2c_x<N, EG, SRC_X0><<<g, b, 0, st>>>(a.vols, lt, a.tw, a.sel, 0, nullptr, nullptr, nullptr, 0); break;
+        case P_R2C_TVDIV_W:
+            k_r2c_x<N, EG, SRC_TVDIV_W><<<g, b, 0, st>>>(a.vols, lt, a.tw, a.sel, a.parity, a.status,
+                                                         nullptr, nullptr, a.ridx);
+            break;
+        case P_R2C_NLTV_BND:
+            k_r2c_x<N, EG, SRC_NLTV_BND><<<g, b, 0, st>>>(a.vols, lt, a.tw, a.sel, a.parity, nullptr,
+                                                          a.status_rw, nullptr, a.ridx);
+            break;
         case P_R2C_WRAPLAP: k_r2c_x<N, EG, SRC_WRAPLAP><<<g, b, 0, st>>>(a.vols, lt, a.tw, a.sel, 0, nullptr, nullptr, nullptr, 0); break;
         case P_C2R_VS_SELECT: k_c2r_vs_select<N, EG><<<g, b, 0, st>>>(a.vols, lt, a.tw, a.ridx, a.delta); break;
         case P_C2R_VS_OUT: k_c2r_vs_out<N, EG><<<g, b, 0, st>>>(a.vols, lt, a.tw); break;

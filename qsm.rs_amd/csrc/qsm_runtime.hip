@@ -20,6 +20,10 @@ void launch_fill_zero(hipStream_t st, float* p, size_t n);
 void launch_status_reset(hipStream_t st, VolStatus* s, int nvol);
 void launch_vs_init(hipStream_t st, const VolDesc* d, const LaunchTable& lt);
 void launch_tv_lbar(hipStream_t st, const VolDesc* d, int nvol, int nrows);
+void launch_nltv_weights(hipStream_t st, const VolDesc* d, const LaunchTable& lt, int parity, int sel);
+void launch_nltv_wmax(hipStream_t st, const VolDesc* d, int nvol, int nblk, int sel);
+void launch_nltv_init(hipStream_t st, const VolDesc* d, const LaunchTable& lt);
+void launch_status_clear_done(hipStream_t st, VolStatus* s, int nvol);
 bool fixed_length(int n);
 
 // ------------------------------------------------------------ error state --
@@ -1212,6 +1216,118 @@ int qsm_laplacian_unwrap(const double* phase, const uint8_t* mask, const qsm_gri
     return filter_host(6, 0.0, phase, mask, grid, nullptr, out, P_R2C_WRAPLAP);
 }
 
+// nltv, src/inversion/nltv.rs:63-133, on one volume: TV's kernels with
+// weighted thresholds, a weight pass and a boundary step between Newton
+// rounds.  The host synchronises after every step when it needs to know
+// about convergence (tol > 0) or report progress.
+void qsm_nltv_params_default(qsm_nltv_params* p) {
+    if (!p) return;  // src/inversion/nltv.rs:38-49
+    p->lambda = 1e-3;
+    p->mu = 1.0;
+    p->tol = 1e-3;
+    p->max_iter = 250;
+    p->newton_iter = 10;
+}
+
+int qsm_nltv(const double* local_field, const uint8_t* mask, const qsm_grid* grid, const double bdir[3],
+             const qsm_nltv_params* params, qsm_progress_fn progress, void* user, double* chi_out) {
+    g_err.clear();
+    if (!local_field || !mask || !grid || !bdir || !params || !chi_out)
+        return fail(QSM_EPARAM, "null argument");
+    if (!(params->lambda > 0)) return fail(QSM_EPARAM, "NLTV needs lambda > 0 (rho = 100 lambda)");
+    int rc = validate_grid(grid);
+    if (rc) return rc;
+    const size_t N = grid->nx * grid->ny * grid->nz;
+    double *d_f = nullptr, *d_chi = nullptr;
+    uint8_t* d_m = nullptr;
+    qsm_tv_batch* B = nullptr;
+    std::vector<void*> extra;
+    auto done = [&](int code) {
+        if (B) batch_free(B);
+        for (void* p : extra) (void)hipFree(p);
+        if (d_f) (void)hipFree(d_f);
+        if (d_chi) (void)hipFree(d_chi);
+        if (d_m) (void)hipFree(d_m);
+        return code;
+    };
+    if ((rc = dev_alloc(&d_f, N)) || (rc = dev_alloc(&d_chi, N)) || (rc = dev_alloc(&d_m, N)))
+        return done(rc);
+    hipError_t e = hipMemcpy(d_f, local_field, N * sizeof(double), hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(d_m, mask, N, hipMemcpyHostToDevice);
+    if (e != hipSuccess) return done(fail(QSM_EHIP, "hipMemcpy H2D: %s", hipGetErrorString(e)));
+    qsm_tv_volume vol;
+    vol.grid = *grid;
+    vol.bdir[0] = bdir[0]; vol.bdir[1] = bdir[1]; vol.bdir[2] = bdir[2];
+    vol.d_field = d_f; vol.d_mask = d_m; vol.d_chi = d_chi;
+    qsm_tv_params tp;
+    tp.lambda = params->lambda;
+    tp.rho = 100.0 * params->lambda;  // src/inversion/nltv.rs:75
+    tp.tol = params->tol;
+    tp.max_iter = params->max_iter;
+    if ((rc = tv_batch_create_impl(&vol, 1, &tp, nullptr, &B))) return done(rc);
+    VolDesc& V = B->h[0];
+    V.filt_b = params->mu;
+    const int nb = (int)((N + qsmfft::kThreads * 4 - 1) / (qsmfft::kThreads * 4));
+    for (int q = 0; q < 2; ++q) {
+        if ((rc = dev_alloc(&V.nltv_w[q], N))) return done(rc);
+        extra.push_back(V.nltv_w[q]);
+    }
+    if ((rc = dev_alloc(&V.nltv_wnorm, 2))) return done(rc);
+    extra.push_back(V.nltv_wnorm);
+    if ((rc = dev_alloc(&V.nltv_wpart, (size_t)nb))) return done(rc);
+    extra.push_back(V.nltv_wpart);
+    hipStream_t st = B->stream;
+    QSM_HIP(hipMemcpy(B->d_vols, B->h.data(), sizeof(VolDesc), hipMemcpyHostToDevice));
+    std::memset(B->h_status, 0, sizeof(VolStatus));
+    if ((rc = enqueue_setup(B))) return done(rc);
+    launch_nltv_init(st, B->d_vols, B->lt_out);
+    const bool sync_each = progress != nullptr || params->tol > 0.0;
+    const size_t total = params->max_iter * params->newton_iter;
+    size_t S = 0, current = 0;
+    int cw = 0;
+    hipEvent_t ev = B->ev[0];
+    for (size_t newton = 0; newton < params->newton_iter; ++newton) {
+        for (size_t it = 1; it <= params->max_iter; ++it) {
+            ++current;
+            if (progress) progress(current, total, user);
+            ++S;
+            PassArgs a = B->args(B->d_status);
+            a.parity = (int)((S - 1) & 1);
+            if (newton > 0 && it == 1) {
+                launch_nltv_weights(st, B->d_vols, B->lt_out, a.parity, cw ^ 1);
+                launch_nltv_wmax(st, B->d_vols, 1, nb, cw ^ 1);
+                PassArgs b = a;
+                b.status_rw = B->d_status;
+                b.ridx = cw;
+                run_groups(P_R2C_NLTV_BND, B->gx, b);
+                launch_status_clear_done(st, B->d_status, 1);
+                cw ^= 1;
+            } else {
+                a.ridx = cw;
+                run_groups(P_R2C_TVDIV_W, B->gx, a);
+            }
+            run_groups(P_Y_FWD, B->gy, a);
+            run_groups(P_Z_TV_SOLVE, B->gz, a);
+            run_groups(P_Y_INV, B->gy, a);
+            run_groups(P_C2R_TV, B->gxc, a);
+            launch_tv_reduce(st, B->d_vols, 1, B->d_status, B->h_status_dev);
+            QSM_CHECK_LAUNCH();
+            if (sync_each) {
+                QSM_HIP(hipEventRecord(ev, st));
+                QSM_HIP(hipEventSynchronize(ev));
+                const volatile VolStatus* hs = B->h_status;
+                if (hs->done) break;  // src/inversion/nltv.rs:105-107
+            }
+        }
+    }
+    launch_tv_output(st, B->d_vols, B->lt_out, B->d_status);
+    QSM_CHECK_LAUNCH();
+    QSM_HIP(hipStreamSynchronize(st));
+    e = hipMemcpy(chi_out, d_chi, N * sizeof(double), hipMemcpyDeviceToHost);
+    if (e != hipSuccess) return done(fail(QSM_EHIP, "hipMemcpy D2H: %s", hipGetErrorString(e)));
+    return done(QSM_OK);
+}
+
 void qsm_tkd_params_default(qsm_tkd_params* p) {
     if (p) p->threshold = 0.15;  // src/inversion/tkd.rs:29-33
 }
@@ -1318,10 +1434,11 @@ void qsm_inversion_config_default(qsm_inversion_config* c) {
     qsm_tikhonov_params_default(&c->tikhonov);
     qsm_tv_params_default(&c->tv);
     qsm_rts_params_default(&c->rts);
+    qsm_nltv_params_default(&c->nltv);
 }
 
 // run_dipole_inversion, src/pipeline/inversion.rs:28-114, for the arms with
-// a GPU backend here (Tkd, Tsvd, Tikhonov, Tv, Rts).
+// a GPU backend here (Tkd, Tsvd, Tikhonov, Tv, Rts, Nltv).
 int qsm_run_dipole_inversion_cfg(const double* local_field_ppm, size_t field_len, const uint8_t* mask,
                                  size_t mask_len, const qsm_scan_metadata* md,
                                  const qsm_inversion_config* cfg, qsm_progress_fn progress, void* user,
@@ -1339,6 +1456,7 @@ int qsm_run_dipole_inversion_cfg(const double* local_field_ppm, size_t field_len
         case QSM_INV_TIKHONOV: return qsm_tikhonov(local_field_ppm, mask, &g, b, &cfg->tikhonov, chi_out);
         case QSM_INV_TV: return qsm_tv_admm(local_field_ppm, mask, &g, b, &cfg->tv, progress, user, chi_out);
         case QSM_INV_RTS: return qsm_rts(local_field_ppm, mask, &g, b, &cfg->rts, progress, user, chi_out);
+        case QSM_INV_NLTV: return qsm_nltv(local_field_ppm, mask, &g, b, &cfg->nltv, progress, user, chi_out);
         case QSM_INV_TGV:
         case QSM_INV_QSMART:
             return fail(QSM_ECONFIG, "invalid config: %s should use run_tgv or run_qsmart",

@@ -10,6 +10,7 @@
 //   qsm::inversion::tkd / tsvd      src/inversion/tkd.rs:48-99
 //   qsm::inversion::tikhonov        src/inversion/tikhonov.rs:62-131
 //   qsm::inversion::rts             src/inversion/rts.rs:71-212
+//   qsm::inversion::nltv            src/inversion/nltv.rs:63-133
 //   qsm::bgremove::sharp            src/bgremove/sharp.rs:48-56
 //   qsm::unwrap::laplacian_unwrap   src/unwrap/laplacian.rs:135-161
 //   qsm::pipeline::run_dipole_inversion / run_bg_removal  src/pipeline/*.rs
@@ -159,6 +160,27 @@ inline std::vector<double> rts(const std::vector<double>& f, const std::vector<u
     return chi;
 }
 
+struct NltvParams {
+    double lambda = 1e-3, mu = 1.0, tol = 1e-3;
+    size_t max_iter = 250, newton_iter = 10;
+    qsm_nltv_params c() const { return qsm_nltv_params{lambda, mu, tol, max_iter, newton_iter}; }
+};
+
+// src/inversion/nltv.rs:63-133
+inline std::vector<double> nltv(const std::vector<double>& f, const std::vector<uint8_t>& m, const Grid& grid,
+                                std::tuple<double, double, double> bdir, const NltvParams& params,
+                                Progress progress = nullptr) {
+    if (f.size() != grid.n_total() || m.size() != grid.n_total())
+        throw Error(QSM_EDIM, "array length does not match the grid");
+    std::vector<double> chi(grid.n_total());
+    const qsm_grid g = grid.c();
+    const double b[3] = {std::get<0>(bdir), std::get<1>(bdir), std::get<2>(bdir)};
+    const qsm_nltv_params p = params.c();
+    check(qsm_nltv(f.data(), m.data(), &g, b, &p, progress ? qsm::detail::trampoline : nullptr,
+                   progress ? &progress : nullptr, chi.data()));
+    return chi;
+}
+
 }  // namespace inversion
 
 namespace bgremove {
@@ -278,8 +300,9 @@ struct InversionConfig {
     inversion::TikhonovParams tikhonov;
     inversion::TvParams tv = inversion::TvParams::default_params();
     inversion::RtsParams rts;
+    inversion::NltvParams nltv;
     qsm_inversion_config c() const {
-        return qsm_inversion_config{algorithm, tkd.c(), tsvd.c(), tikhonov.c(), tv.c(), rts.c()};
+        return qsm_inversion_config{algorithm, tkd.c(), tsvd.c(), tikhonov.c(), tv.c(), rts.c(), nltv.c()};
     }
 };
 

@@ -277,6 +277,21 @@ static void test_laplacian_unwrap_reference_tests() {
     for (size_t i = 0; i < ph.size(); ++i) EXPECT(std::isfinite(s[i]) && std::fabs(ph[i] - s[i]) < 1.0, "smooth");
 }
 
+// src/inversion/nltv.rs:138-185 (zero field, finite; weighted_shrink is
+// shrink(x, t w), pinned in the oracle tests)
+static void test_nltv_reference_tests() {
+    const size_t n = 8;
+    Grid g = Grid::make(n, n, n, 1, 1, 1);
+    inversion::NltvParams p;
+    p.max_iter = 5;
+    p.newton_iter = 2;
+    std::vector<uint8_t> m(n * n * n, 1);
+    auto z = inversion::nltv(std::vector<double>(n * n * n, 0.0), m, g, {0, 0, 1}, p);
+    for (double v : z) EXPECT(std::fabs(v) < 1e-10, "zero field -> zero chi");
+    auto r = inversion::nltv(ramp(n * n * n, 0.001), m, g, {0, 0, 1}, p);
+    for (double v : r) EXPECT(std::isfinite(v), "finite");
+}
+
 int main() {
     void (*tests[])() = {test_tv_admm_zero_field, test_tv_admm_finite, test_tv_admm_smoother_than_tkd,
                          test_tv_params_default, test_vsharp_zero_field, test_vsharp_nonuniform_voxels,
@@ -285,7 +300,7 @@ int main() {
                          test_bg_removal_validates_dims, test_inversion_tv_and_rejects_tgv,
                          test_tkd_reference_tests, test_tikhonov_regularization_types,
                          test_rts_reference_tests, test_pipeline_default_rts_and_sharp_arm,
-                         test_laplacian_unwrap_reference_tests};
+                         test_laplacian_unwrap_reference_tests, test_nltv_reference_tests};
     int i = 0;
     for (auto t : tests) {
         const int before = g_fail;

@@ -56,7 +56,8 @@ def test_param_defaults():
     for cls, fn, py in [(qsm._CTkdParams, "qsm_tkd_params_default", qsm.TkdParams()),
                         (qsm._CTikhonovParams, "qsm_tikhonov_params_default", qsm.TikhonovParams()),
                         (qsm._CRtsParams, "qsm_rts_params_default", qsm.RtsParams()),
-                        (qsm._CSharpParams, "qsm_sharp_params_default", qsm.SharpParams())]:
+                        (qsm._CSharpParams, "qsm_sharp_params_default", qsm.SharpParams()),
+                        (qsm._CNltvParams, "qsm_nltv_params_default", qsm.NltvParams())]:
         c = cls()
         getattr(lib, fn)(C.byref(c))
         pc = py._c()
@@ -66,6 +67,7 @@ def test_param_defaults():
     lib.qsm_inversion_config_default(C.byref(cfg))
     assert cfg.algorithm == int(qsm.InversionAlgorithm.Rts) and cfg.tkd.threshold == 0.15
     assert cfg.tv.max_iter == 250 and cfg.rts.max_iter == 20 and cfg.tikhonov.reg == 0
+    assert cfg.nltv.newton_iter == 10 and cfg.nltv.max_iter == 250
 
 
 def test_pipeline_dimension_mismatch_and_config_errors():

@@ -147,3 +147,37 @@ def test_laplacian_unwrap_matches_oracle(qsm, oracle, dims, vs):
     assert e <= 1e-5
     assert ef <= 1e-4 * np.max(np.abs(true))
     assert np.all(got[m == 0] == 0.0)
+
+
+@pytest.mark.parametrize("dims,vs,b", [((16, 16, 16), (1.0, 1.0, 1.0), (0.0, 0.0, 1.0)),
+                                       ((20, 18, 22), (0.9, 1.1, 1.3), (0.2, -0.1, 0.97)),
+                                       ((64, 64, 64), (1.0, 1.0, 1.0), (0.1, 0.0, 1.0))])
+def test_nltv_matches_oracle(qsm, oracle, dims, vs, b):
+    f, m = _case(5, dims)
+    g = qsm.Grid(dims, vs)
+    p = qsm.NltvParams(lambda_=2e-3, mu=1.0, tol=0.0, max_iter=5, newton_iter=3)
+    calls, ocalls = [], []
+    x = qsm.nltv(f, m, g, b, p, progress=lambda a, c: calls.append((a, c)))
+    xo = oracle.nltv(f, m, dims, vs, b, 2e-3, 1.0, 0.0, 5, 3,
+                     progress=lambda a, c: ocalls.append((a, c)))
+    e = nrmse_dc(x, xo, m)
+    print(dims, "nltv nrmse(dc)", e, "raw", oracle.nrmse(x, xo, m))
+    assert calls == ocalls == [(i, 15) for i in range(1, 16)]
+    assert e <= 2e-5
+    # with a tolerance the rounds stop early; the result stays within the bar
+    p2 = qsm.NltvParams(lambda_=2e-3, mu=1.0, tol=2e-2, max_iter=30, newton_iter=3)
+    x2 = qsm.nltv(f, m, g, b, p2)
+    xo2 = oracle.nltv(f, m, dims, vs, b, 2e-3, 1.0, 2e-2, 30, 3)
+    e2 = nrmse_dc(x2, xo2, m)
+    print(dims, "nltv tol 2e-2 nrmse(dc)", e2)
+    assert e2 <= 1e-3
+
+
+def test_nltv_pipeline_arm(qsm):
+    dims = (16, 16, 16)
+    f, m = _case(6, dims)
+    meta = qsm.ScanMetadata(dims=dims, voxel_size=(1.0, 1.0, 1.0))
+    p = qsm.NltvParams(lambda_=2e-3, tol=0.0, max_iter=4, newton_iter=2)
+    cfg = qsm.InversionConfig(algorithm=qsm.InversionAlgorithm.Nltv, nltv=p)
+    x = qsm.run_dipole_inversion(f, m, meta, cfg)
+    assert np.array_equal(x, qsm.nltv(f, m, qsm.Grid(dims), (0, 0, 1), p))

@@ -153,3 +153,72 @@ def test_laplacian_unwrap_reference_tests_and_closed_form(oracle):
     ones = np.ones(true.size, np.uint8)
     got = oracle.laplacian_unwrap(wrapped, ones, dims, vs)
     assert np.max(np.abs(got - (true - true.mean()))) < 1e-10
+
+
+def _nltv_numpy(f, dims, vs, bdir, lam, mu, max_iter, newton_iter):
+    """Independent numpy restatement of src/inversion/nltv.rs:63-133 (tol = 0,
+    all-ones mask) for tiny volumes: full-spectrum FFTs, f64 throughout."""
+    rho = 100 * lam
+    F = lambda a: np.fft.fftn(a)
+    Fi = lambda a: np.fft.ifftn(a)
+    kx, ky, kz = [np.fft.fftfreq(n, v) for n, v in zip(dims, vs)]
+    KX, KY, KZ = np.meshgrid(kx, ky, kz, indexing="ij")
+    bb = np.asarray(bdir, float) / np.linalg.norm(bdir)
+    k2 = KX ** 2 + KY ** 2 + KZ ** 2
+    with np.errstate(divide="ignore", invalid="ignore"):
+        D = np.where(k2 > 1e-20, 1 / 3 - (KX * bb[0] + KY * bb[1] + KZ * bb[2]) ** 2 / k2, 0.0)
+    h = [1 / v ** 2 for v in vs]
+    stencil = np.zeros(dims)
+    stencil[0, 0, 0] = 2 * sum(h)
+    for a, ha in enumerate(h):
+        for s in (1, -1):
+            idx = [0, 0, 0]
+            idx[a] = s % dims[a]
+            stencil[tuple(idx)] -= ha
+    L = F(stencil).real
+    A = D * D + rho * L
+    inv = np.where(np.abs(A) > 1e-20, 1 / np.where(A == 0, 1, A), 0.0)
+    fh = F(f.reshape(dims, order="F")) * D * inv
+    grad = lambda x: [(np.roll(x, -1, axis=a) - x) / vs[a] for a in range(3)]
+    bdiv = lambda g: sum((g[a] - np.roll(g[a], 1, axis=a)) / vs[a] for a in range(3))
+    shrink = lambda v, t: np.sign(v) * np.maximum(np.abs(v) - t, 0.0)
+    x = np.zeros(dims)
+    u = [np.zeros(dims) for _ in range(3)]
+    g = [np.zeros(dims) for _ in range(3)]
+    w = np.ones(dims)
+    t = lam / rho
+    for _ in range(newton_iter):
+        for _ in range(max_iter):
+            x = Fi(fh - rho * F(bdiv(g)) * inv).real
+            gr = grad(x)
+            for a in range(3):
+                v = gr[a] + u[a]
+                z = shrink(v, t * w)
+                u[a] = v - z
+                g[a] = 2 * z - v
+        g = grad(x)
+        w = 1 / (np.sqrt(sum(q * q for q in g)) + mu * 1e-6)
+        if w.max() > 1:
+            w = w / w.max()
+    return x.ravel(order="F")
+
+
+def test_nltv_oracle_against_numpy_and_tv(oracle):
+    dims, vs, b = (8, 6, 10), (1.0, 1.0, 1.0), (0.1, 0.0, 1.0)
+    rng = np.random.default_rng(6)
+    n = int(np.prod(dims))
+    f = rng.standard_normal(n) * 0.02
+    ones = np.ones(n, np.uint8)
+    got = oracle.nltv(f, ones, dims, vs, b, lam=2e-3, mu=1.0, tol=0.0, max_iter=3, newton_iter=3)
+    ref = _nltv_numpy(f, dims, vs, b, 2e-3, 1.0, 3, 3)
+    assert np.max(np.abs(got - ref)) <= 1e-12 * np.max(np.abs(ref))
+    # one Newton round with unit weights is tv_admm at rho = 100 lambda
+    one = oracle.nltv(f, ones, dims, vs, b, lam=2e-3, mu=1.0, tol=0.0, max_iter=7, newton_iter=1)
+    tv = oracle.tv_admm(f, ones, dims, vs, b, 2e-3, 0.2, 0.0, 7)
+    assert np.array_equal(one, tv)
+    # src/inversion/nltv.rs:138-185: zero field -> zero, finite
+    assert np.max(np.abs(oracle.nltv(np.zeros(512), np.ones(512, np.uint8), (8, 8, 8), vs, (0, 0, 1),
+                                     max_iter=5, newton_iter=2))) < 1e-10
+    ramp = np.arange(512) * 0.001
+    assert np.all(np.isfinite(oracle.nltv(ramp, np.ones(512, np.uint8), (8, 8, 8), vs, (0, 0, 1),
+                                          max_iter=5, newton_iter=2)))
