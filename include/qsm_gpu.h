@@ -322,6 +322,32 @@ int qsm_dipole_forward(const double* d_chi, const qsm_grid* grid, const double b
 int qsm_dipole_adjoint(const double* d_field, const qsm_grid* grid, const double bdir[3], double* d_chi,
                        void* stream);
 
+/* ---- NIfTI-1 input/output, src/io.rs ---------------------------------- */
+/* NiftiData, src/io.rs:13-26 (dims, voxel size, affine, scaling) plus the
+ * number of volumes and the on-disk datatype. */
+typedef struct qsm_nifti_info {
+    size_t nx, ny, nz, nt;       /* nt = 1 for a 3-D file */
+    double vsx, vsy, vsz;        /* pixdim[1..3] */
+    double affine[16];           /* sform rows if sform_code > 0, else diag(voxel size) */
+    double scl_slope, scl_inter; /* slope 0 on disk is reported as 1 */
+    int datatype;
+} qsm_nifti_info;
+/* is_gzip, src/io.rs:28-30 */
+int qsm_nifti_is_gzip(const uint8_t* bytes, size_t len);
+/* load_nifti / load_nifti_4d, src/io.rs:62-229: plain or gzip bytes; data may
+ * be null (header only), else receives nx*ny*nz*nt f64 values scaled by
+ * slope/inter, Fortran order, volume after volume. */
+int qsm_nifti_load(const uint8_t* bytes, size_t len, qsm_nifti_info* info, double* data, size_t data_len);
+/* save_nifti / save_nifti_gz, src/io.rs:262-357: FLOAT32, sform from the
+ * affine; *bytes_out is released with qsm_nifti_free. */
+int qsm_nifti_save(const double* data, size_t nx, size_t ny, size_t nz, const double voxel_size[3],
+                   const double affine[16], int gzip, uint8_t** bytes_out, size_t* len_out);
+void qsm_nifti_free(uint8_t* bytes);
+/* read_nifti_file / save_nifti_to_file, src/io.rs:423-452 (.nii.gz by suffix). */
+int qsm_nifti_read_file(const char* path, qsm_nifti_info* info, double* data, size_t data_len);
+int qsm_nifti_write_file(const char* path, const double* data, size_t nx, size_t ny, size_t nz,
+                         const double voxel_size[3], const double affine[16]);
+
 /* ---- device-resident batched V-SHARP (no reference counterpart) ------- */
 
 typedef struct qsm_vsharp_volume {

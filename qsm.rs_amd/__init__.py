@@ -96,6 +96,13 @@ class _CBgRemovalConfig(C.Structure):
     _fields_ = [("algorithm", C.c_int), ("vsharp", _CVsharpParams), ("sharp", _CSharpParams)]
 
 
+class _CNiftiInfo(C.Structure):
+    _fields_ = [("nx", C.c_size_t), ("ny", C.c_size_t), ("nz", C.c_size_t), ("nt", C.c_size_t),
+                ("vsx", C.c_double), ("vsy", C.c_double), ("vsz", C.c_double),
+                ("affine", C.c_double * 16), ("scl_slope", C.c_double), ("scl_inter", C.c_double),
+                ("datatype", C.c_int)]
+
+
 class _CScanMetadata(C.Structure):
     _fields_ = [("nx", C.c_size_t), ("ny", C.c_size_t), ("nz", C.c_size_t),
                 ("vsx", C.c_double), ("vsy", C.c_double), ("vsz", C.c_double),
@@ -165,6 +172,13 @@ _SIGS = {
     "qsm_run_bg_removal_cfg": (C.c_int, [_D, _SZ, _U8, _SZ, C.POINTER(_CScanMetadata),
                                          C.POINTER(_CBgRemovalConfig), PROGRESS_FN, C.c_void_p,
                                          _D, _U8]),
+    "qsm_nifti_is_gzip": (C.c_int, [C.c_void_p, _SZ]),
+    "qsm_nifti_load": (C.c_int, [C.c_void_p, _SZ, C.POINTER(_CNiftiInfo), _D, _SZ]),
+    "qsm_nifti_save": (C.c_int, [_D, _SZ, _SZ, _SZ, _D, _D, C.c_int, C.POINTER(C.c_void_p),
+                                 C.POINTER(_SZ)]),
+    "qsm_nifti_free": (None, [C.c_void_p]),
+    "qsm_nifti_read_file": (C.c_int, [C.c_char_p, C.POINTER(_CNiftiInfo), _D, _SZ]),
+    "qsm_nifti_write_file": (C.c_int, [C.c_char_p, _D, _SZ, _SZ, _SZ, _D, _D]),
     "qsm_vsharp_batch_create": (C.c_int, [C.POINTER(_CVsharpVolume), _SZ, _D, _SZ, C.c_double,
                                           C.c_void_p, C.POINTER(C.c_void_p)]),
     "qsm_vsharp_batch_run": (C.c_int, [C.c_void_p, PROGRESS_FN, C.c_void_p]),
@@ -653,6 +667,75 @@ def run_bg_removal(field_ppm, mask, metadata: ScanMetadata, config: BgRemovalCon
     return BgRemovalResult(local, mout)
 
 
+# ------------------------------------------------------------------ NIfTI --
+@dataclass
+class NiftiData:
+    """src/io.rs:13-26 (plus the volume count nt and the on-disk datatype);
+    data is float64, Fortran order, volume after volume."""
+    data: np.ndarray
+    dims: tuple
+    voxel_size: tuple
+    affine: np.ndarray
+    scl_slope: float
+    scl_inter: float
+    nt: int = 1
+    datatype: int = 16
+
+
+def _nifti_from(info, data):
+    return NiftiData(data, (int(info.nx), int(info.ny), int(info.nz)),
+                     (info.vsx, info.vsy, info.vsz), np.array(list(info.affine)),
+                     info.scl_slope, info.scl_inter, int(info.nt), int(info.datatype))
+
+
+def load_nifti(raw: bytes) -> NiftiData:
+    """src/io.rs:62-158 / 160-229 (gzip detected by its magic bytes)"""
+    b = np.frombuffer(bytes(raw), dtype=np.uint8)
+    info = _CNiftiInfo()
+    _check(_lib.qsm_nifti_load(b.ctypes.data_as(C.c_void_p), b.size, C.byref(info), None, 0))
+    n = info.nx * info.ny * info.nz * info.nt
+    out = np.empty(n)
+    _check(_lib.qsm_nifti_load(b.ctypes.data_as(C.c_void_p), b.size, C.byref(info),
+                               out.ctypes.data_as(_D), n))
+    return _nifti_from(info, out)
+
+
+def save_nifti(data, dims, voxel_size, affine, gzip: bool = False) -> bytes:
+    """src/io.rs:262-357 (FLOAT32 data, sform from the affine)"""
+    d = _flat64(data)
+    vs = np.ascontiguousarray(np.asarray(voxel_size, dtype=np.float64))
+    af = np.ascontiguousarray(np.asarray(affine, dtype=np.float64).ravel())
+    p, n = C.c_void_p(), _SZ(0)
+    _check(_lib.qsm_nifti_save(d.ctypes.data_as(_D), int(dims[0]), int(dims[1]), int(dims[2]),
+                               vs.ctypes.data_as(_D), af.ctypes.data_as(_D), 1 if gzip else 0,
+                               C.byref(p), C.byref(n)))
+    try:
+        return C.string_at(p, n.value)
+    finally:
+        _lib.qsm_nifti_free(p)
+
+
+def read_nifti_file(path) -> NiftiData:
+    """src/io.rs:423-427"""
+    info = _CNiftiInfo()
+    pb = str(path).encode()
+    _check(_lib.qsm_nifti_read_file(pb, C.byref(info), None, 0))
+    n = info.nx * info.ny * info.nz * info.nt
+    out = np.empty(n)
+    _check(_lib.qsm_nifti_read_file(pb, C.byref(info), out.ctypes.data_as(_D), n))
+    return _nifti_from(info, out)
+
+
+def save_nifti_to_file(path, data, dims, voxel_size, affine):
+    """src/io.rs:429-452 (.nii.gz compressed by suffix)"""
+    d = _flat64(data)
+    vs = np.ascontiguousarray(np.asarray(voxel_size, dtype=np.float64))
+    af = np.ascontiguousarray(np.asarray(affine, dtype=np.float64).ravel())
+    _check(_lib.qsm_nifti_write_file(str(path).encode(), d.ctypes.data_as(_D), int(dims[0]),
+                                     int(dims[1]), int(dims[2]), vs.ctypes.data_as(_D),
+                                     af.ctypes.data_as(_D)))
+
+
 # ---------------------------------------------------------- device memory --
 class DeviceArray:
     """A device allocation owned by this library's HIP runtime, holding a
@@ -921,4 +1004,5 @@ __all__ = [
     "TvUnrolled", "tv_unrolled_vjp", "UnrolledTvFunction", "dipole_forward",
     "TkdParams", "Regularization", "TikhonovParams", "RtsParams", "SharpParams", "tkd", "tsvd",
     "tikhonov", "rts", "sharp", "laplacian_unwrap", "NltvParams", "nltv",
+    "NiftiData", "load_nifti", "save_nifti", "read_nifti_file", "save_nifti_to_file",
 ]

@@ -21,7 +21,8 @@ LIB = os.path.join(HERE, "libqsm_gpu.so")
 # QSM_FIXED_LENGTHS in qsm_misc.hip.
 FIXED_LENGTHS = [64, 128, 144, 160, 176, 192, 208, 224, 240, 256, 384]
 UNITS = ([("qsm_pass.hip", f"qsm_pass_{n}", [f"-DQSM_PASS_N={n}"]) for n in [0] + FIXED_LENGTHS]
-         + [("qsm_misc.hip", "qsm_misc", []), ("qsm_runtime.hip", "qsm_runtime", [])])
+         + [("qsm_misc.hip", "qsm_misc", []), ("qsm_runtime.hip", "qsm_runtime", []),
+            ("qsm_nifti.hip", "qsm_nifti", [])])
 
 HIPCC_FLAGS = [
     "-O3", "-std=c++17", "--offload-arch=gfx950", "-fPIC",
@@ -54,7 +55,7 @@ def build(jobs: int = 8, verbose: bool = True) -> str:
     with ThreadPoolExecutor(max_workers=max(1, jobs)) as ex:
         objs = list(ex.map(_compile, UNITS))
     tmp = LIB + ".tmp"
-    cmd = [hipcc(), "--offload-arch=gfx950", "-shared", "-fPIC", "-o", tmp, *objs]
+    cmd = [hipcc(), "--offload-arch=gfx950", "-shared", "-fPIC", "-o", tmp, *objs, "-lz"]
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         sys.stderr.write(r.stdout + r.stderr)

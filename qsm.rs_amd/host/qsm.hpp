@@ -13,6 +13,7 @@
 //   qsm::inversion::nltv            src/inversion/nltv.rs:63-133
 //   qsm::bgremove::sharp            src/bgremove/sharp.rs:48-56
 //   qsm::unwrap::laplacian_unwrap   src/unwrap/laplacian.rs:135-161
+//   qsm::io::load_nifti / save_nifti src/io.rs:62-357
 //   qsm::pipeline::run_dipole_inversion / run_bg_removal  src/pipeline/*.rs
 //
 // Algorithm functions throw qsm::Error on a non-zero status (the Rust
@@ -20,6 +21,7 @@
 // struct carrying a PipelineError, as the Rust pipeline does.
 #pragma once
 
+#include <array>
 #include <cstddef>
 #include <cstdint>
 #include <functional>
@@ -255,6 +257,44 @@ inline std::vector<double> laplacian_unwrap(const std::vector<double>& phase, co
     return out;
 }
 }  // namespace unwrap
+
+namespace io {
+// NiftiData, src/io.rs:13-26
+struct NiftiData {
+    std::vector<double> data;
+    std::tuple<size_t, size_t, size_t> dims;
+    std::tuple<double, double, double> voxel_size;
+    std::array<double, 16> affine;
+    double scl_slope, scl_inter;
+};
+
+// load_nifti, src/io.rs:62-158 (first volume of a 4-D file)
+inline NiftiData load_nifti(const std::vector<uint8_t>& bytes) {
+    qsm_nifti_info info;
+    check(qsm_nifti_load(bytes.data(), bytes.size(), &info, nullptr, 0));
+    std::vector<double> all(info.nx * info.ny * info.nz * info.nt);
+    check(qsm_nifti_load(bytes.data(), bytes.size(), &info, all.data(), all.size()));
+    all.resize(info.nx * info.ny * info.nz);
+    NiftiData d{std::move(all), {info.nx, info.ny, info.nz}, {info.vsx, info.vsy, info.vsz}, {},
+                info.scl_slope, info.scl_inter};
+    for (int i = 0; i < 16; ++i) d.affine[i] = info.affine[i];
+    return d;
+}
+
+// save_nifti / save_nifti_gz, src/io.rs:262-357
+inline std::vector<uint8_t> save_nifti(const std::vector<double>& data, std::tuple<size_t, size_t, size_t> dims,
+                                       std::tuple<double, double, double> vs,
+                                       const std::array<double, 16>& affine, bool gz = false) {
+    uint8_t* p = nullptr;
+    size_t n = 0;
+    const double v[3] = {std::get<0>(vs), std::get<1>(vs), std::get<2>(vs)};
+    check(qsm_nifti_save(data.data(), std::get<0>(dims), std::get<1>(dims), std::get<2>(dims), v,
+                         affine.data(), gz ? 1 : 0, &p, &n));
+    std::vector<uint8_t> out(p, p + n);
+    qsm_nifti_free(p);
+    return out;
+}
+}  // namespace io
 
 namespace pipeline {
 

@@ -292,6 +292,21 @@ static void test_nltv_reference_tests() {
     for (double v : r) EXPECT(std::isfinite(v), "finite");
 }
 
+// src/io.rs:477-546 (save header layout, round trip)
+static void test_nifti_save_and_load() {
+    std::vector<double> data(64);
+    for (size_t i = 0; i < data.size(); ++i) data[i] = i * 0.5 + 1.0;
+    const std::array<double, 16> aff = {1, 0, 0, 10, 0, 2, 0, 20, 0, 0, 3, 30, 0, 0, 0, 1};
+    auto bytes = io::save_nifti(data, {4, 4, 4}, {1.0, 2.0, 3.0}, aff);
+    EXPECT(bytes.size() == 352 + 64 * 4 && bytes[344] == 'n' && bytes[345] == '+', "header layout");
+    auto d = io::load_nifti(bytes);
+    EXPECT(d.dims == std::make_tuple(size_t(4), size_t(4), size_t(4)), "dims");
+    EXPECT(std::get<1>(d.voxel_size) == 2.0 && d.affine[3] == 10.0 && d.affine[11] == 30.0, "geometry");
+    for (size_t i = 0; i < data.size(); ++i) EXPECT(std::fabs(d.data[i] - data[i]) < 0.01, "values");
+    auto gz = io::save_nifti(data, {4, 4, 4}, {1.0, 2.0, 3.0}, aff, true);
+    EXPECT(gz[0] == 0x1f && gz[1] == 0x8b && io::load_nifti(gz).data == d.data, "gzip round trip");
+}
+
 int main() {
     void (*tests[])() = {test_tv_admm_zero_field, test_tv_admm_finite, test_tv_admm_smoother_than_tkd,
                          test_tv_params_default, test_vsharp_zero_field, test_vsharp_nonuniform_voxels,
@@ -300,7 +315,8 @@ int main() {
                          test_bg_removal_validates_dims, test_inversion_tv_and_rejects_tgv,
                          test_tkd_reference_tests, test_tikhonov_regularization_types,
                          test_rts_reference_tests, test_pipeline_default_rts_and_sharp_arm,
-                         test_laplacian_unwrap_reference_tests, test_nltv_reference_tests};
+                         test_laplacian_unwrap_reference_tests, test_nltv_reference_tests,
+                         test_nifti_save_and_load};
     int i = 0;
     for (auto t : tests) {
         const int before = g_fail;
