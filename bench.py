@@ -240,6 +240,10 @@ def main():
         "iteration_GBs": round(iter_bytes / (iter_ms * 1e-3) / 1e9, 1),
         "per_kernel_ms": {k: round(float(v), 5) for k, v in zip(KERNELS, ms)},
     }
+    # what a plain device copy reaches on this GPU, for reading the roofline
+    copy_gbs = qsm.hbm_copy_probe(1 << 30, 5)
+    roofline["copy_probe_GBs"] = round(copy_gbs, 1)
+    roofline["frac_of_copy"] = round(achieved / copy_gbs, 4)
     cpu = None
     if not a.no_cpu:
         threads = min(16, os.cpu_count() or 1)

@@ -293,6 +293,9 @@ int qsm_memcpy_htod(void* dst, const void* src, size_t bytes);
 int qsm_memcpy_dtoh(void* dst, const void* src, size_t bytes);
 int qsm_memcpy_dtod(void* dst, const void* src, size_t bytes);
 int qsm_device_synchronize(void);
+/* Achievable HBM rate reference: a one-element-per-thread device copy of
+ * `bytes`, read + write counted, best of `reps` launches (GB/s). */
+int qsm_hbm_copy_probe(size_t bytes, int reps, double* gbps);
 
 /* ---- differentiable unrolled TV-ADMM (no reference counterpart) ------- */
 /* K = params->max_iter iterations of tv_admm (src/inversion/tv.rs:63-97)

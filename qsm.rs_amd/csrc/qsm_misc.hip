@@ -112,6 +112,14 @@ __global__ void k_status_clear_done(VolStatus* status, int nvol) {
     if (v < nvol) status[v].done = 0;
 }
 
+// One float2 per thread, read and written once: the reference point for what
+// the FFT passes' HBM rates can reach on this device.
+__global__ __launch_bounds__(kThreads) void k_copy_probe(const float2* __restrict__ a, float2* __restrict__ b,
+                                                         size_t n) {
+    const size_t i = (size_t)blockIdx.x * kThreads + threadIdx.x;
+    if (i < n) b[i] = a[i];
+}
+
 // chi = x where mask != 0, else 0 (src/utils/mask.rs:47-53), widened to f64.
 __global__ __launch_bounds__(kThreads) void k_tv_output(const VolDesc* __restrict__ vols, LaunchTable lt,
                                                         const VolStatus* __restrict__ status) {
@@ -172,6 +180,9 @@ void launch_nltv_init(hipStream_t st, const VolDesc* d, const LaunchTable& lt) {
 }
 void launch_status_clear_done(hipStream_t st, VolStatus* s, int nvol) {
     k_status_clear_done<<<dim3((nvol + 63) / 64), dim3(64), 0, st>>>(s, nvol);
+}
+void launch_copy_probe(hipStream_t st, const float2* a, float2* b, size_t n) {
+    k_copy_probe<<<dim3((unsigned)((n + kThreads - 1) / kThreads)), dim3(kThreads), 0, st>>>(a, b, n);
 }
 void launch_tv_output(hipStream_t st, const VolDesc* d, const LaunchTable& lt, const VolStatus* status) {
     k_tv_output<<<dim3(lt.start[lt.nvol]), dim3(kThreads), 0, st>>>(d, lt, status);

@@ -24,6 +24,7 @@ void launch_nltv_weights(hipStream_t st, const VolDesc* d, const LaunchTable& lt
 void launch_nltv_wmax(hipStream_t st, const VolDesc* d, int nvol, int nblk, int sel);
 void launch_nltv_init(hipStream_t st, const VolDesc* d, const LaunchTable& lt);
 void launch_status_clear_done(hipStream_t st, VolStatus* s, int nvol);
+void launch_copy_probe(hipStream_t st, const float2* a, float2* b, size_t n);
 bool fixed_length(int n);
 
 // ------------------------------------------------------------ error state --
@@ -1547,6 +1548,43 @@ int qsm_run_bg_removal_cfg(const double* field_ppm, size_t field_len, const uint
             return fail(QSM_ECONFIG, "invalid config: algorithm %d has no GPU backend here",
                         (int)cfg->algorithm);
     }
+}
+
+// Device-to-device copy rate of a plain one-element-per-thread kernel over
+// `bytes` (read + write counted), best of `reps` timed launches.
+int qsm_hbm_copy_probe(size_t bytes, int reps, double* gbps) {
+    g_err.clear();
+    if (!gbps || bytes < 16 || reps < 1) return fail(QSM_EPARAM, "bad probe arguments");
+    const size_t n = bytes / sizeof(float2);
+    float2 *a = nullptr, *b = nullptr;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    auto done = [&](int code) {
+        if (a) (void)hipFree(a);
+        if (b) (void)hipFree(b);
+        if (e0) (void)hipEventDestroy(e0);
+        if (e1) (void)hipEventDestroy(e1);
+        return code;
+    };
+    int rc;
+    if ((rc = dev_alloc(&a, n)) || (rc = dev_alloc(&b, n))) return done(rc);
+    QSM_HIP(hipEventCreate(&e0));
+    QSM_HIP(hipEventCreate(&e1));
+    launch_fill_zero(nullptr, reinterpret_cast<float*>(a), 2 * n);
+    launch_copy_probe(nullptr, a, b, n);  // warm-up
+    double best = 0.0;
+    for (int r = 0; r < reps; ++r) {
+        QSM_HIP(hipEventRecord(e0, nullptr));
+        launch_copy_probe(nullptr, a, b, n);
+        QSM_HIP(hipEventRecord(e1, nullptr));
+        QSM_HIP(hipEventSynchronize(e1));
+        float ms = 0.f;
+        QSM_HIP(hipEventElapsedTime(&ms, e0, e1));
+        const double g = 2.0 * (double)n * sizeof(float2) / (ms * 1e-3) / 1e9;
+        if (g > best) best = g;
+    }
+    QSM_CHECK_LAUNCH();
+    *gbps = best;
+    return done(QSM_OK);
 }
 
 int qsm_device_alloc(size_t bytes, void** out) {
