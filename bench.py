@@ -69,7 +69,7 @@ def alg_bytes(dims):
     return [
         28 * N + 8 * Nh,   # read x (4N) + u (12N); write u (12N) + half spectrum (8Nh)
         16 * Nh,           # read + write half spectrum
-        28 * Nh,           # spectrum r/w (16Nh) + f^ (8Nh) + rho*inv_a/N (4Nh)
+        24 * Nh,           # spectrum r/w (16Nh) + f^ (8Nh); rho*inv_a/N is formed in-kernel
         16 * Nh,
         8 * Nh + 8 * N,    # read spectrum + x_prev, write x
     ]
@@ -424,8 +424,8 @@ def bench_vjp(a, qsm, phantom):
     N = n
     Nh = (dims[0] // 2 + 1) * dims[1] * dims[2]
     # backward iteration: adjoint source (c-bar 4N, s-bar 12N in, 12N out, q 3N,
-    # acc 8N) + spectrum 8Nh, y/z/y passes 60Nh, C2R 8Nh + 4N
-    bwd_bytes = K * (43 * N + 76 * Nh)
+    # acc 8N) + spectrum 8Nh, y/z/y passes 48Nh, C2R 8Nh + 4N
+    bwd_bytes = K * (43 * N + 64 * Nh)
     fwd_bytes = sum(alg_bytes(dims)) * K + 3 * N * (K - 1)
     print(json.dumps({
         "metric": "unrolled TV-ADMM (K=20) forward+VJP per second at 192x192x128",

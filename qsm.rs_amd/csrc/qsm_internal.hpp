@@ -124,7 +124,7 @@ enum PassKind : int {
     P_C2R_ADJ, P_C2R_F64OUT,
     // closed-form inversions and RTS
     P_Z_FILTER, P_Z_RTS_SETUP, P_Z_TV_SOLVE0, P_C2R_F64_MASKED, P_C2R_X0, P_R2C_X0,
-    P_R2C_WRAPLAP, P_R2C_TVDIV_W, P_R2C_NLTV_BND,
+    P_R2C_WRAPLAP, P_R2C_TVDIV_W, P_R2C_NLTV_BND, P_Z_RTS_SOLVE,
     P_KIND_COUNT
 };
 

@@ -17,7 +17,7 @@
 // over a compile-time count into registers); twiddles are staged in LDS.
 #pragma once
 
-#include "fft_fixed.hpp"
+#include "fft_col2.hpp"
 #include "qsm_internal.hpp"
 
 namespace qsm {
@@ -63,6 +63,10 @@ __host__ __device__ constexpr int col_emax(int n, int eg) {
 __host__ __device__ constexpr int x_lds(int n) { return n > 0 ? qsmfft::fixed_rows(n) * n : kThreads * 16; }
 __host__ __device__ constexpr int col_lds(int n, int eg) { return n > 0 ? kTile * n : kThreads * eg; }
 __host__ __device__ constexpr int tw_lds(int n) { return n > 0 ? n : kMaxLen; }
+// column passes of the lengths fft_col2.hpp covers use its padded layout
+__host__ __device__ constexpr int col_lds2(int n, int eg) {
+    return qsmfft::col2_ok(n) ? 16 * n + 256 : col_lds(n, eg);
+}
 
 template <int N, int EMAX, bool COLFAST, int LOG2C, bool INV, int NC = 0>
 __device__ __forceinline__ void run_fft(float2* lds, const qsmfft::Plan& P, int ncols, int es, int cs,
@@ -73,10 +77,25 @@ __device__ __forceinline__ void run_fft(float2* lds, const qsmfft::Plan& P, int 
         qsmfft::lds_fft<EMAX, COLFAST, LOG2C, INV>(lds, P, ncols, es, cs, twl);
 }
 
-__device__ __forceinline__ void load_twiddles(float2* twl, const float2* __restrict__ tw, int n) {
-    const QSM_GLOBAL float2* g = glb(tw);
-    for (int q = threadIdx.x; q < n; q += kThreads) twl[q] = ldg2(g + q);
-}
+// A pass's twiddle table (exp(-2 pi i q / n), q < n) is fetched into
+// registers before the pass issues its tile loads and written to LDS after
+// them: a table load whose value had to reach LDS first would hold every block
+// for one memory round trip before its tile loads went out.
+template <int N>
+struct TwStage {
+    static constexpr int K = (tw_lds(N) + kThreads - 1) / kThreads;
+    float2 r[K];
+    __device__ __forceinline__ void fetch(const float2* __restrict__ tw, int n) {
+        const QSM_GLOBAL float2* g = glb(tw);
+#pragma unroll
+        for (int t = 0; t < K; ++t) r[t] = ldg2(g + min((int)threadIdx.x + t * kThreads, n - 1));
+    }
+    // threads past n hold entry n - 1 and write it again (no branch)
+    __device__ __forceinline__ void commit(float2* twl, int n) const {
+#pragma unroll
+        for (int t = 0; t < K; ++t) twl[min((int)threadIdx.x + t * kThreads, n - 1)] = r[t];
+    }
+};
 
 __device__ __forceinline__ float shrinkf(float x, float t) {
     // src/utils/mod.rs:44-52
@@ -167,12 +186,14 @@ __device__ __forceinline__ void col_load(float2* lds, const QSM_GLOBAL float2* _
                                          size_t stride, int n, int kx0, int nxh) {
     float2 r[EMAX];
     const int total = n * kTile;
+    // unconditional loads: rows past n and columns past nxh read in-bounds
+    // elements that are never stored
 #pragma unroll
     for (int t = 0; t < EMAX; ++t) {
         const int e = threadIdx.x + t * kThreads;
-        const int row = e >> kLog2Tile;
-        const int kx = kx0 + (e & (kTile - 1));
-        r[t] = (e < total && kx < nxh) ? ldg2(base + kx + row * stride) : make_float2(0.0f, 0.0f);
+        const int row = min(e >> kLog2Tile, n - 1);
+        const int kx = min(kx0 + (e & (kTile - 1)), nxh - 1);
+        r[t] = ldg2(base + kx + row * stride);
     }
 #pragma unroll
     for (int t = 0; t < EMAX; ++t) {
@@ -192,6 +213,44 @@ __device__ __forceinline__ void col_store(const float2* lds, QSM_GLOBAL float2* 
         const int kx = kx0 + (e & (kTile - 1));
         if (e < total && kx < nxh) stg2(base + kx + row * stride, lds[e]);
     }
+}
+
+// Tile I/O in the register layouts of fft_col2.hpp (thread (j, c) = (t >> 4,
+// t & 15), column kx0 + c).  L16: rows j + 16 r, r < R0; LR: rows j + R0 r,
+// r < 16, threads j < R0.  Columns kx >= nxh load column nxh - 1 (in bounds,
+// their results are never stored) and are not written.
+static_assert(kTile == 16 && kThreads == 256, "fft_col2.hpp assumes 16-column tiles and 256 threads");
+
+template <int R0>
+__device__ __forceinline__ void l16_load(float2 (&v)[16], const QSM_GLOBAL float2* __restrict__ base,
+                                         size_t stride, int kx0, int nxh) {
+    const int c = threadIdx.x & 15, j = threadIdx.x >> 4;
+    const int kx = min(kx0 + c, nxh - 1);
+    const QSM_GLOBAL float2* p = base + kx + (size_t)j * stride;
+#pragma unroll
+    for (int r = 0; r < R0; ++r) v[r] = ldg2(p + (size_t)(16 * r) * stride);
+}
+
+template <int R0>
+__device__ __forceinline__ void l16_store(const float2 (&v)[16], QSM_GLOBAL float2* __restrict__ base,
+                                          size_t stride, int kx0, int nxh) {
+    const int c = threadIdx.x & 15, j = threadIdx.x >> 4;
+    const int kx = kx0 + c;
+    if (kx >= nxh) return;
+    QSM_GLOBAL float2* p = base + kx + (size_t)j * stride;
+#pragma unroll
+    for (int r = 0; r < R0; ++r) stg2(p + (size_t)(16 * r) * stride, v[r]);
+}
+
+template <int R0>
+__device__ __forceinline__ void lr_store(const float2 (&v)[16], QSM_GLOBAL float2* __restrict__ base,
+                                         size_t stride, int kx0, int nxh) {
+    const int c = threadIdx.x & 15, j = threadIdx.x >> 4;
+    const int kx = kx0 + c;
+    if (kx >= nxh || j >= R0) return;
+    QSM_GLOBAL float2* p = base + kx + (size_t)j * stride;
+#pragma unroll
+    for (int r = 0; r < 16; ++r) stg2(p + (size_t)(R0 * r) * stride, v[r]);
 }
 
 // ------------------------------------------------ x-pass source fills ----
@@ -622,7 +681,8 @@ __global__ __launch_bounds__(kThreads) void k_r2c_x(const VolDesc* __restrict__ 
     const int RB = N > 0 ? qsmfft::fixed_rows(N) : V.rows_x;
     const int nrows = V.ny * V.nz;
     const int row0 = lb * RB * 2;
-    load_twiddles(twl, tw + V.px.tw_off, nx);
+    TwStage<N> tws;
+    tws.fetch(tw + V.px.tw_off, nx);
     if constexpr (SRC == SRC_TVDIV) {
         fill_tvdiv<EMAX, false>(lds, V, nx, RB, glb(V.x[aux]), glb(V.u[aux ^ 1]), glb(V.u[aux]), row0,
                                 nrows, nullptr);
@@ -649,6 +709,7 @@ __global__ __launch_bounds__(kThreads) void k_r2c_x(const VolDesc* __restrict__ 
     } else {
         fill_simple<EMAX, SRC>(lds, V, nx, RB, row0, nrows, aux);
     }
+    tws.commit(twl, nx);
     __syncthreads();
     run_fft<N, EMAX, false, 0, false>(lds, V.px, RB, 1, nx, twl);
     QSM_GLOBAL float2* __restrict__ out = glb(V.spec[out_sel]);
@@ -678,7 +739,7 @@ __global__ __launch_bounds__(kThreads) void k_y_pass(const VolDesc* __restrict__
                                                      const float2* __restrict__ tw, int sel, int sel2,
                                                      const VolStatus* __restrict__ status) {
     constexpr int EMAX = col_emax(N, EG);
-    __shared__ float2 lds[col_lds(N, EG)];
+    __shared__ float2 lds[col_lds2(N, EG)];
     __shared__ float2 twl[tw_lds(N)];
     const int bid = xcd_remap(blockIdx.x, gridDim.x);
     int lb;
@@ -694,11 +755,22 @@ __global__ __launch_bounds__(kThreads) void k_y_pass(const VolDesc* __restrict__
     const int nxh = V.nxh;
     const int ny = N > 0 ? N : V.ny;
     QSM_GLOBAL float2* buf = glb(V.spec[s]) + (size_t)kz * nxh * ny;
-    load_twiddles(twl, tw + V.py.tw_off, ny);
-    col_load<EMAX>(lds, buf, nxh, ny, kx0, nxh);
-    __syncthreads();
-    run_fft<N, EMAX, true, kLog2Tile, INV>(lds, V.py, kTile, kTile, 1, twl);
-    col_store<EMAX>(lds, buf, nxh, ny, kx0, nxh);
+    TwStage<N> tws;
+    tws.fetch(tw + V.py.tw_off, ny);
+    if constexpr (qsmfft::col2_ok(N)) {
+        constexpr int R0 = N / 16;
+        float2 v[16];
+        l16_load<R0>(v, buf, nxh, kx0, nxh);
+        tws.commit(twl, ny);
+        qsmfft::Col2<R0>::template l16_to_lr<INV>(v, lds, twl);
+        lr_store<R0>(v, buf, nxh, kx0, nxh);
+    } else {
+        col_load<EMAX>(lds, buf, nxh, ny, kx0, nxh);
+        tws.commit(twl, ny);
+        __syncthreads();
+        run_fft<N, EMAX, true, kLog2Tile, INV>(lds, V.py, kTile, kTile, 1, twl);
+        col_store<EMAX>(lds, buf, nxh, ny, kx0, nxh);
+    }
 }
 
 // ---------------------------------------------------- TV spectral passes --
@@ -807,7 +879,55 @@ enum ZOp : int {
     ZOP_DIPOLE = 7,     // D chi^ / N  (the dipole forward operator, self-adjoint)
     ZOP_FILTER = 8,     // closed-form inversion multiplier / N (filter_mult)
     ZOP_RTS_SETUP = 9,  // as TV_SETUP with the RTS multipliers
-    ZOP_TV_SOLVE0 = 10  // first ADMM step from g = 0: x^ = f^, inverse z only
+    ZOP_TV_SOLVE0 = 10, // first ADMM step from g = 0: x^ = f^, inverse z only
+    ZOP_RTS_SOLVE = 11  // ZOP_TV_SOLVE with the stored multiplier q (RTS)
+};
+
+// The TV x-update multiplier q = rho inv_a / N, inv_a = 1/(D^2 + rho L) (0 at
+// DC; src/inversion/admm.rs:157-168), averaged over k and its mirror as the
+// stored one is, formed in FP32 where it is used instead of being read (it
+// is the same at every iteration; reading it costs 4 bytes per spectral
+// element per iteration).  There is no threshold in it besides DC, so FP32
+// only perturbs it by a few ulp.  k and its mirror differ only on Nyquist
+// planes, where the mirror keeps the (negative) Nyquist frequency.
+// init: the x and y terms of one column (FP64, once per thread); at: one kz,
+// with twz = exp(-2 pi i kz / nz) from the z pass's twiddle table.
+struct TvQ {
+    float sxy, sxym, k2xy, lxy, dfz, bz, hz2, rho, scale;
+    __device__ __forceinline__ void init(const VolDesc& V, int kx, int ky) {
+        const double fx = fftfreq_at(kx, V.nx, V.vsx), fy = fftfreq_at(ky, V.ny, V.vsy);
+        const double fxm = fftfreq_at(mirror_idx(kx, V.nx), V.nx, V.vsx);
+        const double fym = fftfreq_at(mirror_idx(ky, V.ny), V.ny, V.vsy);
+        sxy = (float)(fx * V.bx + fy * V.by);
+        sxym = (float)(fxm * V.bx + fym * V.by);
+        k2xy = (float)(fx * fx + fy * fy);
+        const double two_pi = 6.283185307179586476925286766559;
+        lxy = (float)((2.0 - 2.0 * cos(two_pi * kx / V.nx)) / (V.vsx * V.vsx) +
+                      (2.0 - 2.0 * cos(two_pi * ky / V.ny)) / (V.vsy * V.vsy));
+        dfz = (float)(1.0 / ((double)V.nz * V.vsz));
+        bz = (float)V.bz;
+        hz2 = (float)(1.0 / (V.vsz * V.vsz));
+        rho = (float)V.rho;
+        scale = (float)(0.5 * V.rho / ((double)V.nx * V.ny * V.nz));
+    }
+    __device__ __forceinline__ float at(int kz, int nz, float2 twz) const {
+        const int sz = kz < (nz + 1) / 2 ? kz : kz - nz;           // fftfreq index
+        const int szm = (2 * kz == nz) ? sz : -sz;                  // its mirror's
+        const float fz = (float)sz * dfz, fzm = (float)szm * dfz;
+        const float kdb = sxy + fz * bz, kdbm = sxym + fzm * bz;
+        const float k2 = k2xy + fz * fz;
+        // 2 - 2 cos(t) = 2 sin^2(t) / (1 + cos(t)) where cos(t) > 0 (no cancellation)
+        const float cz = twz.x;
+        const float lz = (cz > 0.0f ? 2.0f * (twz.y * twz.y) / (1.0f + cz) : 2.0f - 2.0f * cz) * hz2;
+        const float rl = rho * (lxy + lz);
+        const float rk = k2 > 1e-20f ? __builtin_amdgcn_rcpf(k2) : 0.0f;  // D = 0 at DC
+        const float D = k2 > 1e-20f ? (1.0f / 3.0f) - kdb * kdb * rk : 0.0f;
+        const float Dm = k2 > 1e-20f ? (1.0f / 3.0f) - kdbm * kdbm * rk : 0.0f;
+        const float a = D * D + rl, am = Dm * Dm + rl;
+        const float ia = a > 1e-20f ? __builtin_amdgcn_rcpf(a) : 0.0f;
+        const float iam = am > 1e-20f ? __builtin_amdgcn_rcpf(am) : 0.0f;
+        return scale * (ia + iam);
+    }
 };
 
 // TV z pass.  SETUP: forward z of FFT(f) (after x, y), then
@@ -819,12 +939,83 @@ enum ZOp : int {
 // 106-108, scaled by 1/N), inverse z.
 // ADJ: forward z, -q * spectrum, inverse z.  APPLYA / DIPOLE: forward z,
 // times the Hermitian part of D inv_a / N or of D / N (in FP64), inverse z.
+// The per-iteration ops (SOLVE, SOLVE0, ADJ) at the lengths of fft_col2.hpp:
+// forward z from the load layout, the product in registers, inverse z back
+// to the load layout; two LDS exchanges in all.
+template <int N, int OP>
+__device__ __forceinline__ void z_tv_col2(const VolDesc& V, float2* lds, float2* twl, const TwStage<N>& tws,
+                                          int ky, int kx0) {
+    constexpr int R0 = N / 16;
+    using C = qsmfft::Col2<R0>;
+    const int nxh = V.nxh;
+    const size_t zs = (size_t)nxh * V.ny;
+    const size_t base = (size_t)ky * nxh;
+    QSM_GLOBAL float2* buf = glb(V.spec[SPEC_WORK]) + base;
+    float2 v[16];
+    if constexpr (OP == ZOP_TV_SOLVE0) {
+        // g = 0 before the first step (AdmmBuffers::new), so div = 0 and x^ = f^
+        l16_load<R0>(v, glb(V.spec[SPEC_A]) + base, zs, kx0, nxh);
+        tws.commit(twl, N);
+        C::template l16_to_lr<true>(v, lds, twl);
+        lr_store<R0>(v, buf, zs, kx0, nxh);
+        return;
+    }
+    l16_load<R0>(v, buf, zs, kx0, nxh);
+    // x-update operands in the LR layout, in flight during the forward FFT
+    const int c = threadIdx.x & 15, j = threadIdx.x >> 4;
+    const bool act = j < R0;
+    constexpr bool HAS_F = OP == ZOP_TV_SOLVE || OP == ZOP_RTS_SOLVE;
+    constexpr bool Q_MEM = OP == ZOP_RTS_SOLVE;
+    float2 fr[HAS_F ? 16 : 1];
+    float qr[16];
+    {
+        // threads j >= R0 load a copy of row j mod R0 (unused): loads under a
+        // branch would make the compiler wait for every load before the FFT
+        const size_t off = base + min(kx0 + c, nxh - 1) + (size_t)(act ? j : j % R0) * zs;
+        const QSM_GLOBAL float2* __restrict__ F = glb(V.spec[SPEC_A]) + off;
+        const QSM_GLOBAL float* __restrict__ Q = glb(V.q) + off;
+#pragma unroll
+        for (int r = 0; r < 16; ++r) {
+            if constexpr (HAS_F) fr[r] = ldg2(F + (size_t)(R0 * r) * zs);
+            if constexpr (Q_MEM) qr[r] = Q[(size_t)(R0 * r) * zs];
+        }
+    }
+    tws.commit(twl, N);
+    C::template a_fwd<false>(threadIdx.x, v, lds);
+    __syncthreads();
+    if constexpr (!Q_MEM) {
+        // the tile is in LDS and the operands are in flight: form q here
+        TvQ tq;
+        tq.init(V, min(kx0 + c, nxh - 1), ky);
+        if (act) {
+            float2 tz[16];  // all LDS reads issued before the first use
+#pragma unroll
+            for (int r = 0; r < 16; ++r) tz[r] = twl[j + R0 * r];
+#pragma unroll
+            for (int r = 0; r < 16; ++r) qr[r] = tq.at(j + R0 * r, N, tz[r]);
+        }
+    }
+    C::template b_fwd<false>(threadIdx.x, v, lds, twl);
+    if (act) {
+#pragma unroll
+        for (int r = 0; r < 16; ++r) {
+            if constexpr (HAS_F)
+                v[r] = make_float2(fr[r].x - qr[r] * v[r].x, fr[r].y - qr[r] * v[r].y);
+            else
+                v[r] = make_float2(-qr[r] * v[r].x, -qr[r] * v[r].y);
+        }
+    }
+    __syncthreads();  // every read of the first exchange precedes the second's writes
+    C::template lr_to_l16<true>(v, lds, twl);
+    l16_store<R0>(v, buf, zs, kx0, nxh);
+}
+
 template <int N, int EG, int OP>
 __global__ __launch_bounds__(kThreads) void k_z_tv(const VolDesc* __restrict__ vols, LaunchTable lt,
                                                    const float2* __restrict__ tw,
                                                    const VolStatus* __restrict__ status) {
     constexpr int EMAX = col_emax(N, EG);
-    __shared__ float2 lds[col_lds(N, EG)];
+    __shared__ float2 lds[col_lds2(N, EG)];
     __shared__ float2 twl[tw_lds(N)];
     const int bid = xcd_remap(blockIdx.x, gridDim.x);
     int lb;
@@ -840,10 +1031,17 @@ __global__ __launch_bounds__(kThreads) void k_z_tv(const VolDesc* __restrict__ v
     const size_t base = (size_t)ky * nxh;
     const int total = nz * kTile;
     QSM_GLOBAL float2* buf = glb(V.spec[SPEC_WORK]) + base;
-    load_twiddles(twl, tw + V.pz.tw_off, nz);
+    TwStage<N> tws;
+    tws.fetch(tw + V.pz.tw_off, nz);
+    if constexpr (qsmfft::col2_ok(N) && (OP == ZOP_TV_SOLVE || OP == ZOP_TV_SOLVE0 || OP == ZOP_TV_ADJ ||
+                                         OP == ZOP_RTS_SOLVE)) {
+        z_tv_col2<N, OP>(V, lds, twl, tws, ky, kx0);
+        return;
+    }
     if constexpr (OP == ZOP_TV_SOLVE0) {
         // g = 0 before the first step (AdmmBuffers::new), so div = 0 and x^ = f^
         col_load<EMAX>(lds, glb(V.spec[SPEC_A]) + base, zs, nz, kx0, nxh);
+        tws.commit(twl, nz);
         __syncthreads();
         run_fft<N, EMAX, true, kLog2Tile, true>(lds, V.pz, kTile, kTile, 1, twl);
         col_store<EMAX>(lds, buf, zs, nz, kx0, nxh);
@@ -851,10 +1049,11 @@ __global__ __launch_bounds__(kThreads) void k_z_tv(const VolDesc* __restrict__ v
     }
     col_load<EMAX>(lds, buf, zs, nz, kx0, nxh);
     // x-update operands, in flight during the forward FFT
-    constexpr bool USE_Q = OP == ZOP_TV_SOLVE || OP == ZOP_TV_ADJ;
-    float2 fr[OP == ZOP_TV_SOLVE ? EMAX : 1];
-    float qr[USE_Q ? EMAX : 1];
-    if constexpr (USE_Q) {
+    constexpr bool HAS_F = OP == ZOP_TV_SOLVE || OP == ZOP_RTS_SOLVE;
+    constexpr bool Q_MEM = OP == ZOP_RTS_SOLVE;
+    float2 fr[HAS_F ? EMAX : 1];
+    float qr[Q_MEM ? EMAX : 1];
+    if constexpr (HAS_F || Q_MEM) {
         const QSM_GLOBAL float2* __restrict__ F = glb(V.spec[SPEC_A]) + base;
         const QSM_GLOBAL float* __restrict__ Q = glb(V.q) + base;
 #pragma unroll
@@ -863,10 +1062,11 @@ __global__ __launch_bounds__(kThreads) void k_z_tv(const VolDesc* __restrict__ v
             const int kz = e >> kLog2Tile;
             const int kx = kx0 + (e & (kTile - 1));
             const bool in = e < total && kx < nxh;
-            if constexpr (OP == ZOP_TV_SOLVE) fr[t] = in ? ldg2(F + kx + kz * zs) : make_float2(0.0f, 0.0f);
-            qr[t] = in ? Q[kx + kz * zs] : 0.0f;
+            if constexpr (HAS_F) fr[t] = in ? ldg2(F + kx + kz * zs) : make_float2(0.0f, 0.0f);
+            if constexpr (Q_MEM) qr[t] = in ? Q[kx + kz * zs] : 0.0f;
         }
     }
+    tws.commit(twl, nz);
     __syncthreads();
     run_fft<N, EMAX, true, kLog2Tile, false>(lds, V.pz, kTile, kTile, 1, twl);
     if constexpr (OP == ZOP_TV_SETUP || OP == ZOP_RTS_SETUP) {
@@ -920,15 +1120,21 @@ __global__ __launch_bounds__(kThreads) void k_z_tv(const VolDesc* __restrict__ v
                 lds[e] = make_float2((float)((double)sv.x * m), (float)((double)sv.y * m));
             }
         } else {
+            // a thread's elements all lie in column threadIdx.x mod kTile
+            TvQ tq;
+            if constexpr (!Q_MEM) tq.init(V, min(kx0 + (int)(threadIdx.x & (kTile - 1)), nxh - 1), ky);
 #pragma unroll
             for (int t = 0; t < EMAX; ++t) {
                 const int e = threadIdx.x + t * kThreads;
                 if (e < total) {
                     const float2 sv = lds[e];
-                    if constexpr (OP == ZOP_TV_SOLVE)
-                        lds[e] = make_float2(fr[t].x - qr[t] * sv.x, fr[t].y - qr[t] * sv.y);
+                    float q;
+                    if constexpr (Q_MEM) q = qr[t];
+                    else q = tq.at(e >> kLog2Tile, nz, twl[e >> kLog2Tile]);
+                    if constexpr (HAS_F)
+                        lds[e] = make_float2(fr[t].x - q * sv.x, fr[t].y - q * sv.y);
                     else
-                        lds[e] = make_float2(-qr[t] * sv.x, -qr[t] * sv.y);
+                        lds[e] = make_float2(-q * sv.x, -q * sv.y);
                 }
             }
         }
@@ -957,10 +1163,13 @@ __device__ __forceinline__ void c2r_load(float2* lds, int nx, int nxh, int RB,
         const int k = e - p * nx;
         const bool neg = (2 * k > nx);
         const int kk = neg ? nx - k : k;
-        const int ra = row0 + step * p;
-        const int rb = (step == 2) ? ra + 1 : ra;
-        float2 A = (e < total && ra < nrows) ? ldg2(sa + kk + (size_t)ra * nxh) : make_float2(0.0f, 0.0f);
-        float2 B = (e < total && rb < nrows) ? ldg2(sb + kk + (size_t)rb * nxh) : make_float2(0.0f, 0.0f);
+        // rows past the volume (or past the block for e >= total) read its
+        // last row: in bounds, and their outputs are never stored (each
+        // half of the packed row is Hermitian, so they do not mix)
+        const int ra = min(row0 + step * p, nrows - 1);
+        const int rb = min((step == 2) ? row0 + step * p + 1 : row0 + step * p, nrows - 1);
+        float2 A = ldg2(sa + kk + (size_t)ra * nxh);
+        float2 B = ldg2(sb + kk + (size_t)rb * nxh);
         if (kk == 0 || 2 * kk == nx) { A.y = 0.0f; B.y = 0.0f; }
         if (neg) { A.y = -A.y; B.y = -B.y; }
         r[t] = make_float2(A.x - B.y, A.y + B.x);
@@ -1003,9 +1212,11 @@ __global__ __launch_bounds__(kThreads) void k_c2r_tv(const VolDesc* __restrict__
     const int row0 = lb * RB * 2;
     const int total = RB * nx;
     const QSM_GLOBAL float2* s = glb(V.spec[SPEC_WORK]);
-    load_twiddles(twl, tw + V.px.tw_off, nx);
+    TwStage<N> tws;
+    tws.fetch(tw + V.px.tw_off, nx);
     c2r_load<EMAX>(lds, nx, nxh, RB, s, s, row0, 2, nrows);
     if constexpr (MODE != C2R_TV) {
+        tws.commit(twl, nx);
         __syncthreads();
         run_fft<N, EMAX, false, 0, true, RBF>(lds, V.px, RB, 1, nx, twl);
 #pragma unroll
@@ -1054,6 +1265,7 @@ __global__ __launch_bounds__(kThreads) void k_c2r_tv(const VolDesc* __restrict__
         xp[t].x = xo[i + (size_t)ra * nx];
         xp[t].y = xo[i + (size_t)rb * nx];
     }
+    tws.commit(twl, nx);
     __syncthreads();
     run_fft<N, EMAX, false, 0, true, RBF>(lds, V.px, RB, 1, nx, twl);
     double sd = 0.0, sx = 0.0;
@@ -1119,8 +1331,10 @@ __global__ __launch_bounds__(kThreads) void k_z_vs(const VolDesc* __restrict__ v
     const size_t base = (size_t)ky * nxh;
     const int total = nz * kTile;
     QSM_GLOBAL float2* buf = glb(V.spec[OP == ZOP_VS_FWD ? sel : SPEC_WORK]) + base;
-    load_twiddles(twl, tw + V.pz.tw_off, nz);
+    TwStage<N> tws;
+    tws.fetch(tw + V.pz.tw_off, nz);
     col_load<EMAX>(lds, buf, zs, nz, kx0, nxh);
+    tws.commit(twl, nz);
     __syncthreads();
     run_fft<N, EMAX, true, kLog2Tile, false>(lds, V.pz, kTile, kTile, 1, twl);
     if constexpr (OP == ZOP_VS_FWD) {
@@ -1219,8 +1433,10 @@ __global__ __launch_bounds__(kThreads) void k_c2r_vs_select(const VolDesc* __res
     const int RB = N > 0 ? qsmfft::fixed_rows(N) : V.rows_x;
     const int nrows = V.ny * V.nz;
     const int row0 = lb * RB;
-    load_twiddles(twl, tw + V.px.tw_off, nx);
+    TwStage<N> tws;
+    tws.fetch(tw + V.px.tw_off, nx);
     c2r_load<EMAX>(lds, nx, nxh, RB, glb(V.spec[SPEC_WORK]), glb(V.spec[SPEC_C]), row0, 1, nrows);
+    tws.commit(twl, nx);
     QSM_GLOBAL uint8_t* fin = glb(V.vs_final);
     QSM_GLOBAL float* loc = glb(V.vs_local);
     __syncthreads();
@@ -1265,8 +1481,10 @@ __global__ __launch_bounds__(kThreads) void k_c2r_vs_out(const VolDesc* __restri
     const int nrows = V.ny * V.nz;
     const int row0 = lb * RB * 2;
     const QSM_GLOBAL float2* s = glb(V.spec[SPEC_WORK]);
-    load_twiddles(twl, tw + V.px.tw_off, nx);
+    TwStage<N> tws;
+    tws.fetch(tw + V.px.tw_off, nx);
     c2r_load<EMAX>(lds, nx, nxh, RB, s, s, row0, 2, nrows);
+    tws.commit(twl, nx);
     __syncthreads();
     run_fft<N, EMAX, false, 0, true>(lds, V.px, RB, 1, nx, twl);
     const int total = RB * nx;

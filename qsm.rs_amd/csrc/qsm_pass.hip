@@ -80,6 +80,7 @@ static void launch_cols(int kind, const LaunchTable& lt, const PassArgs& a) {
         case P_Z_FILTER: k_z_tv<N, EG, ZOP_FILTER><<<g, b, 0, st>>>(a.vols, lt, a.tw, nullptr); break;
         case P_Z_RTS_SETUP: k_z_tv<N, EG, ZOP_RTS_SETUP><<<g, b, 0, st>>>(a.vols, lt, a.tw, nullptr); break;
         case P_Z_TV_SOLVE0: k_z_tv<N, EG, ZOP_TV_SOLVE0><<<g, b, 0, st>>>(a.vols, lt, a.tw, a.status); break;
+        case P_Z_RTS_SOLVE: k_z_tv<N, EG, ZOP_RTS_SOLVE><<<g, b, 0, st>>>(a.vols, lt, a.tw, a.status); break;
         case P_Z_VS_FWD: k_z_vs<N, EG, ZOP_VS_FWD><<<g, b, 0, st>>>(a.vols, lt, a.tw, a.sel, 0, 0); break;
         case P_Z_VS_RADIUS: k_z_vs<N, EG, ZOP_VS_RADIUS><<<g, b, 0, st>>>(a.vols, lt, a.tw, 0, a.ridx, a.first); break;
         case P_Z_VS_DECONV: k_z_vs<N, EG, ZOP_VS_DECONV><<<g, b, 0, st>>>(a.vols, lt, a.tw, 0, 0, a.divide); break;

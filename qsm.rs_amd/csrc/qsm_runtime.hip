@@ -391,7 +391,7 @@ static int enqueue_iteration(qsm_tv_batch* B, size_t it, hipEvent_t closed, int 
     a.status_rw = nullptr;
     a.mirror = nullptr;
     run_groups(P_Y_FWD, B->gy, a);
-    run_groups(P_Z_TV_SOLVE, B->gz, a);
+    run_groups(B->method == 1 ? P_Z_RTS_SOLVE : P_Z_TV_SOLVE, B->gz, a);
     run_groups(P_Y_INV, B->gy, a);
     run_groups(P_C2R_TV, B->gxc, a);
     QSM_CHECK_LAUNCH();
@@ -519,7 +519,7 @@ static int tv_batch_profile_impl(qsm_tv_batch* B, size_t iters, double* ms_out, 
         QSM_HIP(hipEventRecord(ev[1], st));
         run_groups(P_Y_FWD, B->gy, a);
         QSM_HIP(hipEventRecord(ev[2], st));
-        run_groups(P_Z_TV_SOLVE, B->gz, a);
+        run_groups(B->method == 1 ? P_Z_RTS_SOLVE : P_Z_TV_SOLVE, B->gz, a);
         QSM_HIP(hipEventRecord(ev[3], st));
         run_groups(P_Y_INV, B->gy, a);
         QSM_HIP(hipEventRecord(ev[4], st));

@@ -6,7 +6,7 @@
 // directions, and compares against a direct O(n^2) DFT in double precision.
 // Built with hipcc as host code; prints one line per case and exits non-zero
 // on any mismatch.
-#include "../../qsm.rs_amd/csrc/fft_fixed.hpp"
+#include "../../qsm.rs_amd/csrc/fft_col2.hpp"
 
 #include <cmath>
 #include <complex>
@@ -157,7 +157,90 @@ static int check_fixed() {
     return ok ? 0 : 1;
 }
 
+// ---- two-stage column path (fft_col2.hpp) ----
+// Column c of the tile is random; the l16_to_lr transform starts from the
+// load layout and must leave X[j + R0 r] in thread (j, c); the lr_to_l16
+// transform starts from that layout and must leave X[j + 16 r].
+template <int R0, bool INV>
+static double run_col2(bool rev) {
+    using C = Col2<R0>;
+    constexpr int N = C::N;
+    std::vector<float2> tw(N);
+    for (int q = 0; q < N; ++q) {
+        double a = -2.0 * M_PI * q / N;
+        tw[q] = make_float2((float)std::cos(a), (float)std::sin(a));
+    }
+    std::vector<std::complex<double>> in(N * 16);
+    srand(N * 131 + (rev ? 7 : 0) + (INV ? 3 : 0));
+    for (auto& z : in) z = {rand() / (double)RAND_MAX - 0.5, rand() / (double)RAND_MAX - 0.5};
+    std::vector<float2> lds(C::LDS, make_float2(NAN, NAN));
+    std::vector<std::array<float2, C::VMAX>> regs(kThreads);
+    // input layout: L16 (rev = false) or LR (rev = true)
+    for (int tid = 0; tid < kThreads; ++tid) {
+        const int c = tid & 15, j = tid >> 4;
+        for (int r = 0; r < C::VMAX; ++r) regs[tid][r] = make_float2(NAN, NAN);
+        if (!rev) {
+            for (int r = 0; r < R0; ++r) {
+                auto z = in[c * N + j + 16 * r];
+                regs[tid][r] = make_float2((float)z.real(), (float)z.imag());
+            }
+        } else if (j < R0) {
+            for (int r = 0; r < 16; ++r) {
+                auto z = in[c * N + j + R0 * r];
+                regs[tid][r] = make_float2((float)z.real(), (float)z.imag());
+            }
+        }
+    }
+    for (int tid = 0; tid < kThreads; ++tid) {
+        float2 v[C::VMAX];
+        for (int r = 0; r < C::VMAX; ++r) v[r] = regs[tid][r];
+        if (!rev) C::template a_fwd<INV>(tid, v, lds.data());
+        else C::template a_rev<INV>(tid, v, lds.data());
+        for (int r = 0; r < C::VMAX; ++r) regs[tid][r] = v[r];
+    }
+    for (int tid = 0; tid < kThreads; ++tid) {
+        float2 v[C::VMAX];
+        for (int r = 0; r < C::VMAX; ++r) v[r] = regs[tid][r];
+        if (!rev) C::template b_fwd<INV>(tid, v, lds.data(), tw.data());
+        else C::template b_rev<INV>(tid, v, lds.data(), tw.data());
+        for (int r = 0; r < C::VMAX; ++r) regs[tid][r] = v[r];
+    }
+    const double sgn = INV ? 1.0 : -1.0;
+    double maxerr = 0.0, maxref = 0.0;
+    int seen = 0;
+    for (int tid = 0; tid < kThreads; ++tid) {
+        const int c = tid & 15, j = tid >> 4;
+        const int cnt = rev ? R0 : (j < R0 ? 16 : 0);
+        for (int r = 0; r < cnt; ++r) {
+            const int k = rev ? j + 16 * r : j + R0 * r;
+            std::complex<double> acc = 0.0;
+            for (int e = 0; e < N; ++e)
+                acc += in[c * N + e] * std::polar(1.0, sgn * 2.0 * M_PI * ((long)k * e % N) / N);
+            const float2 g = regs[tid][r];
+            maxerr = std::fmax(maxerr, std::abs(acc - std::complex<double>(g.x, g.y)));
+            maxref = std::fmax(maxref, std::abs(acc));
+            ++seen;
+        }
+    }
+    if (seen != N * 16) return 1.0;  // every output exactly once
+    return maxerr / maxref;
+}
+
+template <int R0>
+static int check_col2() {
+    const double e1 = run_col2<R0, false>(false), e2 = run_col2<R0, true>(false);
+    const double e3 = run_col2<R0, false>(true), e4 = run_col2<R0, true>(true);
+    const double worst = std::fmax(std::fmax(e1, e2), std::fmax(e3, e4));
+    const bool ok = worst < 2e-6;
+    printf("col2 n=%4d l16fwd=%.2e l16inv=%.2e lrfwd=%.2e lrinv=%.2e %s\n", 16 * R0, e1, e2, e3, e4,
+           ok ? "ok" : "FAIL");
+    return ok ? 0 : 1;
+}
+
 int main() {
+    int bad_col2 = check_col2<4>() + check_col2<8>() + check_col2<9>() + check_col2<10>() +
+                   check_col2<11>() + check_col2<12>() + check_col2<13>() + check_col2<14>() +
+                   check_col2<15>() + check_col2<16>();
     int bad_fixed = check_fixed<16>() + check_fixed<64>() + check_fixed<128>() + check_fixed<144>() +
                     check_fixed<160>() + check_fixed<176>() + check_fixed<192>() +
                     check_fixed<208>() + check_fixed<224>() + check_fixed<240>() +
@@ -180,7 +263,7 @@ int main() {
                ok ? "ok" : "FAIL");
         if (!ok) ++bad;
     }
-    bad += bad_fixed;
+    bad += bad_fixed + bad_col2;
     printf("%s\n", bad ? "FAILED" : "ALL OK");
     return bad ? 1 : 0;
 }
