@@ -244,6 +244,14 @@ def main():
     copy_gbs = qsm.hbm_copy_probe(1 << 30, 5)
     roofline["copy_probe_GBs"] = round(copy_gbs, 1)
     roofline["frac_of_copy"] = round(achieved / copy_gbs, 4)
+    if dom == 0:
+        # the same access shape as the fused R2C pass: x and the three dual
+        # components read, the three dual components and the spectrum written,
+        # element by element over the batch
+        nvox = int(np.prod(dims)) * max(1, a.batch)
+        shape_gbs = qsm.hbm_stream_probe(nvox, 4, 4, 5)
+        roofline["shape_probe_GBs"] = round(shape_gbs, 1)
+        roofline["frac_of_shape"] = round(achieved / shape_gbs, 4)
     cpu = None
     if not a.no_cpu:
         threads = min(16, os.cpu_count() or 1)

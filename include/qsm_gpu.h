@@ -296,6 +296,10 @@ int qsm_device_synchronize(void);
 /* Achievable HBM rate reference: a one-element-per-thread device copy of
  * `bytes`, read + write counted, best of `reps` launches (GB/s). */
 int qsm_hbm_copy_probe(size_t bytes, int reps, double* gbps);
+/* Achievable rate of a pass's access shape: `reads` arrays of n floats read
+ * and `writes` arrays written element by element (1..4 each), all bytes
+ * counted, best of `reps` launches (GB/s).  No reference counterpart. */
+int qsm_hbm_stream_probe(size_t n, int reads, int writes, int reps, double* gbps);
 
 /* ---- differentiable unrolled TV-ADMM (no reference counterpart) ------- */
 /* K = params->max_iter iterations of tv_admm (src/inversion/tv.rs:63-97)

@@ -198,6 +198,7 @@ _SIGS = {
     "qsm_memcpy_dtod": (C.c_int, [C.c_void_p, C.c_void_p, _SZ]),
     "qsm_device_synchronize": (C.c_int, []),
     "qsm_hbm_copy_probe": (C.c_int, [_SZ, C.c_int, _D]),
+    "qsm_hbm_stream_probe": (C.c_int, [_SZ, C.c_int, C.c_int, C.c_int, _D]),
 }
 for _name, (_res, _args) in _SIGS.items():
     _fn = getattr(_lib, _name)
@@ -782,6 +783,14 @@ def hbm_copy_probe(nbytes: int = 1 << 30, reps: int = 5) -> float:
     """GB/s of a plain one-element-per-thread device copy (read + write)."""
     g = C.c_double(0.0)
     _check(_lib.qsm_hbm_copy_probe(int(nbytes), int(reps), C.byref(g)))
+    return g.value
+
+
+def hbm_stream_probe(n: int, reads: int, writes: int, reps: int = 5) -> float:
+    """GB/s of a pass shape: `reads` arrays of n floats read and `writes`
+    arrays written element by element (1..4 each), all bytes counted."""
+    g = C.c_double(0.0)
+    _check(_lib.qsm_hbm_stream_probe(int(n), int(reads), int(writes), int(reps), C.byref(g)))
     return g.value
 
 

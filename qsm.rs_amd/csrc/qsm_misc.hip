@@ -120,6 +120,32 @@ __global__ __launch_bounds__(kThreads) void k_copy_probe(const float2* __restric
     if (i < n) b[i] = a[i];
 }
 
+// R read and W write streams of n floats each, 16 floats per thread per
+// stream, every read issued before the first write: the access shape of a
+// pass that reads R and writes W arrays element by element.
+template <int R, int W>
+__global__ __launch_bounds__(kThreads) void k_stream_probe(const float* __restrict__ in, float* __restrict__ out,
+                                                           size_t n) {
+    const size_t i0 = (size_t)blockIdx.x * kThreads * 16 + threadIdx.x;
+    float a[16];
+#pragma unroll
+    for (int k = 0; k < 16; ++k) {
+        const size_t q = min(i0 + (size_t)k * kThreads, n - 1);
+        float s = 0.0f;
+#pragma unroll
+        for (int r = 0; r < R; ++r) s += in[r * n + q];
+        a[k] = s;
+    }
+#pragma unroll
+    for (int k = 0; k < 16; ++k) {
+        const size_t q = i0 + (size_t)k * kThreads;
+        if (q < n) {
+#pragma unroll
+            for (int w = 0; w < W; ++w) out[w * n + q] = a[k] + (float)w;
+        }
+    }
+}
+
 // chi = x where mask != 0, else 0 (src/utils/mask.rs:47-53), widened to f64.
 __global__ __launch_bounds__(kThreads) void k_tv_output(const VolDesc* __restrict__ vols, LaunchTable lt,
                                                         const VolStatus* __restrict__ status) {
@@ -183,6 +209,17 @@ void launch_status_clear_done(hipStream_t st, VolStatus* s, int nvol) {
 }
 void launch_copy_probe(hipStream_t st, const float2* a, float2* b, size_t n) {
     k_copy_probe<<<dim3((unsigned)((n + kThreads - 1) / kThreads)), dim3(kThreads), 0, st>>>(a, b, n);
+}
+int launch_stream_probe(hipStream_t st, int reads, int writes, const float* in, float* out, size_t n) {
+    const dim3 g((unsigned)((n + kThreads * 16 - 1) / (kThreads * 16))), b(kThreads);
+#define QSM_PROBE(r, w) \
+    if (reads == r && writes == w) { k_stream_probe<r, w><<<g, b, 0, st>>>(in, out, n); return 0; }
+    QSM_PROBE(1, 1) QSM_PROBE(1, 2) QSM_PROBE(1, 3) QSM_PROBE(1, 4)
+    QSM_PROBE(2, 1) QSM_PROBE(2, 2) QSM_PROBE(2, 3) QSM_PROBE(2, 4)
+    QSM_PROBE(3, 1) QSM_PROBE(3, 2) QSM_PROBE(3, 3) QSM_PROBE(3, 4)
+    QSM_PROBE(4, 1) QSM_PROBE(4, 2) QSM_PROBE(4, 3) QSM_PROBE(4, 4)
+#undef QSM_PROBE
+    return -1;
 }
 void launch_tv_output(hipStream_t st, const VolDesc* d, const LaunchTable& lt, const VolStatus* status) {
     k_tv_output<<<dim3(lt.start[lt.nvol]), dim3(kThreads), 0, st>>>(d, lt, status);

@@ -25,6 +25,7 @@ void launch_nltv_wmax(hipStream_t st, const VolDesc* d, int nvol, int nblk, int 
 void launch_nltv_init(hipStream_t st, const VolDesc* d, const LaunchTable& lt);
 void launch_status_clear_done(hipStream_t st, VolStatus* s, int nvol);
 void launch_copy_probe(hipStream_t st, const float2* a, float2* b, size_t n);
+int launch_stream_probe(hipStream_t st, int reads, int writes, const float* in, float* out, size_t n);
 bool fixed_length(int n);
 
 // ------------------------------------------------------------ error state --
@@ -1550,6 +1551,33 @@ int qsm_run_bg_removal_cfg(const double* field_ppm, size_t field_len, const uint
     }
 }
 
+// Best rate of `reps` timed launches of `launch` moving `bytes` (GB/s),
+// after one untimed launch.
+extern "C++" {
+template <class F>
+static hipError_t best_rate(int reps, double bytes, F launch, double& best) {
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    hipError_t e = hipEventCreate(&e0);
+    if (e == hipSuccess) e = hipEventCreate(&e1);
+    if (e == hipSuccess) { launch(); e = hipGetLastError(); }
+    best = 0.0;
+    for (int r = 0; r < reps && e == hipSuccess; ++r) {
+        float ms = 0.f;
+        if ((e = hipEventRecord(e0, nullptr)) != hipSuccess) break;
+        launch();
+        if ((e = hipGetLastError()) != hipSuccess) break;
+        if ((e = hipEventRecord(e1, nullptr)) != hipSuccess) break;
+        if ((e = hipEventSynchronize(e1)) != hipSuccess) break;
+        if ((e = hipEventElapsedTime(&ms, e0, e1)) != hipSuccess) break;
+        const double g = bytes / (ms * 1e-3) / 1e9;
+        if (g > best) best = g;
+    }
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+    return e;
+}
+}  // extern "C++"
+
 // Device-to-device copy rate of a plain one-element-per-thread kernel over
 // `bytes` (read + write counted), best of `reps` timed launches.
 int qsm_hbm_copy_probe(size_t bytes, int reps, double* gbps) {
@@ -1557,34 +1585,43 @@ int qsm_hbm_copy_probe(size_t bytes, int reps, double* gbps) {
     if (!gbps || bytes < 16 || reps < 1) return fail(QSM_EPARAM, "bad probe arguments");
     const size_t n = bytes / sizeof(float2);
     float2 *a = nullptr, *b = nullptr;
-    hipEvent_t e0 = nullptr, e1 = nullptr;
-    auto done = [&](int code) {
-        if (a) (void)hipFree(a);
-        if (b) (void)hipFree(b);
-        if (e0) (void)hipEventDestroy(e0);
-        if (e1) (void)hipEventDestroy(e1);
-        return code;
-    };
     int rc;
-    if ((rc = dev_alloc(&a, n)) || (rc = dev_alloc(&b, n))) return done(rc);
-    QSM_HIP(hipEventCreate(&e0));
-    QSM_HIP(hipEventCreate(&e1));
-    launch_fill_zero(nullptr, reinterpret_cast<float*>(a), 2 * n);
-    launch_copy_probe(nullptr, a, b, n);  // warm-up
-    double best = 0.0;
-    for (int r = 0; r < reps; ++r) {
-        QSM_HIP(hipEventRecord(e0, nullptr));
-        launch_copy_probe(nullptr, a, b, n);
-        QSM_HIP(hipEventRecord(e1, nullptr));
-        QSM_HIP(hipEventSynchronize(e1));
-        float ms = 0.f;
-        QSM_HIP(hipEventElapsedTime(&ms, e0, e1));
-        const double g = 2.0 * (double)n * sizeof(float2) / (ms * 1e-3) / 1e9;
-        if (g > best) best = g;
+    if ((rc = dev_alloc(&a, n)) || (rc = dev_alloc(&b, n))) {
+        if (a) (void)hipFree(a);
+        return rc;
     }
-    QSM_CHECK_LAUNCH();
+    launch_fill_zero(nullptr, reinterpret_cast<float*>(a), 2 * n);
+    double best = 0.0;
+    const hipError_t e = best_rate(reps, 2.0 * (double)n * sizeof(float2),
+                                   [&] { launch_copy_probe(nullptr, a, b, n); }, best);
+    (void)hipFree(a);
+    (void)hipFree(b);
+    if (e != hipSuccess) return fail(QSM_EHIP, "copy probe: %s", hipGetErrorString(e));
     *gbps = best;
-    return done(QSM_OK);
+    return QSM_OK;
+}
+
+// Rate of a pass shape: `reads` arrays of n floats read and `writes` arrays
+// written element by element (1..4 each), all bytes counted.
+int qsm_hbm_stream_probe(size_t n, int reads, int writes, int reps, double* gbps) {
+    g_err.clear();
+    if (!gbps || n < 4096 || reps < 1 || reads < 1 || reads > 4 || writes < 1 || writes > 4)
+        return fail(QSM_EPARAM, "bad probe arguments");
+    float *in = nullptr, *out = nullptr;
+    int rc;
+    if ((rc = dev_alloc(&in, n * reads)) || (rc = dev_alloc(&out, n * writes))) {
+        if (in) (void)hipFree(in);
+        return rc;
+    }
+    launch_fill_zero(nullptr, in, n * reads);
+    double best = 0.0;
+    const hipError_t e = best_rate(reps, 4.0 * (double)n * (reads + writes),
+                                   [&] { launch_stream_probe(nullptr, reads, writes, in, out, n); }, best);
+    (void)hipFree(in);
+    (void)hipFree(out);
+    if (e != hipSuccess) return fail(QSM_EHIP, "stream probe: %s", hipGetErrorString(e));
+    *gbps = best;
+    return QSM_OK;
 }
 
 int qsm_device_alloc(size_t bytes, void** out) {
