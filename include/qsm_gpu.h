@@ -272,6 +272,11 @@ int qsm_rts_batch_create(const qsm_tv_volume* vols, size_t n_vols, const qsm_rts
  * in one launch per kernel.  Progress: (iter+1, max_iter) per iteration of
  * the batch, (k, k) once every volume has converged.  Blocks until done. */
 int qsm_tv_batch_run(qsm_tv_batch* batch, qsm_progress_fn progress, void* user);
+/* Iterations after the first run as replays of HIP graphs captured on the
+ * batch's stream at the first run (default), or as eager launches
+ * (enable = 0), e.g. for a caller that captures the run into a graph of its
+ * own.  Results are identical either way.  No reference counterpart. */
+int qsm_tv_batch_set_graphs(qsm_tv_batch* batch, int enable);
 /* Iterations run and last relative change of volume v after a run. */
 int qsm_tv_batch_status(const qsm_tv_batch* batch, size_t v, size_t* iters, double* relchange);
 void qsm_tv_batch_destroy(qsm_tv_batch* batch);

@@ -145,6 +145,7 @@ _SIGS = {
     "qsm_tv_batch_create": (C.c_int, [C.POINTER(_CTvVolume), _SZ, C.POINTER(_CTvParams),
                                       C.c_void_p, C.POINTER(C.c_void_p)]),
     "qsm_tv_batch_run": (C.c_int, [C.c_void_p, PROGRESS_FN, C.c_void_p]),
+    "qsm_tv_batch_set_graphs": (C.c_int, [C.c_void_p, C.c_int]),
     "qsm_tv_batch_status": (C.c_int, [C.c_void_p, _SZ, C.POINTER(_SZ), _D]),
     "qsm_tv_batch_destroy": (None, [C.c_void_p]),
     "qsm_tv_batch_profile": (C.c_int, [C.c_void_p, _SZ, _D, _SZ]),
@@ -831,6 +832,11 @@ class TvBatch:
     def run(self, progress: Optional[Callable[[int, int], None]] = None):
         cb = _progress(progress)
         _check(_lib.qsm_tv_batch_run(self._h, cb, None))
+
+    def set_graphs(self, enable: bool):
+        """Iterations after the first as HIP graph replays (default) or as
+        eager launches; the results are identical."""
+        _check(_lib.qsm_tv_batch_set_graphs(self._h, 1 if enable else 0))
 
     def status(self, v: int):
         it, rel = _SZ(0), C.c_double(0.0)

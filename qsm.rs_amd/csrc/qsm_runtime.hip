@@ -232,11 +232,19 @@ struct qsm_tv_batch {
         return a;
     }
     hipEvent_t ev[kWindow] = {};
+    // Iterations it >= 2 replay captured graphs: graph p is one iteration with
+    // (it - 1) & 1 == p (its kernel arguments depend only on the parity).
+    // graph_state: 0 not built yet, 1 built, -1 eager launches (disabled, or
+    // the stream cannot be captured).
+    hipGraphExec_t gexec[2] = {};
+    int graph_state = 0;
 };
 
 static void batch_free(qsm_tv_batch* b) {
     if (!b) return;
     if (b->stream) (void)hipStreamSynchronize(b->stream);
+    for (auto& g : b->gexec)
+        if (g) (void)hipGraphExecDestroy(g);
     for (void* p : b->allocs) (void)hipFree(p);
     if (b->h_status) (void)hipHostFree(b->h_status);
     for (auto& e : b->ev)
@@ -435,9 +443,59 @@ static int enqueue_setup(qsm_tv_batch* B) {
     return QSM_OK;
 }
 
+// Capture the two iteration graphs on the batch's stream.  A stream that
+// cannot be captured (the legacy null stream, or one already capturing)
+// leaves the batch on eager launches; nothing runs during capture.  The
+// graphs hold no event: the host's "iteration closed" event is recorded
+// after the replay instead (see launch_iteration).
+static int build_iteration_graphs(qsm_tv_batch* B) {
+    for (int s = 0; s < 2; ++s) {
+        const size_t it = (size_t)s + 3;  // (it - 1) & 1 == s, it >= 2
+        if (hipStreamBeginCapture(B->stream, hipStreamCaptureModeThreadLocal) != hipSuccess) {
+            (void)hipGetLastError();
+            B->graph_state = -1;
+            return QSM_OK;
+        }
+        const int rc = enqueue_iteration(B, it, nullptr);
+        hipGraph_t g = nullptr;
+        const hipError_t e = hipStreamEndCapture(B->stream, &g);
+        hipError_t ei = hipSuccess;
+        if (!rc && e == hipSuccess && g) ei = hipGraphInstantiate(&B->gexec[s], g, nullptr, nullptr, 0);
+        if (g) (void)hipGraphDestroy(g);
+        if (rc) return rc;
+        if (e != hipSuccess || !g || ei != hipSuccess) {
+            (void)hipGetLastError();
+            for (auto& x : B->gexec)
+                if (x) { (void)hipGraphExecDestroy(x); x = nullptr; }
+            B->graph_state = -1;
+            return QSM_OK;
+        }
+    }
+    B->graph_state = 1;
+    return QSM_OK;
+}
+
+// Iteration `it` of a run: a graph replay where available, else the eager
+// launches of enqueue_iteration (the first iteration always: its kernels
+// differ, and RTS starts differently).  `closed` marks the point after which
+// iteration it-1's convergence state is in the host mirror: eagerly right
+// after the first pass of iteration it, after a replay at its end.
+static int launch_iteration(qsm_tv_batch* B, size_t it, hipEvent_t closed) {
+    if (it >= 2 && B->graph_state == 1) {
+        QSM_HIP(hipGraphLaunch(B->gexec[(it - 1) & 1], B->stream));
+        if (closed) QSM_HIP(hipEventRecord(closed, B->stream));
+        return QSM_OK;
+    }
+    return enqueue_iteration(B, it, closed);
+}
+
 static int tv_batch_run_impl(qsm_tv_batch* B, qsm_progress_fn progress, void* user) {
     if (!B) return fail(QSM_EPARAM, "batch is null");
     hipStream_t st = B->stream;
+    if (B->graph_state == 0 && B->params.max_iter >= 2) {
+        int rc = build_iteration_graphs(B);
+        if (rc) return rc;
+    }
     std::memset(B->h_status, 0, B->nvol * sizeof(VolStatus));
     {
         int rc = enqueue_setup(B);
@@ -451,7 +509,7 @@ static int tv_batch_run_impl(qsm_tv_batch* B, qsm_progress_fn progress, void* us
     constexpr size_t W = qsm_tv_batch::kWindow;
     if (!progress && !can_converge) {
         for (size_t it = 1; it <= max_iter; ++it) {
-            int rc = enqueue_iteration(B, it, nullptr);
+            int rc = launch_iteration(B, it, nullptr);
             if (rc) return rc;
         }
         if (max_iter > 0) launch_tv_reduce(st, B->d_vols, B->nvol, B->d_status, B->h_status_dev);
@@ -465,7 +523,7 @@ static int tv_batch_run_impl(qsm_tv_batch* B, qsm_progress_fn progress, void* us
         if (max_iter > 0 && progress) progress(1, max_iter, user);
         for (size_t it = 1; it <= max_iter; ++it) {
             while (next <= max_iter && next <= it + W) {
-                int rc = enqueue_iteration(B, next, next >= 2 ? B->ev[(next - 1) % W] : nullptr);
+                int rc = launch_iteration(B, next, next >= 2 ? B->ev[(next - 1) % W] : nullptr);
                 if (rc) return rc;
                 ++next;
             }
@@ -1063,6 +1121,19 @@ int qsm_tv_batch_create(const qsm_tv_volume* vols, size_t n_vols, const qsm_tv_p
 int qsm_tv_batch_run(qsm_tv_batch* batch, qsm_progress_fn progress, void* user) {
     g_err.clear();
     return tv_batch_run_impl(batch, progress, user);
+}
+
+int qsm_tv_batch_set_graphs(qsm_tv_batch* B, int enable) {
+    g_err.clear();
+    if (!B) return fail(QSM_EPARAM, "batch is null");
+    if (!enable) {
+        for (auto& g : B->gexec)
+            if (g) { (void)hipGraphExecDestroy(g); g = nullptr; }
+        B->graph_state = -1;
+    } else if (B->graph_state == -1) {
+        B->graph_state = 0;  // captured at the next run
+    }
+    return QSM_OK;
 }
 
 int qsm_tv_batch_status(const qsm_tv_batch* B, size_t v, size_t* iters, double* rel) {

@@ -124,6 +124,42 @@ def test_tv_batch_equals_single(qsm):
     batch.close()
 
 
+def test_tv_batch_graph_replays_match_eager(qsm):
+    # Iterations 2.. replay HIP graphs captured at the first run; an eager
+    # batch over the same inputs must give bit-identical results on every
+    # replay, with other eager GPU work and synchronisations in between, on
+    # the no-callback path (tol 0) and on the progress/early-exit path.
+    rng = np.random.default_rng(21)
+    shapes = [((64, 64, 32), (1.0, 1.0, 1.0), (0, 0, 1)),
+              ((20, 18, 22), (0.9, 1.0, 1.1), (0.1, 0.2, 0.95))]
+    inputs, outs, vg, ve = [], [], [], []
+    for dims, vs, b in shapes:
+        f, m = _case(rng, dims)
+        tf, tm = qsm.DeviceArray.from_numpy(f), qsm.DeviceArray.from_numpy(m)
+        cg, ce = qsm.DeviceArray(f.size, np.float64), qsm.DeviceArray(f.size, np.float64)
+        inputs.append((tf, tm))
+        outs.append((cg, ce))
+        vg.append((qsm.Grid(dims, vs), b, tf.ptr, tm.ptr, cg.ptr))
+        ve.append((qsm.Grid(dims, vs), b, tf.ptr, tm.ptr, ce.ptr))
+    other = rng.standard_normal(16 * 16 * 16)
+    for p in (qsm.TvParams(lambda_=1e-3, rho=0.1, tol=0.0, max_iter=11),
+              qsm.TvParams(lambda_=2e-4, rho=2e-2, tol=5e-3, max_iter=40)):
+        bg, be = qsm.TvBatch(vg, p), qsm.TvBatch(ve, p)
+        be.set_graphs(False)
+        for rep in range(3):
+            pe, pg = [], []
+            be.run((lambda c, t: pe.append((c, t))) if p.tol > 0 else None)
+            qsm.dipole_forward(other, qsm.Grid((16, 16, 16)), (0, 0, 1))  # eager work in between
+            qsm.synchronize()
+            bg.run((lambda c, t: pg.append((c, t))) if p.tol > 0 else None)
+            assert pe == pg, (rep, pe[-3:], pg[-3:])
+            for v, (cg, ce) in enumerate(outs):
+                assert np.array_equal(cg.numpy(), ce.numpy()), (p.tol, rep, v)
+                assert bg.status(v) == be.status(v)
+        bg.close()
+        be.close()
+
+
 def test_tv_rejects_bad_input(qsm):
     grid = qsm.Grid((17, 16, 16), (1.0, 1.0, 1.0))  # 17 is prime: unsupported
     p = qsm.TvParams()
