@@ -283,6 +283,8 @@ def main():
             "value": round(plain_value, 3), "ms_per_step": round(plain_ms, 3),
             "per_kernel_ms": {k: round(float(v), 5) for k, v in zip(KERNELS, plain_prof)}},
         "batched_over_plain": None if plain_value is None else round(value / plain_value, 3),
+        "fft_schedule": {f"y{dims[1]}": qsm.fft_schedule("y", dims[1]),
+                         f"z{dims[2]}": qsm.fft_schedule("z", dims[2])},
         "roofline": roofline,
         "cpu_baseline": cpu,
     }
@@ -349,6 +351,9 @@ def bench_uneven(a, qsm, phantom):
                    "total_voxels": voxels, "iterations": iters_done,
                    "distinct_lengths": sorted({n for d, _ in shapes for n in d})},
         "voxel_iterations_per_s": round(voxels * iters_done / (ms_per_step * 1e-3), 1),
+        "fft_schedule": {f"{ax}{n}": qsm.fft_schedule(ax, n)
+                         for ax, k in (("y", 1), ("z", 2))
+                         for n in sorted({d[k] for d, _ in shapes})},
         "roofline": _roofline(nbytes, ms, KERNELS),
     }))
 

@@ -146,6 +146,8 @@ _SIGS = {
                                       C.c_void_p, C.POINTER(C.c_void_p)]),
     "qsm_tv_batch_run": (C.c_int, [C.c_void_p, PROGRESS_FN, C.c_void_p]),
     "qsm_tv_batch_set_graphs": (C.c_int, [C.c_void_p, C.c_int]),
+    "qsm_set_fft_schedule": (C.c_int, [C.c_int]),
+    "qsm_fft_schedule": (C.c_int, [C.c_int, C.c_int, C.POINTER(C.c_int)]),
     "qsm_tv_batch_status": (C.c_int, [C.c_void_p, _SZ, C.POINTER(_SZ), _D]),
     "qsm_tv_batch_destroy": (None, [C.c_void_p]),
     "qsm_tv_batch_profile": (C.c_int, [C.c_void_p, _SZ, _D, _SZ]),
@@ -793,6 +795,22 @@ def hbm_stream_probe(n: int, reads: int, writes: int, reps: int = 5) -> float:
     g = C.c_double(0.0)
     _check(_lib.qsm_hbm_stream_probe(int(n), int(reads), int(writes), int(reps), C.byref(g)))
     return g.value
+
+
+FFT_SCHEDULES = {0: "two-stage", 1: "staged", -1: "untuned"}
+
+
+def set_fft_schedule(mode: int):
+    """0: tuned per axis length (default); 1: always the two-stage column
+    FFT; 2: always the LDS-staged one."""
+    _check(_lib.qsm_set_fft_schedule(int(mode)))
+
+
+def fft_schedule(axis: str, n: int) -> str:
+    """Schedule used by the 'y' or 'z' pass of length n."""
+    out = C.c_int(0)
+    _check(_lib.qsm_fft_schedule({"y": 1, "z": 2}[axis], int(n), C.byref(out)))
+    return FFT_SCHEDULES[out.value]
 
 
 def synchronize():

@@ -140,7 +140,13 @@ struct PassArgs {
     int ridx, first, divide;  // V-SHARP radius index / first radius / SHARP division
                               // (TV record / adjoint passes: ridx = checkpoint index)
     double delta;             // erosion threshold
+    int sched;                // column-pass FFT schedule (y / z passes): 0 two-stage, 1 LDS-staged
 };
+
+// Column-pass FFT schedule of (pass kind, axis length): the two-stage
+// transform where the length allows it, unless the tuning table or the
+// process-wide mode says otherwise.
+int column_schedule(int kind, int n);
 
 // Launch `kind` over the volumes of `lt`, all of which share the length of
 // the pass axis `n`: a compile-time-length kernel when n is one of the

@@ -734,12 +734,15 @@ __global__ __launch_bounds__(kThreads) void k_r2c_x(const VolDesc* __restrict__ 
 // ------------------------------------------------------------- C2C y pass --
 // Block = one tile of kTile consecutive kx columns at one kz, full ny.
 // sel2 >= 0 transforms a second spectrum with a second set of blocks.
-template <int N, int EG, bool INV>
+// SCHED 0: two-stage column FFT (fft_col2.hpp) where the length allows;
+// SCHED 1: the LDS-staged Stockham stages (less LDS per block).
+template <int N, int EG, bool INV, int SCHED>
 __global__ __launch_bounds__(kThreads) void k_y_pass(const VolDesc* __restrict__ vols, LaunchTable lt,
                                                      const float2* __restrict__ tw, int sel, int sel2,
                                                      const VolStatus* __restrict__ status) {
     constexpr int EMAX = col_emax(N, EG);
-    __shared__ float2 lds[col_lds2(N, EG)];
+    constexpr bool TWO = SCHED == 0 && qsmfft::col2_ok(N);
+    __shared__ float2 lds[TWO ? col_lds2(N, EG) : col_lds(N, EG)];
     __shared__ float2 twl[tw_lds(N)];
     const int bid = xcd_remap(blockIdx.x, gridDim.x);
     int lb;
@@ -757,7 +760,7 @@ __global__ __launch_bounds__(kThreads) void k_y_pass(const VolDesc* __restrict__
     QSM_GLOBAL float2* buf = glb(V.spec[s]) + (size_t)kz * nxh * ny;
     TwStage<N> tws;
     tws.fetch(tw + V.py.tw_off, ny);
-    if constexpr (qsmfft::col2_ok(N)) {
+    if constexpr (TWO) {
         constexpr int R0 = N / 16;
         float2 v[16];
         l16_load<R0>(v, buf, nxh, kx0, nxh);
@@ -1010,12 +1013,14 @@ __device__ __forceinline__ void z_tv_col2(const VolDesc& V, float2* lds, float2*
     l16_store<R0>(v, buf, zs, kx0, nxh);
 }
 
-template <int N, int EG, int OP>
+template <int N, int EG, int OP, int SCHED>
 __global__ __launch_bounds__(kThreads) void k_z_tv(const VolDesc* __restrict__ vols, LaunchTable lt,
                                                    const float2* __restrict__ tw,
                                                    const VolStatus* __restrict__ status) {
     constexpr int EMAX = col_emax(N, EG);
-    __shared__ float2 lds[col_lds2(N, EG)];
+    constexpr bool TWO = SCHED == 0 && qsmfft::col2_ok(N) &&
+                         (OP == ZOP_TV_SOLVE || OP == ZOP_TV_SOLVE0 || OP == ZOP_TV_ADJ || OP == ZOP_RTS_SOLVE);
+    __shared__ float2 lds[TWO ? col_lds2(N, EG) : col_lds(N, EG)];
     __shared__ float2 twl[tw_lds(N)];
     const int bid = xcd_remap(blockIdx.x, gridDim.x);
     int lb;
@@ -1033,8 +1038,7 @@ __global__ __launch_bounds__(kThreads) void k_z_tv(const VolDesc* __restrict__ v
     QSM_GLOBAL float2* buf = glb(V.spec[SPEC_WORK]) + base;
     TwStage<N> tws;
     tws.fetch(tw + V.pz.tw_off, nz);
-    if constexpr (qsmfft::col2_ok(N) && (OP == ZOP_TV_SOLVE || OP == ZOP_TV_SOLVE0 || OP == ZOP_TV_ADJ ||
-                                         OP == ZOP_RTS_SOLVE)) {
+    if constexpr (TWO) {
         z_tv_col2<N, OP>(V, lds, twl, tws, ky, kx0);
         return;
     }

@@ -67,20 +67,40 @@ static void launch_rows(int kind, const LaunchTable& lt, const PassArgs& a) {
 
 template <int N, int EG>
 static void launch_cols(int kind, const LaunchTable& lt, const PassArgs& a) {
+    // lengths without the two-stage transform compile only the staged one
+    constexpr int S0 = qsmfft::col2_ok(N) ? 0 : 1;
     const dim3 g(lt.start[lt.nvol]), b(kThreads);
     hipStream_t st = a.stream;
     switch (kind) {
-        case P_Y_FWD: k_y_pass<N, EG, false><<<g, b, 0, st>>>(a.vols, lt, a.tw, a.sel, a.sel2, a.status); break;
-        case P_Y_INV: k_y_pass<N, EG, true><<<g, b, 0, st>>>(a.vols, lt, a.tw, a.sel, a.sel2, a.status); break;
-        case P_Z_TV_SETUP: k_z_tv<N, EG, ZOP_TV_SETUP><<<g, b, 0, st>>>(a.vols, lt, a.tw, a.status); break;
-        case P_Z_TV_SOLVE: k_z_tv<N, EG, ZOP_TV_SOLVE><<<g, b, 0, st>>>(a.vols, lt, a.tw, a.status); break;
-        case P_Z_TV_ADJ: k_z_tv<N, EG, ZOP_TV_ADJ><<<g, b, 0, st>>>(a.vols, lt, a.tw, nullptr); break;
-        case P_Z_TV_APPLYA: k_z_tv<N, EG, ZOP_TV_APPLYA><<<g, b, 0, st>>>(a.vols, lt, a.tw, nullptr); break;
-        case P_Z_DIPOLE: k_z_tv<N, EG, ZOP_DIPOLE><<<g, b, 0, st>>>(a.vols, lt, a.tw, nullptr); break;
-        case P_Z_FILTER: k_z_tv<N, EG, ZOP_FILTER><<<g, b, 0, st>>>(a.vols, lt, a.tw, nullptr); break;
-        case P_Z_RTS_SETUP: k_z_tv<N, EG, ZOP_RTS_SETUP><<<g, b, 0, st>>>(a.vols, lt, a.tw, nullptr); break;
-        case P_Z_TV_SOLVE0: k_z_tv<N, EG, ZOP_TV_SOLVE0><<<g, b, 0, st>>>(a.vols, lt, a.tw, a.status); break;
-        case P_Z_RTS_SOLVE: k_z_tv<N, EG, ZOP_RTS_SOLVE><<<g, b, 0, st>>>(a.vols, lt, a.tw, a.status); break;
+        case P_Y_FWD:
+            if (a.sched) k_y_pass<N, EG, false, 1><<<g, b, 0, st>>>(a.vols, lt, a.tw, a.sel, a.sel2, a.status);
+            else k_y_pass<N, EG, false, S0><<<g, b, 0, st>>>(a.vols, lt, a.tw, a.sel, a.sel2, a.status);
+            break;
+        case P_Y_INV:
+            if (a.sched) k_y_pass<N, EG, true, 1><<<g, b, 0, st>>>(a.vols, lt, a.tw, a.sel, a.sel2, a.status);
+            else k_y_pass<N, EG, true, S0><<<g, b, 0, st>>>(a.vols, lt, a.tw, a.sel, a.sel2, a.status);
+            break;
+        case P_Z_TV_SETUP: k_z_tv<N, EG, ZOP_TV_SETUP, 1><<<g, b, 0, st>>>(a.vols, lt, a.tw, a.status); break;
+        case P_Z_TV_SOLVE:
+            if (a.sched) k_z_tv<N, EG, ZOP_TV_SOLVE, 1><<<g, b, 0, st>>>(a.vols, lt, a.tw, a.status);
+            else k_z_tv<N, EG, ZOP_TV_SOLVE, S0><<<g, b, 0, st>>>(a.vols, lt, a.tw, a.status);
+            break;
+        case P_Z_TV_ADJ:
+            if (a.sched) k_z_tv<N, EG, ZOP_TV_ADJ, 1><<<g, b, 0, st>>>(a.vols, lt, a.tw, nullptr);
+            else k_z_tv<N, EG, ZOP_TV_ADJ, S0><<<g, b, 0, st>>>(a.vols, lt, a.tw, nullptr);
+            break;
+        case P_Z_TV_APPLYA: k_z_tv<N, EG, ZOP_TV_APPLYA, 1><<<g, b, 0, st>>>(a.vols, lt, a.tw, nullptr); break;
+        case P_Z_DIPOLE: k_z_tv<N, EG, ZOP_DIPOLE, 1><<<g, b, 0, st>>>(a.vols, lt, a.tw, nullptr); break;
+        case P_Z_FILTER: k_z_tv<N, EG, ZOP_FILTER, 1><<<g, b, 0, st>>>(a.vols, lt, a.tw, nullptr); break;
+        case P_Z_RTS_SETUP: k_z_tv<N, EG, ZOP_RTS_SETUP, 1><<<g, b, 0, st>>>(a.vols, lt, a.tw, nullptr); break;
+        case P_Z_TV_SOLVE0:
+            if (a.sched) k_z_tv<N, EG, ZOP_TV_SOLVE0, 1><<<g, b, 0, st>>>(a.vols, lt, a.tw, a.status);
+            else k_z_tv<N, EG, ZOP_TV_SOLVE0, S0><<<g, b, 0, st>>>(a.vols, lt, a.tw, a.status);
+            break;
+        case P_Z_RTS_SOLVE:
+            if (a.sched) k_z_tv<N, EG, ZOP_RTS_SOLVE, 1><<<g, b, 0, st>>>(a.vols, lt, a.tw, a.status);
+            else k_z_tv<N, EG, ZOP_RTS_SOLVE, S0><<<g, b, 0, st>>>(a.vols, lt, a.tw, a.status);
+            break;
         case P_Z_VS_FWD: k_z_vs<N, EG, ZOP_VS_FWD><<<g, b, 0, st>>>(a.vols, lt, a.tw, a.sel, 0, 0); break;
         case P_Z_VS_RADIUS: k_z_vs<N, EG, ZOP_VS_RADIUS><<<g, b, 0, st>>>(a.vols, lt, a.tw, 0, a.ridx, a.first); break;
         case P_Z_VS_DECONV: k_z_vs<N, EG, ZOP_VS_DECONV><<<g, b, 0, st>>>(a.vols, lt, a.tw, 0, 0, a.divide); break;

@@ -2,6 +2,7 @@
 // iteration loop with the reference's progress semantics, and the extern "C"
 // entry points declared in include/qsm_gpu.h.
 #include "qsm_internal.hpp"
+#include "fft_col2.hpp"
 #include "../../include/qsm_gpu.h"
 
 #include <algorithm>
@@ -9,7 +10,9 @@
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
+#include <atomic>
 #include <map>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -183,8 +186,43 @@ Groups make_groups(const std::vector<int>& len, const std::vector<int>& blocks, 
     return gs;
 }
 
+// ------------------------------------------------- column-pass schedules --
+// Per (pass, axis length): -1 not tuned yet, 0 two-stage, 1 LDS-staged.
+// Pass 0: the y passes; pass 1: the per-iteration z passes.  Mode: 0 tuned
+// (two-stage until a length is tuned), 1 always two-stage, 2 always staged.
+namespace {
+constexpr int kMaxAxis = 512;
+std::mutex g_sched_mu;
+int8_t g_sched_tab[2][kMaxAxis + 1];
+std::atomic<int> g_sched_mode{0};
+struct SchedInit {
+    SchedInit() { std::memset(g_sched_tab, -1, sizeof g_sched_tab); }
+} g_sched_init;
+
+int sched_pass(int kind) {
+    if (kind == P_Y_FWD || kind == P_Y_INV) return 0;
+    if (kind == P_Z_TV_SOLVE || kind == P_Z_TV_SOLVE0 || kind == P_Z_TV_ADJ || kind == P_Z_RTS_SOLVE) return 1;
+    return -1;
+}
+}  // namespace
+
+int column_schedule(int kind, int n) {
+    const int pass = sched_pass(kind);
+    if (pass < 0 || n < 0 || n > kMaxAxis || !qsmfft::col2_ok(n)) return 1;
+    const int mode = g_sched_mode.load();
+    if (mode == 1) return 0;
+    if (mode == 2) return 1;
+    std::lock_guard<std::mutex> lk(g_sched_mu);
+    const int t = g_sched_tab[pass][n];
+    return t < 0 ? 0 : t;
+}
+
 void run_groups(int kind, const Groups& gs, const PassArgs& a) {
-    for (const auto& g : gs) launch_pass(kind, g.n, g.emax, g.lt, a);
+    PassArgs b = a;
+    for (const auto& g : gs) {
+        b.sched = column_schedule(kind, g.n);
+        launch_pass(kind, g.n, g.emax, g.lt, b);
+    }
 }
 
 LaunchTable identity_table(const std::vector<int>& blocks) {
@@ -251,6 +289,57 @@ static void batch_free(qsm_tv_batch* b) {
         if (e) (void)hipEventDestroy(e);
     if (b->own_stream && b->stream) (void)hipStreamDestroy(b->stream);
     delete b;
+}
+
+// The autotuner: for every y and z length of a new batch that has no entry
+// yet, time the forward y pass and the z solve pass under both schedules on
+// the batch's own launch tables and buffers (before any run, so only scratch
+// is overwritten; convergence flags are not consulted) and keep the faster
+// one for the rest of the process.  One warm-up and the best of three timed
+// launches each.
+static int tune_column_schedules(qsm_tv_batch* B) {
+    if (g_sched_mode.load() != 0) return QSM_OK;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    hipError_t e = hipEventCreate(&e0);
+    if (e == hipSuccess) e = hipEventCreate(&e1);
+    auto best_ms = [&](int kind, const PassGroup& g, int sched, float& best) {
+        PassArgs a = B->args(nullptr);
+        a.sched = sched;
+        launch_pass(kind, g.n, g.emax, g.lt, a);
+        best = 1e30f;
+        for (int r = 0; r < 3 && e == hipSuccess; ++r) {
+            float ms = 0.f;
+            if ((e = hipEventRecord(e0, B->stream)) != hipSuccess) break;
+            launch_pass(kind, g.n, g.emax, g.lt, a);
+            if ((e = hipGetLastError()) != hipSuccess) break;
+            if ((e = hipEventRecord(e1, B->stream)) != hipSuccess) break;
+            if ((e = hipEventSynchronize(e1)) != hipSuccess) break;
+            if ((e = hipEventElapsedTime(&ms, e0, e1)) != hipSuccess) break;
+            best = std::min(best, ms);
+        }
+    };
+    const struct { const Groups* gs; int kind; int pass; } work[2] = {
+        {&B->gy, P_Y_FWD, 0}, {&B->gz, P_Z_TV_SOLVE, 1}};
+    for (const auto& w : work) {
+        for (const auto& g : *w.gs) {
+            if (e != hipSuccess) break;
+            if (g.n > kMaxAxis || !qsmfft::col2_ok(g.n)) continue;
+            {
+                std::lock_guard<std::mutex> lk(g_sched_mu);
+                if (g_sched_tab[w.pass][g.n] >= 0) continue;
+            }
+            float t0 = 0.f, t1 = 0.f;
+            best_ms(w.kind, g, 0, t0);
+            best_ms(w.kind, g, 1, t1);
+            if (e != hipSuccess) break;
+            std::lock_guard<std::mutex> lk(g_sched_mu);
+            g_sched_tab[w.pass][g.n] = (int8_t)(t1 < t0 ? 1 : 0);
+        }
+    }
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+    if (e != hipSuccess) return fail(QSM_EHIP, "schedule tuning: %s", hipGetErrorString(e));
+    return QSM_OK;
 }
 
 // rts != null selects RTS: its rho, tol, max_iter replace params' (params
@@ -369,6 +458,10 @@ static int tv_batch_create_impl(const qsm_tv_volume* vols, size_t n_vols, const 
     if (e1 != hipSuccess || e2 != hipSuccess)
         return cleanup(fail(QSM_EHIP, "hipMemcpy of plan tables: %s",
                             hipGetErrorString(e1 != hipSuccess ? e1 : e2)));
+    {
+        int rc = tune_column_schedules(B);
+        if (rc) return cleanup(rc);
+    }
     *out = B;
     return QSM_OK;
 }
@@ -1121,6 +1214,30 @@ int qsm_tv_batch_create(const qsm_tv_volume* vols, size_t n_vols, const qsm_tv_p
 int qsm_tv_batch_run(qsm_tv_batch* batch, qsm_progress_fn progress, void* user) {
     g_err.clear();
     return tv_batch_run_impl(batch, progress, user);
+}
+
+int qsm_set_fft_schedule(int mode) {
+    g_err.clear();
+    if (mode < 0 || mode > 2) return fail(QSM_EPARAM, "schedule mode must be 0, 1 or 2");
+    g_sched_mode.store(mode);
+    return QSM_OK;
+}
+
+int qsm_fft_schedule(int axis, int n, int* sched) {
+    g_err.clear();
+    if (!sched || (axis != 1 && axis != 2)) return fail(QSM_EPARAM, "axis must be 1 (y) or 2 (z)");
+    if (n <= 0 || n > kMaxAxis || !qsmfft::col2_ok(n)) {
+        *sched = 1;  // only the staged transform exists for this length
+        return QSM_OK;
+    }
+    const int mode = g_sched_mode.load();
+    if (mode) {
+        *sched = mode - 1;
+        return QSM_OK;
+    }
+    std::lock_guard<std::mutex> lk(g_sched_mu);
+    *sched = g_sched_tab[axis - 1][n];
+    return QSM_OK;
 }
 
 int qsm_tv_batch_set_graphs(qsm_tv_batch* B, int enable) {

@@ -160,6 +160,39 @@ def test_tv_batch_graph_replays_match_eager(qsm):
         be.close()
 
 
+def test_fft_schedules_agree_with_oracle(qsm, oracle):
+    # The y and z lengths 144 and 160 have both column-FFT schedules (the
+    # two-stage transform, radix 9 / 10 then 16, and the LDS-staged one):
+    # forced either way, TV-ADMM matches the f64 oracle, and the default
+    # (tuned) mode records a choice for both lengths once a batch used them.
+    rng = np.random.default_rng(33)
+    dims, vs, b = (24, 144, 160), (1.0, 0.9, 1.1), (0.1, 0.1, 1.0)
+    f, m = _case(rng, dims)
+    grid = qsm.Grid(dims, vs)
+    p = qsm.TvParams(lambda_=1e-3, rho=0.1, tol=0.0, max_iter=8)
+    xo = oracle.tv_admm(f, m, dims, vs, b, 1e-3, 0.1, 0.0, 8)
+    got = {}
+    try:
+        for mode in (1, 2):
+            qsm.set_fft_schedule(mode)
+            assert qsm.fft_schedule("y", 144) == ("two-stage" if mode == 1 else "staged")
+            got[mode] = qsm.tv_admm(f, m, grid, b, p)
+            e = nrmse_dc(got[mode], xo, m)
+            print("schedule", mode, "nrmse(dc removed)", e)
+            assert e < 2e-5, (mode, e)
+        assert nrmse_dc(got[1], got[2], m) < 2e-6
+        qsm.set_fft_schedule(0)
+        x0 = qsm.tv_admm(f, m, grid, b, p)
+        assert qsm.fft_schedule("y", 144) in ("two-stage", "staged")
+        assert qsm.fft_schedule("z", 160) in ("two-stage", "staged")
+        assert qsm.fft_schedule("y", 384) == "staged"  # no two-stage form
+        assert nrmse_dc(x0, xo, m) < 2e-5
+    finally:
+        qsm.set_fft_schedule(0)
+    with pytest.raises(qsm.QsmError):
+        qsm.set_fft_schedule(3)
+
+
 def test_tv_rejects_bad_input(qsm):
     grid = qsm.Grid((17, 16, 16), (1.0, 1.0, 1.0))  # 17 is prime: unsupported
     p = qsm.TvParams()
