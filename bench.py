@@ -105,8 +105,12 @@ def vsharp_alg_bytes(dims, R):
     return (3 * R + 4) * U + R * (8 * Nh + 18 * N) + 4 * Nh + 9 * N
 
 
-def cpu_baseline(ph, params, iters_sample, threads):
-    """Oracle TV-ADMM on a bounded number of iterations, scaled to one solve."""
+def cpu_baseline(ph, params, iters_sample, threads, qsm=None, grid=None):
+    """Oracle TV-ADMM on a bounded number of iterations, scaled to one solve.
+    With qsm and grid, the same iterations are also run on the GPU and the
+    two results compared (NRMSE of tests/common/mod.rs:34-51, raw and with
+    the mean difference inside the mask removed, and max-abs error): the
+    oracle is the checker here, the GPU result is what is checked."""
     os.environ["OMP_NUM_THREADS"] = str(threads)
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle as O
@@ -115,13 +119,32 @@ def cpu_baseline(ph, params, iters_sample, threads):
     O.prepare_admm_spectral(ph.local_field, ph.dims, ph.voxel_size, ph.bdir, params.rho)
     t_setup = time.perf_counter() - t0
     t0 = time.perf_counter()
-    O.tv_admm(ph.local_field, ph.mask, ph.dims, ph.voxel_size, ph.bdir, params.lambda_,
-              params.rho, 0.0, iters_sample)
+    x_ref = O.tv_admm(ph.local_field, ph.mask, ph.dims, ph.voxel_size, ph.bdir, params.lambda_,
+                      params.rho, 0.0, iters_sample)
     t_run = time.perf_counter() - t0
     # tv_admm(iters_sample) = setup + iters_sample iterations
     t_iter = max(t_run - t_setup, 1e-9) / iters_sample
     t_solve = t_setup + params.max_iter * t_iter
+    parity = None
+    if qsm is not None:
+        p = qsm.TvParams(lambda_=params.lambda_, rho=params.rho, tol=0.0, max_iter=iters_sample)
+        x = qsm.tv_admm(ph.local_field, ph.mask, grid, ph.bdir, p)
+        m = np.asarray(ph.mask).ravel() > 0
+        d = (np.asarray(x).ravel() - np.asarray(x_ref).ravel())[m]
+        xr = np.asarray(x_ref).ravel()[m]
+        rng = float(xr.max() - xr.min())
+        parity = {
+            "iterations": iters_sample,
+            "nrmse": float(O.nrmse(x, x_ref, ph.mask)),
+            "nrmse_mean_removed": float(np.sqrt(np.mean((d - d.mean()) ** 2)) / rng),
+            "max_abs": float(np.max(np.abs(d))),
+            "max_abs_mean_removed": float(np.max(np.abs(d - d.mean()))),
+            "bar": 1e-3,
+            "note": ("GPU FP32 vs the f64 oracle on this workload's phantom, same iterations; "
+                     "the mean difference is the reference's DC term for anisotropic voxels"),
+        }
     return {
+        "parity": parity,
         "value": 1.0 / t_solve,
         "unit": "solves/s",
         "cores": int(O.num_threads()),
@@ -142,7 +165,7 @@ def main():
                     help="volumes per launch on the batched path (value); 1 = plain only")
     ap.add_argument("--iters", type=int, default=200)
     ap.add_argument("--profile-iters", type=int, default=20)
-    ap.add_argument("--cpu-iters", type=int, default=4)
+    ap.add_argument("--cpu-iters", type=int, default=10)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--skip-plain", action="store_true",
                     help="time only the batched path (so a profiler sees batched launches only)")
@@ -255,8 +278,9 @@ def main():
     cpu = None
     if not a.no_cpu:
         threads = min(16, os.cpu_count() or 1)
-        cpu = cpu_baseline(ph, params, a.cpu_iters, threads)
+        cpu = cpu_baseline(ph, params, a.cpu_iters, threads, qsm, grid)
         cpu["ratio_gpu_over_cpu"] = round(value / cpu["value"], 1)
+    parity = cpu.pop("parity") if cpu else None
 
     out = {
         "metric": METRIC,
@@ -287,6 +311,7 @@ def main():
                          f"z{dims[2]}": qsm.fft_schedule("z", dims[2])},
         "roofline": roofline,
         "cpu_baseline": cpu,
+        "parity": parity,
     }
     print(json.dumps(out))
 
