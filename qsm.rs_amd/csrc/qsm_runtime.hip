@@ -276,6 +276,14 @@ struct qsm_tv_batch {
     // the stream cannot be captured).
     hipGraphExec_t gexec[2] = {};
     int graph_state = 0;
+    // Passes with several launch groups (volumes of several axis lengths)
+    // fork them over the batch's stream and up to kSide side streams and
+    // join before the next pass, so the groups' last waves overlap.  Created
+    // only for batches with more than one group in some pass.
+    static constexpr int kSide = 3;
+    hipStream_t side[kSide] = {};
+    hipEvent_t ev_fork[5] = {};
+    hipEvent_t ev_join[5][kSide] = {};
 };
 
 static void batch_free(qsm_tv_batch* b) {
@@ -283,7 +291,16 @@ static void batch_free(qsm_tv_batch* b) {
     if (b->stream) (void)hipStreamSynchronize(b->stream);
     for (auto& g : b->gexec)
         if (g) (void)hipGraphExecDestroy(g);
+    for (auto& st : b->side)
+        if (st) (void)hipStreamSynchronize(st);
     for (void* p : b->allocs) (void)hipFree(p);
+    for (auto& e : b->ev_fork)
+        if (e) (void)hipEventDestroy(e);
+    for (auto& row : b->ev_join)
+        for (auto& e : row)
+            if (e) (void)hipEventDestroy(e);
+    for (auto& st : b->side)
+        if (st) (void)hipStreamDestroy(st);
     if (b->h_status) (void)hipHostFree(b->h_status);
     for (auto& e : b->ev)
         if (e) (void)hipEventDestroy(e);
@@ -462,6 +479,22 @@ static int tv_batch_create_impl(const qsm_tv_volume* vols, size_t n_vols, const 
         int rc = tune_column_schedules(B);
         if (rc) return cleanup(rc);
     }
+    if (B->gx.size() > 1 || B->gy.size() > 1 || B->gz.size() > 1 || B->gxc.size() > 1) {
+        for (auto& st : B->side) {
+            hipError_t e = hipStreamCreateWithFlags(&st, hipStreamNonBlocking);
+            if (e != hipSuccess) { st = nullptr; return cleanup(fail(QSM_EHIP, "hipStreamCreate: %s", hipGetErrorString(e))); }
+        }
+        auto mk = [&](hipEvent_t& ev) {
+            hipError_t e = hipEventCreateWithFlags(&ev, hipEventDisableTiming);
+            if (e != hipSuccess) ev = nullptr;
+            return e;
+        };
+        for (auto& ev : B->ev_fork)
+            if (mk(ev) != hipSuccess) return cleanup(fail(QSM_EHIP, "hipEventCreate failed"));
+        for (auto& row : B->ev_join)
+            for (auto& ev : row)
+                if (mk(ev) != hipSuccess) return cleanup(fail(QSM_EHIP, "hipEventCreate failed"));
+    }
     *out = B;
     return QSM_OK;
 }
@@ -469,6 +502,31 @@ static int tv_batch_create_impl(const qsm_tv_volume* vols, size_t n_vols, const 
 // One TV-ADMM iteration (src/inversion/admm.rs:76-145) for every volume.
 // Its first pass also closes iteration it-1 (relative change, convergence);
 // `closed` (if set) is recorded right after that pass.
+// One pass of an iteration over all groups: sequential on the batch's
+// stream, or forked over the side streams (group i on lane i mod (S+1), lane
+// 0 the batch's stream) and joined back.  slot: the pass's event set.
+static int run_pass(qsm_tv_batch* B, int slot, int kind, const Groups& gs, const PassArgs& a) {
+    if (gs.size() < 2 || !B->side[0]) {
+        run_groups(kind, gs, a);
+        return QSM_OK;
+    }
+    const int S = std::min<int>(qsm_tv_batch::kSide, (int)gs.size() - 1);
+    QSM_HIP(hipEventRecord(B->ev_fork[slot], B->stream));
+    for (int s = 0; s < S; ++s) QSM_HIP(hipStreamWaitEvent(B->side[s], B->ev_fork[slot], 0));
+    PassArgs b = a;
+    for (size_t i = 0; i < gs.size(); ++i) {
+        const int lane = (int)(i % (size_t)(S + 1));
+        b.stream = lane == 0 ? B->stream : B->side[lane - 1];
+        b.sched = column_schedule(kind, gs[i].n);
+        launch_pass(kind, gs[i].n, gs[i].emax, gs[i].lt, b);
+    }
+    for (int s = 0; s < S; ++s) {
+        QSM_HIP(hipEventRecord(B->ev_join[slot][s], B->side[s]));
+        QSM_HIP(hipStreamWaitEvent(B->stream, B->ev_join[slot][s], 0));
+    }
+    return QSM_OK;
+}
+
 // rec >= 0 also records the shrink pattern of iteration it-1 as checkpoint
 // `rec` (unrolled adjoint).
 static int enqueue_iteration(qsm_tv_batch* B, size_t it, hipEvent_t closed, int rec = -1) {
@@ -488,14 +546,15 @@ static int enqueue_iteration(qsm_tv_batch* B, size_t it, hipEvent_t closed, int 
         a.mirror = B->h_status_dev;
     }
     a.ridx = rec;
-    run_groups(rec >= 0 ? P_R2C_TVDIV_REC : P_R2C_TVDIV, B->gx, a);
+    int rc;
+    if ((rc = run_pass(B, 0, rec >= 0 ? P_R2C_TVDIV_REC : P_R2C_TVDIV, B->gx, a))) return rc;
     if (closed) QSM_HIP(hipEventRecord(closed, B->stream));
     a.status_rw = nullptr;
     a.mirror = nullptr;
-    run_groups(P_Y_FWD, B->gy, a);
-    run_groups(B->method == 1 ? P_Z_RTS_SOLVE : P_Z_TV_SOLVE, B->gz, a);
-    run_groups(P_Y_INV, B->gy, a);
-    run_groups(P_C2R_TV, B->gxc, a);
+    if ((rc = run_pass(B, 1, P_Y_FWD, B->gy, a))) return rc;
+    if ((rc = run_pass(B, 2, B->method == 1 ? P_Z_RTS_SOLVE : P_Z_TV_SOLVE, B->gz, a))) return rc;
+    if ((rc = run_pass(B, 3, P_Y_INV, B->gy, a))) return rc;
+    if ((rc = run_pass(B, 4, P_C2R_TV, B->gxc, a))) return rc;
     QSM_CHECK_LAUNCH();
     return QSM_OK;
 }
