@@ -53,7 +53,7 @@ struct VolDesc {
     float* x[2];                 // ping-pong solution
     float* u[2];                 // ping-pong dual variable (3N each)
     float2* spec[SPEC_COUNT];    // half spectra
-    float* q;                    // rho * inv_a / N (half spectrum, real)
+    float* q;                    // rho * inv_a / N (half spectrum, real; read by RTS, TV forms it in-kernel)
     double* partial;             // per-block (sum d^2, sum x^2) of the C2R pass
     int npartial;
     // inputs / outputs

@@ -938,13 +938,14 @@ struct TvQ {
 // with both multipliers averaged over k and -k: the reference keeps
 // Re(IFFT(.)) of a full spectrum, which equals the inverse of its Hermitian
 // part, and on Nyquist planes D(k) != D(-k) when B0 is tilted.
-// SOLVE: forward z, x^ = f^ - rho * FFT(div) * inv_a (src/inversion/admm.rs:
-// 106-108, scaled by 1/N), inverse z.
-// ADJ: forward z, -q * spectrum, inverse z.  APPLYA / DIPOLE: forward z,
-// times the Hermitian part of D inv_a / N or of D / N (in FP64), inverse z.
-// The per-iteration ops (SOLVE, SOLVE0, ADJ) at the lengths of fft_col2.hpp:
-// forward z from the load layout, the product in registers, inverse z back
-// to the load layout; two LDS exchanges in all.
+// SOLVE: forward z, x^ = f^ - q * FFT(div) (src/inversion/admm.rs:106-108,
+// scaled by 1/N), inverse z, with q formed in-kernel (TvQ); RTS_SOLVE reads
+// the stored q instead.  ADJ: forward z, -q * spectrum, inverse z (q from
+// TvQ).  APPLYA / DIPOLE: forward z, times the Hermitian part of D inv_a / N
+// or of D / N (in FP64), inverse z.
+// The per-iteration ops (SOLVE, SOLVE0, ADJ, RTS_SOLVE) at the lengths of
+// fft_col2.hpp: forward z from the load layout, the product in registers,
+// inverse z back to the load layout; two LDS exchanges in all.
 template <int N, int OP>
 __device__ __forceinline__ void z_tv_col2(const VolDesc& V, float2* lds, float2* twl, const TwStage<N>& tws,
                                           int ky, int kx0) {

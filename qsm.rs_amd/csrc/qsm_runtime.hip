@@ -316,6 +316,14 @@ static void batch_free(qsm_tv_batch* b) {
 // launches each.
 static int tune_column_schedules(qsm_tv_batch* B) {
     if (g_sched_mode.load() != 0) return QSM_OK;
+    {
+        // a stream under capture cannot be timed: keep the defaults
+        hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+        if (hipStreamIsCapturing(B->stream, &cs) != hipSuccess || cs != hipStreamCaptureStatusNone) {
+            (void)hipGetLastError();
+            return QSM_OK;
+        }
+    }
     hipEvent_t e0 = nullptr, e1 = nullptr;
     hipError_t e = hipEventCreate(&e0);
     if (e == hipSuccess) e = hipEventCreate(&e1);
