@@ -307,6 +307,7 @@ def main():
             "value": round(plain_value, 3), "ms_per_step": round(plain_ms, 3),
             "per_kernel_ms": {k: round(float(v), 5) for k, v in zip(KERNELS, plain_prof)}},
         "batched_over_plain": None if plain_value is None else round(value / plain_value, 3),
+        "voxel_iterations_per_s": round(value * int(np.prod(dims)) * iters_done, 1),
         "fft_schedule": {f"y{dims[1]}": qsm.fft_schedule("y", dims[1]),
                          f"z{dims[2]}": qsm.fft_schedule("z", dims[2])},
         "roofline": roofline,
