@@ -141,7 +141,11 @@ struct PassArgs {
                               // (TV record / adjoint passes: ridx = checkpoint index)
     double delta;             // erosion threshold
     int sched;                // column-pass FFT schedule (y / z passes): 0 two-stage, 1 LDS-staged
+    int xshift;               // x passes: 0 or 2 (fixed_rows >> xshift pair-rows per block)
 };
+
+// Axis lengths with compiled-length kernels.
+bool has_fixed_kernels(int n);
 
 // Column-pass FFT schedule of (pass kind, axis length): the two-stage
 // transform where the length allows it, unless the tuning table or the

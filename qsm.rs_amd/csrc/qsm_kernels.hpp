@@ -654,13 +654,17 @@ __device__ __forceinline__ void fill_tvadj(float2* lds, const VolDesc& V, int nx
 // aux: TV iteration parity for SRC_TVDIV, radius index for SRC_SPHERE.
 // ck: checkpoint index (SRC_TVDIV_REC: the record written; SRC_TVADJ: the
 // record read, and the row of adj_tpart written).
-template <int N, int EG, int SRC>
+// RBS: the block takes fixed_rows(N) >> RBS pair-rows (small volumes: more,
+// smaller blocks); the host sizes rows_x and the launch the same way.
+template <int N, int EG, int SRC, int RBS = 0>
 __global__ __launch_bounds__(kThreads) void k_r2c_x(const VolDesc* __restrict__ vols, LaunchTable lt,
                                                     const float2* __restrict__ tw, int out_sel, int aux,
                                                     const VolStatus* status, VolStatus* status_rw,
                                                     VolStatus* mirror, int ck) {
-    constexpr int EMAX = x_emax(N);
-    __shared__ float2 lds[x_lds(N)];
+    constexpr int RBX = N > 0 ? (qsmfft::fixed_rows(N) >> RBS) : 0;
+    static_assert(N == 0 || RBX >= 1, "row shift too large for this length");
+    constexpr int EMAX = N > 0 ? (RBX * N + kThreads - 1) / kThreads : 16;
+    __shared__ float2 lds[N > 0 ? RBX * N : x_lds(0)];
     __shared__ float2 twl[tw_lds(N)];
     const int bid = xcd_remap(blockIdx.x, gridDim.x);
     int lb;
@@ -678,7 +682,7 @@ __global__ __launch_bounds__(kThreads) void k_r2c_x(const VolDesc* __restrict__ 
     }
     const int nx = N > 0 ? N : V.nx;
     const int nxh = nx / 2 + 1;
-    const int RB = N > 0 ? qsmfft::fixed_rows(N) : V.rows_x;
+    const int RB = N > 0 ? RBX : V.rows_x;
     const int nrows = V.ny * V.nz;
     const int row0 = lb * RB * 2;
     TwStage<N> tws;
@@ -711,7 +715,7 @@ __global__ __launch_bounds__(kThreads) void k_r2c_x(const VolDesc* __restrict__ 
     }
     tws.commit(twl, nx);
     __syncthreads();
-    run_fft<N, EMAX, false, 0, false>(lds, V.px, RB, 1, nx, twl);
+    run_fft<N, EMAX, false, 0, false, RBX>(lds, V.px, RB, 1, nx, twl);
     QSM_GLOBAL float2* __restrict__ out = glb(V.spec[out_sel]);
     const int total = RB * nxh;
 #pragma unroll
@@ -1196,12 +1200,13 @@ __device__ __forceinline__ void c2r_load(float2* lds, int nx, int nxh, int RB,
 // C2R_X0: FP32 into x[0], 0 where mask == 0 (the RTS starting point).
 enum C2RMode : int { C2R_TV = 0, C2R_ADJ = 1, C2R_F64 = 2, C2R_F64_MASKED = 3, C2R_X0 = 4 };
 
-template <int N, int EG, int MODE>
+template <int N, int EG, int MODE, int RBS = 0>
 __global__ __launch_bounds__(kThreads) void k_c2r_tv(const VolDesc* __restrict__ vols, LaunchTable lt,
                                                      const float2* __restrict__ tw, int parity,
                                                      const VolStatus* __restrict__ status) {
-    constexpr int EMAX = xc_emax(N);
-    constexpr int RBF = N > 0 ? qsmfft::fixed_rows_c(N) : 0;
+    // half the R2C pass's pair-rows (after its row shift RBS), at least 1
+    constexpr int RBF = N > 0 ? ((qsmfft::fixed_rows(N) >> RBS) / 2 > 0 ? (qsmfft::fixed_rows(N) >> RBS) / 2 : 1) : 0;
+    constexpr int EMAX = N > 0 ? (RBF * N + kThreads - 1) / kThreads : 16;
     __shared__ float2 lds[N > 0 ? RBF * N : kThreads * 16];
     __shared__ float2 twl[tw_lds(N)];
     __shared__ double sh[kThreads / 64];

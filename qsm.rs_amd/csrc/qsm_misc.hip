@@ -249,6 +249,15 @@ bool fixed_length(int n) {
     return false;
 }
 
+bool has_fixed_kernels(int n) {
+    switch (n) {
+#define QSM_FIXED_CASE(m) case m: return true;
+        QSM_FIXED_LENGTHS(QSM_FIXED_CASE)
+#undef QSM_FIXED_CASE
+        default: return false;
+    }
+}
+
 void launch_pass(int kind, int n, int emax, const LaunchTable& lt, const PassArgs& a) {
     if (lt.nvol == 0 || lt.start[lt.nvol] == 0) return;
     switch (n) {

@@ -18,51 +18,64 @@ static bool is_row_kind(int kind) {
            kind == P_R2C_NLTV_BND;
 }
 
-template <int N, int EG>
-static void launch_rows(int kind, const LaunchTable& lt, const PassArgs& a) {
+template <int N, int EG, int RBS>
+static void launch_rows_s(int kind, const LaunchTable& lt, const PassArgs& a) {
     const dim3 g(lt.start[lt.nvol]), b(kThreads);
     hipStream_t st = a.stream;
     switch (kind) {
         case P_R2C_TVDIV:
-            k_r2c_x<N, EG, SRC_TVDIV><<<g, b, 0, st>>>(a.vols, lt, a.tw, a.sel, a.parity, a.status,
+            k_r2c_x<N, EG, SRC_TVDIV, RBS><<<g, b, 0, st>>>(a.vols, lt, a.tw, a.sel, a.parity, a.status,
                                                        a.status_rw, a.mirror, 0);
             break;
         case P_R2C_TVDIV_REC:
-            k_r2c_x<N, EG, SRC_TVDIV_REC><<<g, b, 0, st>>>(a.vols, lt, a.tw, a.sel, a.parity, a.status,
+            k_r2c_x<N, EG, SRC_TVDIV_REC, RBS><<<g, b, 0, st>>>(a.vols, lt, a.tw, a.sel, a.parity, a.status,
                                                            a.status_rw, a.mirror, a.ridx);
             break;
         case P_R2C_TVADJ_FIRST:
-            k_r2c_x<N, EG, SRC_TVADJ_FIRST><<<g, b, 0, st>>>(a.vols, lt, a.tw, a.sel, a.parity, nullptr,
+            k_r2c_x<N, EG, SRC_TVADJ_FIRST, RBS><<<g, b, 0, st>>>(a.vols, lt, a.tw, a.sel, a.parity, nullptr,
                                                              nullptr, nullptr, a.ridx);
             break;
         case P_R2C_TVADJ:
-            k_r2c_x<N, EG, SRC_TVADJ><<<g, b, 0, st>>>(a.vols, lt, a.tw, a.sel, a.parity, nullptr, nullptr,
+            k_r2c_x<N, EG, SRC_TVADJ, RBS><<<g, b, 0, st>>>(a.vols, lt, a.tw, a.sel, a.parity, nullptr, nullptr,
                                                        nullptr, a.ridx);
             break;
-        case P_R2C_F64: k_r2c_x<N, EG, SRC_F64><<<g, b, 0, st>>>(a.vols, lt, a.tw, a.sel, 0, a.status, nullptr, nullptr, 0); break;
-        case P_R2C_U8: k_r2c_x<N, EG, SRC_U8><<<g, b, 0, st>>>(a.vols, lt, a.tw, a.sel, 0, a.status, nullptr, nullptr, 0); break;
-        case P_R2C_F32: k_r2c_x<N, EG, SRC_F32><<<g, b, 0, st>>>(a.vols, lt, a.tw, a.sel, 0, a.status, nullptr, nullptr, 0); break;
-        case P_R2C_ACC: k_r2c_x<N, EG, SRC_ACC><<<g, b, 0, st>>>(a.vols, lt, a.tw, a.sel, 0, nullptr, nullptr, nullptr, 0); break;
-        case P_R2C_SPHERE: k_r2c_x<N, EG, SRC_SPHERE><<<g, b, 0, st>>>(a.vols, lt, a.tw, a.sel, a.ridx, a.status, nullptr, nullptr, 0); break;
-        case P_C2R_TV: k_c2r_tv<N, EG, C2R_TV><<<g, b, 0, st>>>(a.vols, lt, a.tw, a.parity, a.status); break;
-        case P_C2R_ADJ: k_c2r_tv<N, EG, C2R_ADJ><<<g, b, 0, st>>>(a.vols, lt, a.tw, 0, nullptr); break;
-        case P_C2R_F64OUT: k_c2r_tv<N, EG, C2R_F64><<<g, b, 0, st>>>(a.vols, lt, a.tw, 0, nullptr); break;
-        case P_C2R_F64_MASKED: k_c2r_tv<N, EG, C2R_F64_MASKED><<<g, b, 0, st>>>(a.vols, lt, a.tw, 0, nullptr); break;
-        case P_C2R_X0: k_c2r_tv<N, EG, C2R_X0><<<g, b, 0, st>>>(a.vols, lt, a.tw, 0, nullptr); break;
-        case P_R2C_X0: k_r2c_x<N, EG, SRC_X0><<<g, b, 0, st>>>(a.vols, lt, a.tw, a.sel, 0, nullptr, nullptr, nullptr, 0); break;
+        case P_R2C_F64: k_r2c_x<N, EG, SRC_F64, RBS><<<g, b, 0, st>>>(a.vols, lt, a.tw, a.sel, 0, a.status, nullptr, nullptr, 0); break;
+        case P_R2C_U8: k_r2c_x<N, EG, SRC_U8, RBS><<<g, b, 0, st>>>(a.vols, lt, a.tw, a.sel, 0, a.status, nullptr, nullptr, 0); break;
+        case P_R2C_F32: k_r2c_x<N, EG, SRC_F32, RBS><<<g, b, 0, st>>>(a.vols, lt, a.tw, a.sel, 0, a.status, nullptr, nullptr, 0); break;
+        case P_R2C_ACC: k_r2c_x<N, EG, SRC_ACC, RBS><<<g, b, 0, st>>>(a.vols, lt, a.tw, a.sel, 0, nullptr, nullptr, nullptr, 0); break;
+        case P_R2C_SPHERE: k_r2c_x<N, EG, SRC_SPHERE, RBS><<<g, b, 0, st>>>(a.vols, lt, a.tw, a.sel, a.ridx, a.status, nullptr, nullptr, 0); break;
+        case P_C2R_TV: k_c2r_tv<N, EG, C2R_TV, RBS><<<g, b, 0, st>>>(a.vols, lt, a.tw, a.parity, a.status); break;
+        case P_C2R_ADJ: k_c2r_tv<N, EG, C2R_ADJ, RBS><<<g, b, 0, st>>>(a.vols, lt, a.tw, 0, nullptr); break;
+        case P_C2R_F64OUT: k_c2r_tv<N, EG, C2R_F64, RBS><<<g, b, 0, st>>>(a.vols, lt, a.tw, 0, nullptr); break;
+        case P_C2R_F64_MASKED: k_c2r_tv<N, EG, C2R_F64_MASKED, RBS><<<g, b, 0, st>>>(a.vols, lt, a.tw, 0, nullptr); break;
+        case P_C2R_X0: k_c2r_tv<N, EG, C2R_X0, RBS><<<g, b, 0, st>>>(a.vols, lt, a.tw, 0, nullptr); break;
+        case P_R2C_X0: k_r2c_x<N, EG, SRC_X0, RBS><<<g, b, 0, st>>>(a.vols, lt, a.tw, a.sel, 0, nullptr, nullptr, nullptr, 0); break;
         case P_R2C_TVDIV_W:
-            k_r2c_x<N, EG, SRC_TVDIV_W><<<g, b, 0, st>>>(a.vols, lt, a.tw, a.sel, a.parity, a.status,
+            k_r2c_x<N, EG, SRC_TVDIV_W, RBS><<<g, b, 0, st>>>(a.vols, lt, a.tw, a.sel, a.parity, a.status,
                                                          nullptr, nullptr, a.ridx);
             break;
         case P_R2C_NLTV_BND:
-            k_r2c_x<N, EG, SRC_NLTV_BND><<<g, b, 0, st>>>(a.vols, lt, a.tw, a.sel, a.parity, nullptr,
+            k_r2c_x<N, EG, SRC_NLTV_BND, RBS><<<g, b, 0, st>>>(a.vols, lt, a.tw, a.sel, a.parity, nullptr,
                                                           a.status_rw, nullptr, a.ridx);
             break;
-        case P_R2C_WRAPLAP: k_r2c_x<N, EG, SRC_WRAPLAP><<<g, b, 0, st>>>(a.vols, lt, a.tw, a.sel, 0, nullptr, nullptr, nullptr, 0); break;
+        case P_R2C_WRAPLAP: k_r2c_x<N, EG, SRC_WRAPLAP, RBS><<<g, b, 0, st>>>(a.vols, lt, a.tw, a.sel, 0, nullptr, nullptr, nullptr, 0); break;
         case P_C2R_VS_SELECT: k_c2r_vs_select<N, EG><<<g, b, 0, st>>>(a.vols, lt, a.tw, a.ridx, a.delta); break;
         case P_C2R_VS_OUT: k_c2r_vs_out<N, EG><<<g, b, 0, st>>>(a.vols, lt, a.tw); break;
         default: break;
     }
+}
+
+// xshift 2: the R2C / TV C2R kernels with a quarter of the rows per block
+// (compiled-length kernels only; the runtime-plan ones read rows_x).
+template <int N, int EG>
+static void launch_rows(int kind, const LaunchTable& lt, const PassArgs& a) {
+    if constexpr (N > 0) {
+        if (a.xshift == 2 && (qsmfft::fixed_rows(N) >> 2) >= 4) {
+            launch_rows_s<N, EG, 2>(kind, lt, a);
+            return;
+        }
+    }
+    launch_rows_s<N, EG, 0>(kind, lt, a);
 }
 
 template <int N, int EG>

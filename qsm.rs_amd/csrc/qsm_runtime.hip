@@ -127,12 +127,18 @@ struct PassBlocks {
     int xpass, cpass, ypass, zpass, out;
 };
 
-PassBlocks fill_geometry(VolDesc& V, const qsm_grid& g, TwiddleBank& tb) {
+// Pair-rows per block of the x passes: as many as 16 elements per thread
+// hold, at most 32.
+static int default_rows_x(int nx) { return std::max(1, std::min(32, (qsmfft::kThreads * 16) / nx)); }
+
+// xshift / cshift 2 (compiled lengths only): the R2C / TV C2R passes take a
+// quarter of their default rows per block, as the kernels launched with
+// PassArgs::xshift = 2 do (C2R: half the R2C pass's rows before its shift).
+PassBlocks fill_geometry(VolDesc& V, const qsm_grid& g, TwiddleBank& tb, int xshift = 0, int cshift = 0) {
     V.nx = (int)g.nx; V.ny = (int)g.ny; V.nz = (int)g.nz;
     V.nxh = V.nx / 2 + 1;
-    int rb = (qsmfft::kThreads * 16) / V.nx;
-    V.rows_x = std::max(1, std::min(32, rb));
-    V.rows_c = std::max(1, V.rows_x / 2);
+    V.rows_x = default_rows_x(V.nx) >> xshift;
+    V.rows_c = std::max(1, (default_rows_x(V.nx) >> cshift) / 2);
     V.ntx = (V.nxh + kTile - 1) / kTile;
     V.m_nx = qsmfft::magic_for((uint32_t)V.nx);
     V.m_nxh = qsmfft::magic_for((uint32_t)V.nxh);
@@ -159,6 +165,7 @@ struct PassGroup {
     int n;
     int emax;
     LaunchTable lt;
+    int xshift = 0;  // x passes: row shift of every volume in the group
 };
 using Groups = std::vector<PassGroup>;
 
@@ -221,6 +228,7 @@ void run_groups(int kind, const Groups& gs, const PassArgs& a) {
     PassArgs b = a;
     for (const auto& g : gs) {
         b.sched = column_schedule(kind, g.n);
+        b.xshift = g.xshift;
         launch_pass(kind, g.n, g.emax, g.lt, b);
     }
 }
@@ -404,10 +412,33 @@ static int tv_batch_create_impl(const qsm_tv_volume* vols, size_t n_vols, const 
     TwiddleBank tb;
     std::vector<int> lx, ly, lz, bx, bc, by, bz, bo;
     auto cleanup = [&](int rc) { batch_free(B); return rc; };
+    // A compiled x length whose R2C launch over the batch would have fewer
+    // than 1536 blocks (one small volume: 128 blocks for 128x128x64 against
+    // ~768 resident) runs that pass with a quarter of the rows per block, and
+    // its TV C2R pass likewise below 1024 blocks, where either keeps at least
+    // 4 pair-rows per block (fewer made the C2R pass slower, measured).
+    std::map<int, int> xshift, cshift;
+    {
+        std::map<int, long> xblocks, cblocks;
+        for (size_t v = 0; v < n_vols; ++v) {
+            const int nx = (int)vols[v].grid.nx;
+            const long npairs = ((long)vols[v].grid.ny * (long)vols[v].grid.nz + 1) / 2;
+            const int rx = default_rows_x(nx), rc = std::max(1, rx / 2);
+            xblocks[nx] += (npairs + rx - 1) / rx;
+            cblocks[nx] += (npairs + rc - 1) / rc;
+        }
+        for (const auto& kv : xblocks) {
+            const int nx = kv.first, r0 = default_rows_x(nx);
+            const bool fixed = has_fixed_kernels(nx);
+            xshift[nx] = (fixed && kv.second < 1536 && (r0 >> 2) >= 4) ? 2 : 0;
+            cshift[nx] = (fixed && cblocks[nx] < 1024 && (r0 >> 2) / 2 >= 4) ? 2 : 0;
+        }
+    }
     for (size_t v = 0; v < n_vols; ++v) {
         VolDesc& V = B->h[v];
         std::memset(&V, 0, sizeof V);
-        PassBlocks pb = fill_geometry(V, vols[v].grid, tb);
+        const int nxv = (int)vols[v].grid.nx;
+        PassBlocks pb = fill_geometry(V, vols[v].grid, tb, xshift[nxv], cshift[nxv]);
         const double* b = vols[v].bdir;
         double bn = std::sqrt(b[0] * b[0] + b[1] * b[1] + b[2] * b[2]);
         V.bx = b[0] / bn; V.by = b[1] / bn; V.bz = b[2] / bn;  // src/kernels/dipole.rs:39-43
@@ -449,6 +480,8 @@ static int tv_batch_create_impl(const qsm_tv_volume* vols, size_t n_vols, const 
     }
     B->gx = make_groups(lx, bx, false);
     B->gxc = make_groups(lx, bc, false);
+    for (auto& g : B->gx) g.xshift = xshift[g.n];
+    for (auto& g : B->gxc) g.xshift = cshift[g.n];
     B->gy = make_groups(ly, by, true);
     B->gz = make_groups(lz, bz, true);
     B->lt_out = identity_table(bo);
@@ -526,6 +559,7 @@ static int run_pass(qsm_tv_batch* B, int slot, int kind, const Groups& gs, const
         const int lane = (int)(i % (size_t)(S + 1));
         b.stream = lane == 0 ? B->stream : B->side[lane - 1];
         b.sched = column_schedule(kind, gs[i].n);
+        b.xshift = gs[i].xshift;
         launch_pass(kind, gs[i].n, gs[i].emax, gs[i].lt, b);
     }
     for (int s = 0; s < S; ++s) {

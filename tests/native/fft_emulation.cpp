@@ -150,6 +150,12 @@ static int check_fixed() {
     double e2 = run_fixed<N, 16, true, true>();
     double e3 = run_fixed<N, fixed_rows(N), false, false>();
     double e4 = run_fixed<N, fixed_rows(N), false, true>();
+    // the row counts of the small-volume x passes: a quarter of the rows
+    // (R2C) and half of that (C2R)
+    constexpr int q = (fixed_rows(N) >> 2) > 0 ? (fixed_rows(N) >> 2) : 1;
+    constexpr int qc = q / 2 > 0 ? q / 2 : 1;
+    e3 = std::fmax(e3, run_fixed<N, q, false, false>());
+    e4 = std::fmax(e4, run_fixed<N, qc, false, true>());
     double worst = std::fmax(std::fmax(e1, e2), std::fmax(e3, e4));
     bool ok = worst < 2e-6;
     printf("fixed n=%4d stages=%d colfwd=%.2e colinv=%.2e rowfwd=%.2e rowinv=%.2e %s\n", N,
