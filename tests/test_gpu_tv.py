@@ -30,6 +30,7 @@ def test_backend_info(qsm):
     ((26, 14, 10), (0.8, 0.8, 1.2), (0.0, 0.0, 1.0)),    # radices 13, 7, 5
     ((8, 8, 8), (1.0, 1.0, 1.0), (0.0, 0.0, 1.0)),
     ((7, 9, 5), (1.0, 1.0, 1.0), (0.1, 0.0, 1.0)),       # odd nx (no Nyquist bin)
+    ((64, 48, 40), (0.9, 1.0, 1.1), (0.1, 0.2, 0.97)),   # compiled nx = 64, quarter-row x passes
 ])
 @pytest.mark.parametrize("iters", [1, 2, 20])
 def test_tv_matches_oracle(qsm, oracle, dims, vs, bdir, iters):
