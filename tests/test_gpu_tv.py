@@ -31,6 +31,8 @@ def test_backend_info(qsm):
     ((8, 8, 8), (1.0, 1.0, 1.0), (0.0, 0.0, 1.0)),
     ((7, 9, 5), (1.0, 1.0, 1.0), (0.1, 0.0, 1.0)),       # odd nx (no Nyquist bin)
     ((64, 48, 40), (0.9, 1.0, 1.1), (0.1, 0.2, 0.97)),   # compiled nx = 64, quarter-row x passes
+    ((200, 12, 10), (1.0, 1.0, 1.0), (0.0, 0.1, 1.0)),   # compiled nx = 200 (radices 8, 5, 5)
+    ((12, 180, 220), (1.0, 0.9, 1.1), (0.1, 0.0, 1.0)),  # compiled y = 180, z = 220 (staged)
 ])
 @pytest.mark.parametrize("iters", [1, 2, 20])
 def test_tv_matches_oracle(qsm, oracle, dims, vs, bdir, iters):
