@@ -91,6 +91,27 @@ def test_pipeline_dimension_mismatch_and_config_errors():
     assert e.value.kind == "InvalidConfig"
 
 
+def test_schedule_and_graph_switches_host_side():
+    # host-only entry points (no GPU work): argument checks of the schedule
+    # mode / query and of the graph switch
+    qsm = load_package()
+    lib = qsm.c_library()
+    assert lib.qsm_set_fft_schedule(3) == 2 and lib.qsm_set_fft_schedule(-1) == 2
+    assert lib.qsm_set_fft_schedule(0) == 0
+    out = ctypes.c_int(7)
+    assert lib.qsm_fft_schedule(3, 256, ctypes.byref(out)) == 2
+    assert lib.qsm_fft_schedule(1, 384, ctypes.byref(out)) == 0 and out.value == 1  # staged only
+    assert lib.qsm_fft_schedule(2, 180, ctypes.byref(out)) == 0 and out.value == 1
+    try:
+        lib.qsm_set_fft_schedule(1)
+        assert qsm.fft_schedule("y", 256) == "two-stage"
+        lib.qsm_set_fft_schedule(2)
+        assert qsm.fft_schedule("z", 160) == "staged"
+    finally:
+        lib.qsm_set_fft_schedule(0)
+    assert lib.qsm_tv_batch_set_graphs(None, 1) == 2
+
+
 def test_bad_parameters_are_errors_not_hangs():
     qsm = load_package()
     g = qsm.Grid((8, 8, 8))
