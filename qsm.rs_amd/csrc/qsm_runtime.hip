@@ -320,7 +320,7 @@ static void batch_free(qsm_tv_batch* b) {
 // yet, time the forward y pass and the z solve pass under both schedules on
 // the batch's own launch tables and buffers (before any run, so only scratch
 // is overwritten; convergence flags are not consulted) and keep the faster
-// one for the rest of the process.  One warm-up and the best of three timed
+// one for the rest of the process.  One warm-up and the best of five timed
 // launches each.
 static int tune_column_schedules(qsm_tv_batch* B) {
     if (g_sched_mode.load() != 0) return QSM_OK;
@@ -340,7 +340,7 @@ static int tune_column_schedules(qsm_tv_batch* B) {
         a.sched = sched;
         launch_pass(kind, g.n, g.emax, g.lt, a);
         best = 1e30f;
-        for (int r = 0; r < 3 && e == hipSuccess; ++r) {
+        for (int r = 0; r < 5 && e == hipSuccess; ++r) {
             float ms = 0.f;
             if ((e = hipEventRecord(e0, B->stream)) != hipSuccess) break;
             launch_pass(kind, g.n, g.emax, g.lt, a);
