@@ -33,6 +33,9 @@ def test_backend_info(qsm):
     ((64, 48, 40), (0.9, 1.0, 1.1), (0.1, 0.2, 0.97)),   # compiled nx = 64, quarter-row x passes
     ((200, 12, 10), (1.0, 1.0, 1.0), (0.0, 0.1, 1.0)),   # compiled nx = 200 (radices 8, 5, 5)
     ((12, 180, 220), (1.0, 0.9, 1.1), (0.1, 0.0, 1.0)),  # compiled y = 180, z = 220 (staged)
+    ((512, 10, 8), (0.5, 1.0, 1.0), (0.0, 0.1, 1.0)),    # compiled nx = 512
+    ((8, 512, 6), (1.0, 0.5, 1.0), (0.1, 0.0, 1.0)),     # compiled y = 512 (staged)
+    ((6, 10, 512), (1.0, 1.0, 0.5), (0.0, 0.0, 1.0)),    # compiled z = 512 (staged)
 ])
 @pytest.mark.parametrize("iters", [1, 2, 20])
 def test_tv_matches_oracle(qsm, oracle, dims, vs, bdir, iters):

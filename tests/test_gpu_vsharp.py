@@ -49,6 +49,25 @@ def test_vsharp_matches_oracle(qsm, oracle, n, vs, radii):
     assert np.all(local[m == 0] == 0.0)
 
 
+@pytest.mark.parametrize("dims", [(512, 16, 12), (12, 512, 10), (10, 12, 512)])
+def test_vsharp_long_axis_matches_oracle(qsm, oracle, dims):
+    # one axis of the compiled length 512 (x row passes; y and z staged columns)
+    rng = np.random.default_rng(sum(dims))
+    axes = [np.arange(n) - (n - 1) / 2 for n in dims]
+    X, Y, Z = np.meshgrid(*axes, indexing="ij")
+    a = [n / 2 - 2 for n in dims]
+    mask = ((X / a[0]) ** 2 + (Y / a[1]) ** 2 + (Z / a[2]) ** 2 <= 1.0).astype(np.uint8).ravel(order="F")
+    field = (0.01 * X + 0.02 * Y + 0.03 * Z).ravel(order="F") + rng.standard_normal(mask.size) * 0.01
+    radii = [3.0, 2.0, 1.0]
+    local, m = qsm.vsharp_with_radii(field, mask, qsm.Grid(dims), radii, 0.05)
+    lo, mo = oracle.vsharp_with_radii(field, mask, dims, (1.0, 1.0, 1.0), radii, 0.05)
+    assert np.array_equal(m, mo)
+    err = _rel(local, lo, mo)
+    print(dims, "mask voxels", int(mo.sum()), "max rel err", err)
+    assert mo.sum() > 0 and err < 1e-4
+    assert np.all(local[m == 0] == 0.0)
+
+
 def test_single_radius_is_sharp(qsm, oracle):
     # src/bgremove/vsharp.rs:341-371
     n = 8
