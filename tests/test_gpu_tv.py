@@ -36,6 +36,10 @@ def test_backend_info(qsm):
     ((512, 10, 8), (0.5, 1.0, 1.0), (0.0, 0.1, 1.0)),    # compiled nx = 512
     ((8, 512, 6), (1.0, 0.5, 1.0), (0.1, 0.0, 1.0)),     # compiled y = 512 (staged)
     ((6, 10, 512), (1.0, 1.0, 0.5), (0.0, 0.0, 1.0)),    # compiled z = 512 (staged)
+    ((288, 10, 8), (1.0, 1.0, 1.0), (0.1, 0.0, 1.0)),    # compiled nx = 288
+    ((480, 8, 6), (0.5, 1.0, 1.0), (0.0, 0.1, 1.0)),     # compiled nx = 480
+    ((8, 416, 6), (1.0, 0.6, 1.0), (0.0, 0.0, 1.0)),     # compiled y = 416 (staged)
+    ((6, 10, 448), (1.0, 1.0, 0.6), (0.1, 0.1, 1.0)),    # compiled z = 448 (staged)
 ])
 @pytest.mark.parametrize("iters", [1, 2, 20])
 def test_tv_matches_oracle(qsm, oracle, dims, vs, bdir, iters):

@@ -49,9 +49,10 @@ def test_vsharp_matches_oracle(qsm, oracle, n, vs, radii):
     assert np.all(local[m == 0] == 0.0)
 
 
-@pytest.mark.parametrize("dims", [(512, 16, 12), (12, 512, 10), (10, 12, 512)])
+@pytest.mark.parametrize("dims", [(512, 16, 12), (12, 512, 10), (10, 12, 512),
+                                  (288, 16, 12), (12, 448, 10), (10, 12, 480)])
 def test_vsharp_long_axis_matches_oracle(qsm, oracle, dims):
-    # one axis of the compiled length 512 (x row passes; y and z staged columns)
+    # one long axis of a compiled length (x row passes; y and z staged columns)
     rng = np.random.default_rng(sum(dims))
     axes = [np.arange(n) - (n - 1) / 2 for n in dims]
     X, Y, Z = np.meshgrid(*axes, indexing="ij")
