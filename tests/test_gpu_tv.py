@@ -56,6 +56,33 @@ def test_tv_matches_oracle(qsm, oracle, dims, vs, bdir, iters):
     assert np.all(x[m == 0] == 0.0)
 
 
+def _compiled_lengths():
+    import importlib.util
+    import os
+    from conftest import PKG_DIR
+    spec = importlib.util.spec_from_file_location("qsm_build", os.path.join(PKG_DIR, "build.py"))
+    mod = importlib.util.module_from_spec(spec)
+    spec.loader.exec_module(mod)
+    return mod.FIXED_LENGTHS
+
+
+@pytest.mark.parametrize("n", _compiled_lengths())
+def test_tv_every_compiled_length(qsm, oracle, n):
+    # the compiled-length kernels of n on each axis in turn (x: row passes;
+    # y, z: column passes, two-stage or staged as the tuner picks)
+    for k, dims in enumerate([(n, 6, 4), (4, n, 6), (6, 4, n)]):
+        rng = np.random.default_rng(n * 3 + k)
+        f, m = _case(rng, dims)
+        vs, bdir = (1.0, 0.9, 1.1), (0.1, 0.0, 1.0)
+        p = qsm.TvParams(lambda_=1e-3, rho=0.1, tol=0.0, max_iter=3)
+        x = qsm.tv_admm(f, m, qsm.Grid(dims, vs), bdir, p)
+        xo = oracle.tv_admm(f, m, dims, vs, bdir, 1e-3, 0.1, 0.0, 3)
+        e = nrmse_dc(x, xo, m)
+        print(dims, "nrmse(dc removed)", e)
+        assert e < 2e-5, (dims, e)
+        assert np.all(x[m == 0] == 0.0)
+
+
 def test_tv_zero_field(qsm):
     # src/inversion/tv.rs:113-126
     n = 8
