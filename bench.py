@@ -15,7 +15,10 @@ The JSON line also carries
                 against 8 TB/s;
   cpu_baseline  the C++ f64 oracle (restatement of the reference's `parallel`
                 CPU path: OpenMP over FFT axis loops only) timed on this host
-                on a bounded number of iterations, scaled to one solve.
+                for one full solve of the first volume, on every CPU the job
+                may use (host CPU model and counts recorded);
+  parity        the GPU result of the timed path for that volume against the
+                oracle's solve, at the same iteration count (NRMSE, max-abs).
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--batch B]
 
 Other workloads (--workload), each printing one JSON line of its own:
@@ -105,54 +108,99 @@ def vsharp_alg_bytes(dims, R):
     return (3 * R + 4) * U + R * (8 * Nh + 18 * N) + 4 * Nh + 9 * N
 
 
-def cpu_baseline(ph, params, iters_sample, threads, qsm=None, grid=None):
-    """Oracle TV-ADMM on a bounded number of iterations, scaled to one solve.
-    With qsm and grid, the same iterations are also run on the GPU and the
-    two results compared (NRMSE of tests/common/mod.rs:34-51, raw and with
-    the mean difference inside the mask removed, and max-abs error): the
-    oracle is the checker here, the GPU result is what is checked."""
+def host_cpu_info():
+    """What this host offers the CPU baseline: logical CPUs (nproc), the ones
+    this process may run on (affinity), the cgroup CPU quota if one is set,
+    physical cores among the allowed CPUs, and the CPU model."""
+    info = {"nproc": os.cpu_count() or 1}
+    try:
+        allowed = sorted(os.sched_getaffinity(0))
+    except (AttributeError, OSError):
+        allowed = list(range(info["nproc"]))
+    info["affinity"] = len(allowed)
+    quota = None
+    try:
+        q, period = open("/sys/fs/cgroup/cpu.max").read().split()[:2]
+        if q != "max":
+            quota = float(q) / float(period)
+    except (OSError, ValueError):
+        pass
+    info["cgroup_quota_cpus"] = quota
+    model, cores, cur = None, set(), {}
+    try:
+        for line in open("/proc/cpuinfo"):
+            if ":" not in line:
+                if cur.get("processor") is not None and int(cur["processor"]) in allowed:
+                    cores.add((cur.get("physical id", "0"), cur.get("core id", cur["processor"])))
+                cur = {}
+                continue
+            k, v = (t.strip() for t in line.split(":", 1))
+            cur[k] = v
+            if k == "model name" and model is None:
+                model = v
+        if cur.get("processor") is not None and int(cur["processor"]) in allowed:
+            cores.add((cur.get("physical id", "0"), cur.get("core id", cur["processor"])))
+    except OSError:
+        pass
+    info["physical_cores_allowed"] = len(cores) or None
+    info["model"] = model
+    usable = info["affinity"]
+    if quota is not None:
+        usable = max(1, min(usable, int(quota)))
+    info["usable_threads"] = usable
+    return info
+
+
+def cpu_baseline(ph, params, threads, x_gpu=None, cpu_info=None):
+    """One full f64 oracle solve (setup + params.max_iter iterations, tol 0)
+    of the workload's first volume, wall-clock timed.  With x_gpu (the GPU
+    result of the timed path for the same volume) the two are compared:
+    NRMSE of tests/common/mod.rs:34-51, raw and with the mean difference
+    inside the mask removed, and max-abs error.  The oracle is the checker
+    here; the GPU result is what is checked."""
     os.environ["OMP_NUM_THREADS"] = str(threads)
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle as O
     O.build()
+    O.set_num_threads(threads)
+    print(f"# cpu baseline: oracle solve of {'x'.join(map(str, ph.dims))} x{params.max_iter} "
+          f"on {threads} threads", file=sys.stderr, flush=True)
     t0 = time.perf_counter()
-    O.prepare_admm_spectral(ph.local_field, ph.dims, ph.voxel_size, ph.bdir, params.rho)
-    t_setup = time.perf_counter() - t0
-    t0 = time.perf_counter()
-    x_ref = O.tv_admm(ph.local_field, ph.mask, ph.dims, ph.voxel_size, ph.bdir, params.lambda_,
-                      params.rho, 0.0, iters_sample)
-    t_run = time.perf_counter() - t0
-    # tv_admm(iters_sample) = setup + iters_sample iterations
-    t_iter = max(t_run - t_setup, 1e-9) / iters_sample
-    t_solve = t_setup + params.max_iter * t_iter
+    x_ref, it_ref, _ = O.tv_admm(ph.local_field, ph.mask, ph.dims, ph.voxel_size, ph.bdir,
+                                 params.lambda_, params.rho, 0.0, params.max_iter, return_info=True)
+    t_solve = time.perf_counter() - t0
     parity = None
-    if qsm is not None:
-        p = qsm.TvParams(lambda_=params.lambda_, rho=params.rho, tol=0.0, max_iter=iters_sample)
-        x = qsm.tv_admm(ph.local_field, ph.mask, grid, ph.bdir, p)
+    if x_gpu is not None:
         m = np.asarray(ph.mask).ravel() > 0
-        d = (np.asarray(x).ravel() - np.asarray(x_ref).ravel())[m]
+        d = (np.asarray(x_gpu).ravel() - np.asarray(x_ref).ravel())[m]
         xr = np.asarray(x_ref).ravel()[m]
         rng = float(xr.max() - xr.min())
         parity = {
-            "iterations": iters_sample,
-            "nrmse": float(O.nrmse(x, x_ref, ph.mask)),
+            "iterations": int(it_ref),
+            "nrmse": float(O.nrmse(x_gpu, x_ref, ph.mask)),
             "nrmse_mean_removed": float(np.sqrt(np.mean((d - d.mean()) ** 2)) / rng),
             "max_abs": float(np.max(np.abs(d))),
             "max_abs_mean_removed": float(np.max(np.abs(d - d.mean()))),
+            "oracle_range": rng,
             "bar": 1e-3,
-            "note": ("GPU FP32 vs the f64 oracle on this workload's phantom, same iterations; "
-                     "the mean difference is the reference's DC term for anisotropic voxels"),
+            "pass": bool(float(O.nrmse(x_gpu, x_ref, ph.mask)) <= 1e-3),
+            "note": ("GPU FP32 result of the timed batched path (volume 0) vs one full f64 oracle "
+                     "solve of the same phantom, same iterations; the mean difference is the "
+                     "reference's DC term for anisotropic voxels"),
         }
+    ci = cpu_info or {}
     return {
         "parity": parity,
         "value": 1.0 / t_solve,
         "unit": "solves/s",
         "cores": int(O.num_threads()),
         "kind": "port",
-        "sample": (f"C++ f64 restatement of the reference's `parallel` path (own FFT, OpenMP over "
-                   f"FFT axis loops only): setup + {iters_sample} of {params.max_iter} iterations "
-                   f"on {'x'.join(map(str, ph.dims))} timed ({t_run:.1f} s), scaled to one solve "
-                   f"({t_solve:.1f} s)"),
+        "sample": (f"1 full solve: C++ f64 restatement of the reference's `parallel` path (own FFT, "
+                   f"OpenMP over FFT axis loops only, other passes serial), setup + "
+                   f"{params.max_iter} iterations on {'x'.join(map(str, ph.dims))}, "
+                   f"{t_solve:.1f} s wall on {int(O.num_threads())} threads"),
+        "host": {k: ci.get(k) for k in ("model", "nproc", "affinity", "cgroup_quota_cpus",
+                                         "physical_cores_allowed", "usable_threads")},
     }
 
 
@@ -165,7 +213,8 @@ def main():
                     help="volumes per launch on the batched path (value); 1 = plain only")
     ap.add_argument("--iters", type=int, default=200)
     ap.add_argument("--profile-iters", type=int, default=20)
-    ap.add_argument("--cpu-iters", type=int, default=10)
+    ap.add_argument("--cpu-threads", type=int, default=0,
+                    help="oracle threads for the CPU baseline (default: every CPU this job may use)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--skip-plain", action="store_true",
                     help="time only the batched path (so a profiler sees batched launches only)")
@@ -226,7 +275,7 @@ def main():
         return a.steps * nvol / dt, dt / a.steps * 1e3
 
     nk = len(KERNELS)
-    plain_value = plain_ms = plain_prof = None
+    plain_value = plain_ms = plain_prof = x_plain = None
     if not (a.skip_plain and a.batch > 1):
         # plain: one volume per launch
         plain = qsm.TvBatch(vols[:1], params)
@@ -234,6 +283,7 @@ def main():
         iters_done = plain.status(0)[0]
         plain_prof = plain.profile(a.profile_iters)[:nk]
         plain.close()
+        x_plain = keep[0][2].numpy()
     # batched: every volume in one launch per kernel
     if a.batch > 1:
         batch = qsm.TvBatch(vols, params)
@@ -277,8 +327,11 @@ def main():
         roofline["frac_of_shape"] = round(achieved / shape_gbs, 4)
     cpu = None
     if not a.no_cpu:
-        threads = min(16, os.cpu_count() or 1)
-        cpu = cpu_baseline(ph, params, a.cpu_iters, threads, qsm, grid)
+        ci = host_cpu_info()
+        threads = a.cpu_threads if a.cpu_threads > 0 else ci["usable_threads"]
+        # the timed path's own output for volume 0 (every run writes the same)
+        x_gpu = keep[0][2].numpy()
+        cpu = cpu_baseline(ph, params, threads, x_gpu, ci)
         cpu["ratio_gpu_over_cpu"] = round(value / cpu["value"], 1)
     parity = cpu.pop("parity") if cpu else None
 
@@ -307,6 +360,9 @@ def main():
             "value": round(plain_value, 3), "ms_per_step": round(plain_ms, 3),
             "per_kernel_ms": {k: round(float(v), 5) for k, v in zip(KERNELS, plain_prof)}},
         "batched_over_plain": None if plain_value is None else round(value / plain_value, 3),
+        # volume 0 solved alone and inside the batch: the same bits
+        "plain_equals_batched": (None if x_plain is None or a.batch <= 1
+                                 else bool(np.array_equal(x_plain, keep[0][2].numpy()))),
         "voxel_iterations_per_s": round(value * int(np.prod(dims)) * iters_done, 1),
         "fft_schedule": {f"y{dims[1]}": qsm.fft_schedule("y", dims[1]),
                          f"z{dims[2]}": qsm.fft_schedule("z", dims[2])},

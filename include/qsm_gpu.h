@@ -274,12 +274,15 @@ int qsm_rts_batch_create(const qsm_tv_volume* vols, size_t n_vols, const qsm_rts
 int qsm_tv_batch_run(qsm_tv_batch* batch, qsm_progress_fn progress, void* user);
 /* FFT schedule of the y and z passes (no reference counterpart).  Lengths
  * 16*R0 (64, 128, 144 ... 256) have two: a two-stage transform with one LDS
- * exchange (0) and the LDS-staged Stockham stages (1).  mode 0 (default):
+ * exchange (0) and the LDS-staged Stockham stages (1); they round
+ * differently in the last bits.  mode 0 (default): a fixed per-length table
+ * (staged for y = 144 and 176, two-stage otherwise), so every process
+ * returns the same bits; 1: always two-stage; 2: always staged; 3: tuned,
  * the first batch using a length times both on its own data and the faster
- * is kept for the process; 1: always two-stage; 2: always staged. */
+ * is kept for the process (results may then differ between processes). */
 int qsm_set_fft_schedule(int mode);
 /* Schedule in use for the y (axis 1) or z (axis 2) pass of length n:
- * 0 two-stage, 1 staged, -1 not tuned yet. */
+ * 0 two-stage, 1 staged, -1 not tuned yet (mode 3). */
 int qsm_fft_schedule(int axis, int n, int* sched);
 /* Iterations after the first run as replays of HIP graphs captured on the
  * batch's stream at the first run (default), or as eager launches

@@ -801,8 +801,9 @@ FFT_SCHEDULES = {0: "two-stage", 1: "staged", -1: "untuned"}
 
 
 def set_fft_schedule(mode: int):
-    """0: tuned per axis length (default); 1: always the two-stage column
-    FFT; 2: always the LDS-staged one."""
+    """0: fixed per-length table (default, the same bits in every process);
+    1: always the two-stage column FFT; 2: always the LDS-staged one;
+    3: tuned per axis length by timing both on the first batch using it."""
     _check(_lib.qsm_set_fft_schedule(int(mode)))
 
 

@@ -72,6 +72,12 @@ bool length_ok(size_t n) {
 // EMAX (elements per thread) needed for a column pass of length n.
 int emax_for_col(int n) { return n <= 256 ? 16 : (n <= 384 ? 24 : 32); }
 
+// exp(-2 pi i q/n) from an exactly reduced angle, rounded once
+inline float2 twiddle_entry(int q, int n) {
+    long double a = -2.0L * 3.14159265358979323846264338327950288L * q / n;
+    return make_float2((float)cosl(a), (float)sinl(a));
+}
+
 struct TwiddleBank {
     std::vector<float2> host;
     std::map<int, int> off;
@@ -79,17 +85,32 @@ struct TwiddleBank {
         auto it = off.find(n);
         if (it != off.end()) return it->second;
         int o = (int)host.size();
-        for (int q = 0; q < n; ++q) {
-            // exp(-2 pi i q/n) from an exactly reduced angle, rounded once
-            long double a = -2.0L * 3.14159265358979323846264338327950288L * q / n;
-            host.push_back(make_float2((float)cosl(a), (float)sinl(a)));
-        }
+        for (int q = 0; q < n; ++q) host.push_back(twiddle_entry(q, n));
         off[n] = o;
         return o;
     }
 };
 
-qsmfft::Plan make_plan(int n, TwiddleBank& tb) {
+// The same tables for one volume (at most three lengths <= kMaxLen) in
+// fixed storage, for the entry point that allocates nothing.
+struct FixedTwiddleBank {
+    static constexpr int kCap = 3 * kMaxLen;
+    float2 host[kCap];
+    int len[3] = {0, 0, 0}, off[3] = {0, 0, 0};
+    int nlen = 0, count = 0;
+    int add(int n) {
+        for (int i = 0; i < nlen; ++i)
+            if (len[i] == n) return off[i];
+        const int o = count;
+        for (int q = 0; q < n; ++q) host[count++] = twiddle_entry(q, n);
+        len[nlen] = n;
+        off[nlen++] = o;
+        return o;
+    }
+};
+
+template <class TB>
+qsmfft::Plan make_plan(int n, TB& tb) {
     qsmfft::Plan p{};
     qsmfft::make_plan_stages(n, p);
     p.tw_off = tb.add(n);
@@ -134,7 +155,8 @@ static int default_rows_x(int nx) { return std::max(1, std::min(32, (qsmfft::kTh
 // xshift / cshift 2 (compiled lengths only): the R2C / TV C2R passes take a
 // quarter of their default rows per block, as the kernels launched with
 // PassArgs::xshift = 2 do (C2R: half the R2C pass's rows before its shift).
-PassBlocks fill_geometry(VolDesc& V, const qsm_grid& g, TwiddleBank& tb, int xshift = 0, int cshift = 0) {
+template <class TB>
+PassBlocks fill_geometry(VolDesc& V, const qsm_grid& g, TB& tb, int xshift = 0, int cshift = 0) {
     V.nx = (int)g.nx; V.ny = (int)g.ny; V.nz = (int)g.nz;
     V.nxh = V.nx / 2 + 1;
     V.rows_x = default_rows_x(V.nx) >> xshift;
@@ -194,9 +216,11 @@ Groups make_groups(const std::vector<int>& len, const std::vector<int>& blocks, 
 }
 
 // ------------------------------------------------- column-pass schedules --
-// Per (pass, axis length): -1 not tuned yet, 0 two-stage, 1 LDS-staged.
-// Pass 0: the y passes; pass 1: the per-iteration z passes.  Mode: 0 tuned
-// (two-stage until a length is tuned), 1 always two-stage, 2 always staged.
+// Per (pass, axis length): 0 two-stage, 1 LDS-staged.  Pass 0: the y passes;
+// pass 1: the per-iteration z passes.  Mode 0 (default): a fixed table, so
+// every process computes the same bits; 1 always two-stage; 2 always staged;
+// 3 tuned: the first batch using a length times both forms and the faster
+// one is kept for the process (-1 in the table until then).
 namespace {
 constexpr int kMaxAxis = 512;
 std::mutex g_sched_mu;
@@ -211,6 +235,12 @@ int sched_pass(int kind) {
     if (kind == P_Z_TV_SOLVE || kind == P_Z_TV_SOLVE0 || kind == P_Z_TV_ADJ || kind == P_Z_RTS_SOLVE) return 1;
     return -1;
 }
+
+// The fixed table: the forms the tuner chose on MI355X over the TV workloads
+// (256x256x160, the uneven 32-volume batch): the staged form for y passes of
+// length 144 and 176, where radices 9 and 11 in registers cost more than
+// LDS stages, and the two-stage form everywhere else.
+int fixed_schedule(int pass, int n) { return (pass == 0 && (n == 144 || n == 176)) ? 1 : 0; }
 }  // namespace
 
 int column_schedule(int kind, int n) {
@@ -219,9 +249,10 @@ int column_schedule(int kind, int n) {
     const int mode = g_sched_mode.load();
     if (mode == 1) return 0;
     if (mode == 2) return 1;
+    if (mode == 0) return fixed_schedule(pass, n);
     std::lock_guard<std::mutex> lk(g_sched_mu);
     const int t = g_sched_tab[pass][n];
-    return t < 0 ? 0 : t;
+    return t < 0 ? fixed_schedule(pass, n) : t;
 }
 
 void run_groups(int kind, const Groups& gs, const PassArgs& a) {
@@ -316,14 +347,17 @@ static void batch_free(qsm_tv_batch* b) {
     delete b;
 }
 
-// The autotuner: for every y and z length of a new batch that has no entry
-// yet, time the forward y pass and the z solve pass under both schedules on
-// the batch's own launch tables and buffers (before any run, so only scratch
-// is overwritten; convergence flags are not consulted) and keep the faster
-// one for the rest of the process.  One warm-up and the best of five timed
-// launches each.
+// The autotuner (schedule mode 3): for every y and z length of a new batch
+// that has no entry yet, time the forward y pass and the z solve pass under
+// both schedules on the batch's own launch tables and buffers (before any
+// run, so only scratch is overwritten, after clearing it so that no
+// uninitialised values enter the timed launches; convergence flags are not
+// consulted) and keep the faster one for the rest of the process.  One
+// warm-up and the best of five timed launches each.  The two forms round
+// differently, so tuned results can differ between processes in the last
+// bits; the default mode uses the fixed table instead.
 static int tune_column_schedules(qsm_tv_batch* B) {
-    if (g_sched_mode.load() != 0) return QSM_OK;
+    if (g_sched_mode.load() != 3) return QSM_OK;
     {
         // a stream under capture cannot be timed: keep the defaults
         hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
@@ -332,6 +366,10 @@ static int tune_column_schedules(qsm_tv_batch* B) {
             return QSM_OK;
         }
     }
+    for (const auto& V : B->h)
+        for (int sidx : {SPEC_WORK, SPEC_A})
+            launch_fill_zero(B->stream, reinterpret_cast<float*>(V.spec[sidx]),
+                             2 * (size_t)V.nxh * V.ny * V.nz);
     hipEvent_t e0 = nullptr, e1 = nullptr;
     hipError_t e = hipEventCreate(&e0);
     if (e == hipSuccess) e = hipEventCreate(&e1);
@@ -940,6 +978,39 @@ static int unrolled_backward_impl(qsm_tv_unrolled* U, const double* const* d_chi
 // parameter `fa`, output masked by d_mask).
 // src_kind P_R2C_F64 (d_in as is) or P_R2C_WRAPLAP (masked wrapped
 // Laplacian of d_in); bdir may be null for filters that do not use D.
+// The five passes of one spectral round trip of a single volume: R2C of
+// src_kind, y, z pass zkind, inverse y, C2R to f64 (masked for filters).
+// Launch tables live on the stack.
+static void enqueue_round_trip(int src_kind, int zkind, const VolDesc& V, const PassBlocks& pb,
+                               const VolDesc* d_vols, const float2* d_tw, hipStream_t st) {
+    auto table = [](int blocks) {
+        LaunchTable lt{};
+        lt.nvol = 1;
+        lt.start[0] = 0;
+        lt.start[1] = blocks;
+        lt.vol[0] = 0;
+        return lt;
+    };
+    const LaunchTable tx = table(pb.xpass), txc = table(pb.cpass), ty = table(pb.ypass),
+                      tz = table(pb.zpass);
+    PassArgs a{};
+    a.stream = st;
+    a.vols = d_vols;
+    a.tw = d_tw;
+    a.sel = SPEC_WORK;
+    a.sel2 = -1;
+    a.xshift = 0;
+    auto run = [&](int kind, int n, bool column, const LaunchTable& lt) {
+        a.sched = column_schedule(kind, n);
+        launch_pass(kind, n, column ? emax_for_col(n) : 16, lt, a);
+    };
+    run(src_kind, V.nx, false, tx);
+    run(P_Y_FWD, V.ny, true, ty);
+    run(zkind, V.nz, true, tz);
+    run(P_Y_INV, V.ny, true, ty);
+    run(zkind == P_Z_FILTER ? P_C2R_F64_MASKED : P_C2R_F64OUT, V.nx, false, txc);
+}
+
 static int spectral_apply(int zkind, int filt, double fa, const double* d_in, const uint8_t* d_mask,
                           const qsm_grid* grid, const double bdir_in[3], double* d_out, void* stream,
                           int src_kind = P_R2C_F64) {
@@ -989,21 +1060,7 @@ static int spectral_apply(int zkind, int filt, double fa, const double* d_in, co
     hipError_t e2 = hipMemcpy(d_vols, &V, sizeof V, hipMemcpyHostToDevice);
     if (e1 != hipSuccess || e2 != hipSuccess)
         return done(fail(QSM_EHIP, "hipMemcpy of plan tables: %s", hipGetErrorString(e1 != hipSuccess ? e1 : e2)));
-    Groups gx = make_groups({V.nx}, {pb.xpass}, false);
-    Groups gxc = make_groups({V.nx}, {pb.cpass}, false);
-    Groups gy = make_groups({V.ny}, {pb.ypass}, true);
-    Groups gz = make_groups({V.nz}, {pb.zpass}, true);
-    PassArgs a{};
-    a.stream = st;
-    a.vols = d_vols;
-    a.tw = d_tw;
-    a.sel = SPEC_WORK;
-    a.sel2 = -1;
-    run_groups(src_kind, gx, a);
-    run_groups(P_Y_FWD, gy, a);
-    run_groups(zkind, gz, a);
-    run_groups(P_Y_INV, gy, a);
-    run_groups(zkind == P_Z_FILTER ? P_C2R_F64_MASKED : P_C2R_F64OUT, gxc, a);
+    enqueue_round_trip(src_kind, zkind, V, pb, d_vols, d_tw, st);
     hipError_t le = hipGetLastError();
     if (le != hipSuccess) return done(fail(QSM_EHIP, "kernel launch: %s", hipGetErrorString(le)));
     hipError_t se = hipStreamSynchronize(st);
@@ -1322,7 +1379,7 @@ int qsm_tv_batch_run(qsm_tv_batch* batch, qsm_progress_fn progress, void* user) 
 
 int qsm_set_fft_schedule(int mode) {
     g_err.clear();
-    if (mode < 0 || mode > 2) return fail(QSM_EPARAM, "schedule mode must be 0, 1 or 2");
+    if (mode < 0 || mode > 3) return fail(QSM_EPARAM, "schedule mode must be 0, 1, 2 or 3");
     g_sched_mode.store(mode);
     return QSM_OK;
 }
@@ -1335,8 +1392,12 @@ int qsm_fft_schedule(int axis, int n, int* sched) {
         return QSM_OK;
     }
     const int mode = g_sched_mode.load();
-    if (mode) {
+    if (mode == 1 || mode == 2) {
         *sched = mode - 1;
+        return QSM_OK;
+    }
+    if (mode == 0) {
+        *sched = fixed_schedule(axis - 1, n);
         return QSM_OK;
     }
     std::lock_guard<std::mutex> lk(g_sched_mu);

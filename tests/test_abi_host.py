@@ -96,17 +96,23 @@ def test_schedule_and_graph_switches_host_side():
     # mode / query and of the graph switch
     qsm = load_package()
     lib = qsm.c_library()
-    assert lib.qsm_set_fft_schedule(3) == 2 and lib.qsm_set_fft_schedule(-1) == 2
+    assert lib.qsm_set_fft_schedule(4) == 2 and lib.qsm_set_fft_schedule(-1) == 2
     assert lib.qsm_set_fft_schedule(0) == 0
     out = ctypes.c_int(7)
     assert lib.qsm_fft_schedule(3, 256, ctypes.byref(out)) == 2
     assert lib.qsm_fft_schedule(1, 384, ctypes.byref(out)) == 0 and out.value == 1  # staged only
     assert lib.qsm_fft_schedule(2, 180, ctypes.byref(out)) == 0 and out.value == 1
+    # default mode: the fixed table, the same in every process
+    assert qsm.fft_schedule("y", 256) == "two-stage" and qsm.fft_schedule("z", 160) == "two-stage"
+    assert qsm.fft_schedule("y", 144) == "staged" and qsm.fft_schedule("y", 176) == "staged"
+    assert qsm.fft_schedule("z", 144) == "two-stage"
     try:
         lib.qsm_set_fft_schedule(1)
         assert qsm.fft_schedule("y", 256) == "two-stage"
         lib.qsm_set_fft_schedule(2)
         assert qsm.fft_schedule("z", 160) == "staged"
+        lib.qsm_set_fft_schedule(3)  # tuned: nothing timed yet in this process
+        assert qsm.fft_schedule("y", 240) == "untuned"
     finally:
         lib.qsm_set_fft_schedule(0)
     assert lib.qsm_tv_batch_set_graphs(None, 1) == 2
