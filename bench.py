@@ -65,10 +65,12 @@ def _load(name, path, search=None):
 
 def alg_bytes(dims):
     """Algorithmic HBM bytes per launch of each iteration kernel (FP32):
-    every input read once, every output written once."""
+    every input read once, every output written once.  A half spectrum holds
+    (nx + 1) // 2 complex values per row (even nx: DC and Nyquist share one),
+    i.e. Nh = ceil(nx / 2) ny nz elements of 8 bytes."""
     nx, ny, nz = dims
     N = nx * ny * nz
-    Nh = (nx // 2 + 1) * ny * nz
+    Nh = ((nx + 1) // 2) * ny * nz
     return [
         28 * N + 8 * Nh,   # read x (4N) + u (12N); write u (12N) + half spectrum (8Nh)
         16 * Nh,           # read + write half spectrum
@@ -103,7 +105,7 @@ def vsharp_alg_bytes(dims, R):
     real transform)."""
     nx, ny, nz = dims
     N = nx * ny * nz
-    Nh = (nx // 2 + 1) * ny * nz
+    Nh = ((nx + 1) // 2) * ny * nz
     U = 4 * N + 40 * Nh
     return (3 * R + 4) * U + R * (8 * Nh + 18 * N) + 4 * Nh + 9 * N
 
@@ -517,7 +519,7 @@ def bench_vjp(a, qsm, phantom):
     value, ms_per_step = _timed(a, step, 1, qsm.synchronize)
     u.close()
     N = n
-    Nh = (dims[0] // 2 + 1) * dims[1] * dims[2]
+    Nh = ((dims[0] + 1) // 2) * dims[1] * dims[2]
     # backward iteration: adjoint source (c-bar 4N, s-bar 12N in, 12N out, q 3N,
     # acc 8N) + spectrum 8Nh, y/z/y passes 48Nh, C2R 8Nh + 4N
     bwd_bytes = K * (43 * N + 64 * Nh)

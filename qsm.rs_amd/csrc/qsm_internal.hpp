@@ -4,7 +4,8 @@
 //   real volumes   x, field, ...  : nx*ny*nz floats, Fortran order
 //                                   (idx = i + j*nx + k*nx*ny)
 //   dual variable  u              : 3 planes of N floats (ux | uy | uz)
-//   half spectrum  spec, F, ...   : nxh*ny*nz complex (float2), nxh = nx/2+1,
+//   half spectrum  spec, F, ...   : nxh*ny*nz complex (float2), nxh = (nx+1)/2
+//                                   (even nx: DC and Nyquist share column 0),
 //                                   idx = kx + ky*nxh + kz*nxh*ny
 // Every kernel is launched once for a whole batch of volumes; a LaunchTable
 // maps the flat block index to (volume, block-within-volume).

@@ -155,6 +155,82 @@ __device__ __forceinline__ double fftfreq_at(int i, int n, double d) {
 
 __device__ __forceinline__ int mirror_idx(int i, int n) { return i == 0 ? 0 : n - i; }
 
+// ------------------------------------------------ packed half spectrum --
+// A half spectrum stores nxh = (nx + 1) / 2 columns kx.  For even nx the x
+// transform of a real row has two real bins, DC and Nyquist (kx = nx/2); both
+// go to column 0 as (DC, Nyquist), so every row holds nx/2 complex values and
+// the y / z passes work on whole 16-column tiles with rows of nx/2 * 8 bytes.
+// Column 0 then carries the kx = 0 plane P0 and the kx = nx/2 plane PN, both
+// real in (y, z) and so Hermitian in (ky, kz).  The forward y pass
+// separates them: after it, entry ky of column 0 holds
+//   ky = 0, ny/2 : P0(ky) + i PN(ky)            (both real in ky, left packed)
+//   0 < ky < ny/2: P0(ky)
+//   ny/2 < ky    : PN(ny - ky)
+// so the z pass sees column 0 at ky < ny/2 as frequency (0, ky), at
+// ky > ny/2 as (nx/2, ny - ky), and at ky = 0, ny/2 a column Z = A + i B of
+// two z spectra A (kx = 0) and B (kx = nx/2), each Hermitian in kz, which a
+// real per-frequency multiplier pair (mA, mB) maps to
+//   alpha Z(kz) + beta conj(Z(-kz)),  alpha = (mA + mB)/2, beta = (mA - mB)/2.
+// The inverse y pass packs the entries back before transforming.  Odd nx has
+// no Nyquist bin and no packing.
+__device__ __forceinline__ bool packed_x(const VolDesc& V) { return (V.nx & 1) == 0; }
+
+// Frequency (fx, fy) of column kx of a z-pass tile at ky (not for the two
+// doubly packed columns, see pk_double).
+__device__ __forceinline__ void pk_freq(const VolDesc& V, int kx, int ky, int& fx, int& fy) {
+    const bool nyq = kx == 0 && packed_x(V) && 2 * ky > V.ny;
+    fx = nyq ? V.nx / 2 : kx;
+    fy = nyq ? V.ny - ky : ky;
+}
+
+// Tile tx at ky holds a doubly packed column 0 (block-uniform).
+__device__ __forceinline__ bool pk_double(const VolDesc& V, int tx, int ky) {
+    return tx == 0 && packed_x(V) && (ky == 0 || 2 * ky == V.ny);
+}
+
+// alpha z + beta conj(zm) for the multiplier pair (mA, mB)
+__device__ __forceinline__ float2 pk_apply(float2 z, float2 zm, float mA, float mB) {
+    const float a = 0.5f * (mA + mB), b = 0.5f * (mA - mB);
+    return make_float2(a * z.x + b * zm.x, a * z.y - b * zm.y);
+}
+__device__ __forceinline__ float2 pk_apply_d(float2 z, float2 zm, double mA, double mB) {
+    const double a = 0.5 * (mA + mB), b = 0.5 * (mA - mB);
+    return make_float2((float)(a * (double)z.x + b * (double)zm.x), (float)(a * (double)z.y - b * (double)zm.y));
+}
+
+// Entry s of column 0 after the forward y transform (w = W(s), wm = W(ny-s)).
+__device__ __forceinline__ float2 ypk_fwd(int s, int ny, float2 w, float2 wm) {
+    if (s == 0 || 2 * s == ny) return w;
+    if (2 * s < ny) return make_float2(0.5f * (w.x + wm.x), 0.5f * (w.y - wm.y));  // P0(s)
+    return make_float2(0.5f * (wm.y + w.y), 0.5f * (w.x - wm.x));                  // PN(ny-s)
+}
+// W(s) rebuilt before the inverse y transform from entries s and ny - s.
+__device__ __forceinline__ float2 ypk_inv(int s, int ny, float2 own, float2 mir) {
+    if (s == 0 || 2 * s == ny) return own;
+    if (2 * s < ny) return make_float2(own.x - mir.y, own.y + mir.x);  // P0(s) + i PN(s)
+    return make_float2(mir.x + own.y, own.x - mir.y);                  // conj P0 + i conj PN
+}
+
+// The same on column 0 of a tile held in LDS (lds[row * 16 + c]); every
+// thread of the block calls it.
+template <bool INV>
+__device__ __forceinline__ void ypk_lds(float2* lds, int ny) {
+    float2 own[2], mir[2];
+#pragma unroll
+    for (int q = 0; q < 2; ++q) {
+        const int s = min((int)threadIdx.x + q * kThreads, ny - 1);
+        own[q] = lds[s * kTile];
+        mir[q] = lds[mirror_idx(s, ny) * kTile];
+    }
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < 2; ++q) {
+        const int s = threadIdx.x + q * kThreads;
+        if (s < ny) lds[s * kTile] = INV ? ypk_inv(s, ny, own[q], mir[q]) : ypk_fwd(s, ny, own[q], mir[q]);
+    }
+    __syncthreads();
+}
+
 // Binary sphere of src/kernels/smv.rs:19-67 (un-normalised: 1 inside).  The
 // squared distance is formed in FP64 in the reference's order with every
 // product and sum rounded (contraction off), so boundary voxels are
@@ -681,7 +757,7 @@ __global__ __launch_bounds__(kThreads) void k_r2c_x(const VolDesc* __restrict__ 
         if (status_rw && lb == 0) tv_reduce_block(V, gv, status_rw, mirror, sh);
     }
     const int nx = N > 0 ? N : V.nx;
-    const int nxh = nx / 2 + 1;
+    const int nxh = (nx + 1) / 2;
     const int RB = N > 0 ? RBX : V.rows_x;
     const int nrows = V.ny * V.nz;
     const int row0 = lb * RB * 2;
@@ -727,8 +803,14 @@ __global__ __launch_bounds__(kThreads) void k_r2c_x(const VolDesc* __restrict__ 
             const int ra = row0 + 2 * p, rb = ra + 1;
             const float2 zk = lds[p * nx + k];
             const float2 zm = lds[p * nx + (k == 0 ? 0 : nx - k)];
-            const float2 A = make_float2(0.5f * (zk.x + zm.x), 0.5f * (zk.y - zm.y));
-            const float2 B = make_float2(0.5f * (zk.y + zm.y), -0.5f * (zk.x - zm.x));
+            float2 A = make_float2(0.5f * (zk.x + zm.x), 0.5f * (zk.y - zm.y));
+            float2 B = make_float2(0.5f * (zk.y + zm.y), -0.5f * (zk.x - zm.x));
+            if ((nx & 1) == 0 && k == 0) {
+                // (DC, Nyquist) of both rows: real parts of Z[0] and Z[nx/2]
+                const float2 zn = lds[p * nx + nx / 2];
+                A = make_float2(zk.x, zn.x);
+                B = make_float2(zk.y, zn.y);
+            }
             if (ra < nrows) stg2(out + k + (size_t)ra * nxh, A);
             if (rb < nrows) stg2(out + k + (size_t)rb * nxh, B);
         }
@@ -764,18 +846,59 @@ __global__ __launch_bounds__(kThreads) void k_y_pass(const VolDesc* __restrict__
     QSM_GLOBAL float2* buf = glb(V.spec[s]) + (size_t)kz * nxh * ny;
     TwStage<N> tws;
     tws.fetch(tw + V.py.tw_off, ny);
+    // tile 0 of an even-nx volume: column 0 is packed (DC and Nyquist planes)
+    const bool pkc = tx == 0 && packed_x(V);
     if constexpr (TWO) {
         constexpr int R0 = N / 16;
+        const int c = threadIdx.x & 15, j = threadIdx.x >> 4;
         float2 v[16];
         l16_load<R0>(v, buf, nxh, kx0, nxh);
         tws.commit(twl, ny);
+        // column 0 of a packed tile goes through LDS (free before the
+        // first exchange and after the last one)
+        if constexpr (INV) {
+            if (pkc) {  // L16 layout: thread (j, c) holds rows j + 16 r
+                if (c == 0) {
+#pragma unroll
+                    for (int r = 0; r < R0; ++r) lds[j + 16 * r] = v[r];
+                }
+                __syncthreads();
+                if (c == 0) {
+#pragma unroll
+                    for (int r = 0; r < R0; ++r) {
+                        const int row = j + 16 * r;
+                        v[r] = ypk_inv(row, N, v[r], lds[mirror_idx(row, N)]);
+                    }
+                }
+                __syncthreads();  // before the transform's first exchange writes
+            }
+        }
         qsmfft::Col2<R0>::template l16_to_lr<INV>(v, lds, twl);
+        if constexpr (!INV) {
+            if (pkc) {  // LR layout: thread (j < R0, c) holds rows j + R0 r
+                __syncthreads();  // the transform's last reads are done
+                if (c == 0 && j < R0) {
+#pragma unroll
+                    for (int r = 0; r < 16; ++r) lds[j + R0 * r] = v[r];
+                }
+                __syncthreads();
+                if (c == 0 && j < R0) {
+#pragma unroll
+                    for (int r = 0; r < 16; ++r) {
+                        const int row = j + R0 * r;
+                        v[r] = ypk_fwd(row, N, v[r], lds[mirror_idx(row, N)]);
+                    }
+                }
+            }
+        }
         lr_store<R0>(v, buf, nxh, kx0, nxh);
     } else {
         col_load<EMAX>(lds, buf, nxh, ny, kx0, nxh);
         tws.commit(twl, ny);
         __syncthreads();
+        if (INV && pkc) ypk_lds<true>(lds, ny);
         run_fft<N, EMAX, true, kLog2Tile, INV>(lds, V.py, kTile, kTile, 1, twl);
+        if (!INV && pkc) ypk_lds<false>(lds, ny);
         col_store<EMAX>(lds, buf, nxh, ny, kx0, nxh);
     }
 }
@@ -937,6 +1060,53 @@ struct TvQ {
     }
 };
 
+// Symmetrised multipliers of the spectral setup at one frequency, averaged
+// over k and its mirror (the reference keeps Re(IFFT(.)) of a full
+// spectrum, the inverse of its Hermitian part; on Nyquist planes D(k) !=
+// D(-k) when B0 is tilted): m for f^ and q = rho inv_a / N (TV) or
+// inv_a / N (RTS).
+template <int OP>
+__device__ inline void setup_mult(const VolDesc& V, int kx, int ky, int kz, double& m, double& qv) {
+    const double invN = 1.0 / ((double)V.nx * V.ny * V.nz);
+    const int mx = mirror_idx(kx, V.nx), my = mirror_idx(ky, V.ny), mz = mirror_idx(kz, V.nz);
+    double d1, i1, d2, i2;
+    if constexpr (OP == ZOP_TV_SETUP) {
+        tv_multipliers(V, kx, ky, kz, d1, i1);
+        tv_multipliers(V, mx, my, mz, d2, i2);
+        m = 0.5 * (d1 + d2) * invN;
+        qv = V.rho * 0.5 * (i1 + i2) * invN;
+    } else {
+        rts_multipliers(V, kx, ky, kz, d1, i1);
+        rts_multipliers(V, mx, my, mz, d2, i2);
+        m = 0.5 * (d1 + d2) * invN;
+        qv = 0.5 * (i1 + i2) * invN;
+    }
+}
+
+// Symmetrised multiplier of APPLYA (D inv_a / N), DIPOLE (D / N) or FILTER.
+template <int OP>
+__device__ inline double apply_mult(const VolDesc& V, int kx, int ky, int kz) {
+    const double invN = 1.0 / ((double)V.nx * V.ny * V.nz);
+    const int mx = mirror_idx(kx, V.nx), my = mirror_idx(ky, V.ny), mz = mirror_idx(kz, V.nz);
+    if constexpr (OP == ZOP_TV_APPLYA) {
+        double d1, i1, d2, i2;
+        tv_multipliers(V, kx, ky, kz, d1, i1);
+        tv_multipliers(V, mx, my, mz, d2, i2);
+        return 0.5 * (d1 + d2) * invN;
+    } else if constexpr (OP == ZOP_DIPOLE) {
+        return 0.5 * (dipole_at(V, kx, ky, kz) + dipole_at(V, mx, my, mz)) * invN;
+    } else {
+        return 0.5 * (filter_mult(V, kx, ky, kz) + filter_mult(V, mx, my, mz)) * invN;
+    }
+}
+
+// RTS step multiplier q = inv_a / N (symmetrised), in FP64.
+__device__ inline float rts_q(const VolDesc& V, int kx, int ky, int kz) {
+    double m, qv;
+    setup_mult<ZOP_RTS_SETUP>(V, kx, ky, kz, m, qv);
+    return (float)qv;
+}
+
 // TV z pass.  SETUP: forward z of FFT(f) (after x, y), then
 //   f^ = FFT(f) * D * inv_a / N  -> SPEC_A,   rho * inv_a / N -> q,
 // with both multipliers averaged over k and -k: the reference keeps
@@ -952,7 +1122,7 @@ struct TvQ {
 // inverse z back to the load layout; two LDS exchanges in all.
 template <int N, int OP>
 __device__ __forceinline__ void z_tv_col2(const VolDesc& V, float2* lds, float2* twl, const TwStage<N>& tws,
-                                          int ky, int kx0) {
+                                          int ky, int kx0, bool dpk) {
     constexpr int R0 = N / 16;
     using C = qsmfft::Col2<R0>;
     const int nxh = V.nxh;
@@ -994,7 +1164,9 @@ __device__ __forceinline__ void z_tv_col2(const VolDesc& V, float2* lds, float2*
     if constexpr (!Q_MEM) {
         // the tile is in LDS and the operands are in flight: form q here
         TvQ tq;
-        tq.init(V, min(kx0 + c, nxh - 1), ky);
+        int fx, fy;
+        pk_freq(V, min(kx0 + c, nxh - 1), ky, fx, fy);
+        tq.init(V, fx, fy);
         if (act) {
             float2 tz[16];  // all LDS reads issued before the first use
 #pragma unroll
@@ -1004,13 +1176,37 @@ __device__ __forceinline__ void z_tv_col2(const VolDesc& V, float2* lds, float2*
         }
     }
     C::template b_fwd<false>(threadIdx.x, v, lds, twl);
+    // doubly packed column 0 (block-uniform dpk): its entries go through LDS
+    // so that each element can see Z(-kz); q at (0, ky) is qr, the Nyquist
+    // plane's multiplier is formed here
+    const bool dcol = dpk && c == 0 && act;
+    if (dpk) {
+        __syncthreads();  // every read of the first exchange is done
+        if (dcol) {
+#pragma unroll
+            for (int r = 0; r < 16; ++r) lds[j + R0 * r] = v[r];
+        }
+        __syncthreads();
+    }
     if (act) {
+        TvQ tqB;
+        if constexpr (!Q_MEM) {
+            if (dcol) tqB.init(V, V.nx / 2, ky);
+        }
 #pragma unroll
         for (int r = 0; r < 16; ++r) {
+            float2 qs = make_float2(qr[r] * v[r].x, qr[r] * v[r].y);
+            if (dcol) {
+                const int kz = j + R0 * r;
+                float qB;
+                if constexpr (Q_MEM) qB = rts_q(V, V.nx / 2, ky, kz);
+                else qB = tqB.at(kz, N, twl[kz]);
+                qs = pk_apply(v[r], lds[mirror_idx(kz, N)], qr[r], qB);
+            }
             if constexpr (HAS_F)
-                v[r] = make_float2(fr[r].x - qr[r] * v[r].x, fr[r].y - qr[r] * v[r].y);
+                v[r] = make_float2(fr[r].x - qs.x, fr[r].y - qs.y);
             else
-                v[r] = make_float2(-qr[r] * v[r].x, -qr[r] * v[r].y);
+                v[r] = make_float2(-qs.x, -qs.y);
         }
     }
     __syncthreads();  // every read of the first exchange precedes the second's writes
@@ -1040,11 +1236,13 @@ __global__ __launch_bounds__(kThreads) void k_z_tv(const VolDesc* __restrict__ v
     const size_t zs = (size_t)nxh * V.ny;
     const size_t base = (size_t)ky * nxh;
     const int total = nz * kTile;
+    // column 0 holds two z spectra (the DC and Nyquist planes at ky = 0, ny/2)
+    const bool dpk = pk_double(V, tx, ky);
     QSM_GLOBAL float2* buf = glb(V.spec[SPEC_WORK]) + base;
     TwStage<N> tws;
     tws.fetch(tw + V.pz.tw_off, nz);
     if constexpr (TWO) {
-        z_tv_col2<N, OP>(V, lds, twl, tws, ky, kx0);
+        z_tv_col2<N, OP>(V, lds, twl, tws, ky, kx0, dpk);
         return;
     }
     if constexpr (OP == ZOP_TV_SOLVE0) {
@@ -1078,72 +1276,108 @@ __global__ __launch_bounds__(kThreads) void k_z_tv(const VolDesc* __restrict__ v
     tws.commit(twl, nz);
     __syncthreads();
     run_fft<N, EMAX, true, kLog2Tile, false>(lds, V.pz, kTile, kTile, 1, twl);
+    // a thread's elements all lie in column c
+    const int c = threadIdx.x & (kTile - 1);
+    const bool dcol = dpk && c == 0;
+    // doubly packed column: Z(-kz) of each element, read before any is rewritten
+    float2 zmir[EMAX];
+    if (dpk) {
+        if (dcol) {
+#pragma unroll
+            for (int t = 0; t < EMAX; ++t) {
+                const int kz = min((int)(threadIdx.x + t * kThreads) >> kLog2Tile, nz - 1);
+                zmir[t] = lds[mirror_idx(kz, nz) * kTile];
+            }
+        }
+        __syncthreads();
+    }
     if constexpr (OP == ZOP_TV_SETUP || OP == ZOP_RTS_SETUP) {
-        const double invN = 1.0 / ((double)V.nx * V.ny * V.nz);
         QSM_GLOBAL float2* F = glb(V.spec[SPEC_A]) + base;
         QSM_GLOBAL float* Q = glb(V.q) + base;
-        for (int e = threadIdx.x; e < total; e += kThreads) {
-            const int kz = e >> kLog2Tile, c = e & (kTile - 1);
+#pragma unroll
+        for (int t = 0; t < EMAX; ++t) {
+            const int e = threadIdx.x + t * kThreads;
+            const int kz = e >> kLog2Tile;
             const int kx = kx0 + c;
-            if (kx < nxh) {
-                double d1, i1, d2, i2;
-                const int mx = mirror_idx(kx, V.nx), my = mirror_idx(ky, V.ny), mz = mirror_idx(kz, nz);
-                double m, qv;
-                if constexpr (OP == ZOP_TV_SETUP) {
-                    tv_multipliers(V, kx, ky, kz, d1, i1);
-                    tv_multipliers(V, mx, my, mz, d2, i2);
-                    m = 0.5 * (d1 + d2) * invN;
-                    qv = V.rho * 0.5 * (i1 + i2) * invN;
-                } else {
-                    rts_multipliers(V, kx, ky, kz, d1, i1);
-                    rts_multipliers(V, mx, my, mz, d2, i2);
-                    m = 0.5 * (d1 + d2) * invN;
-                    qv = 0.5 * (i1 + i2) * invN;
-                }
+            if (e < total && kx < nxh) {
                 const float2 sv = lds[e];
-                stg2(F + kx + kz * zs, make_float2((float)((double)sv.x * m), (float)((double)sv.y * m)));
-                Q[kx + kz * zs] = (float)qv;
+                float2 fv;
+                double qv;
+                if (dcol) {
+                    double mA, mB, qB;
+                    setup_mult<OP>(V, 0, ky, kz, mA, qv);
+                    setup_mult<OP>(V, V.nx / 2, ky, kz, mB, qB);
+                    fv = pk_apply_d(sv, zmir[t], mA, mB);
+                } else {
+                    int fx, fy;
+                    pk_freq(V, kx, ky, fx, fy);
+                    double m;
+                    setup_mult<OP>(V, fx, fy, kz, m, qv);
+                    fv = make_float2((float)((double)sv.x * m), (float)((double)sv.y * m));
+                }
+                stg2(F + kx + kz * zs, fv);
+                Q[kx + kz * zs] = (float)qv;  // (RTS recomputes q in the doubly packed column)
             }
         }
     } else {
         if constexpr (OP == ZOP_TV_APPLYA || OP == ZOP_DIPOLE || OP == ZOP_FILTER) {
-            const double invN = 1.0 / ((double)V.nx * V.ny * V.nz);
-            for (int e = threadIdx.x; e < total; e += kThreads) {
-                const int kz = e >> kLog2Tile, c = e & (kTile - 1);
+#pragma unroll
+            for (int t = 0; t < EMAX; ++t) {
+                const int e = threadIdx.x + t * kThreads;
+                const int kz = e >> kLog2Tile;
                 const int kx = kx0 + c;
-                double m = 0.0;
-                if (kx < nxh) {
-                    const int mx = mirror_idx(kx, V.nx), my = mirror_idx(ky, V.ny), mz = mirror_idx(kz, nz);
-                    if constexpr (OP == ZOP_TV_APPLYA) {
-                        double d1, i1, d2, i2;
-                        tv_multipliers(V, kx, ky, kz, d1, i1);
-                        tv_multipliers(V, mx, my, mz, d2, i2);
-                        m = 0.5 * (d1 + d2) * invN;
-                    } else if constexpr (OP == ZOP_DIPOLE) {
-                        m = 0.5 * (dipole_at(V, kx, ky, kz) + dipole_at(V, mx, my, mz)) * invN;
-                    } else {
-                        m = 0.5 * (filter_mult(V, kx, ky, kz) + filter_mult(V, mx, my, mz)) * invN;
+                if (e < total) {
+                    const float2 sv = lds[e];
+                    float2 out = make_float2(0.0f, 0.0f);
+                    if (kx < nxh) {
+                        if (dcol) {
+                            out = pk_apply_d(sv, zmir[t], apply_mult<OP>(V, 0, ky, kz),
+                                             apply_mult<OP>(V, V.nx / 2, ky, kz));
+                        } else {
+                            int fx, fy;
+                            pk_freq(V, kx, ky, fx, fy);
+                            const double m = apply_mult<OP>(V, fx, fy, kz);
+                            out = make_float2((float)((double)sv.x * m), (float)((double)sv.y * m));
+                        }
                     }
+                    lds[e] = out;
                 }
-                const float2 sv = lds[e];
-                lds[e] = make_float2((float)((double)sv.x * m), (float)((double)sv.y * m));
             }
         } else {
-            // a thread's elements all lie in column threadIdx.x mod kTile
-            TvQ tq;
-            if constexpr (!Q_MEM) tq.init(V, min(kx0 + (int)(threadIdx.x & (kTile - 1)), nxh - 1), ky);
+            TvQ tq, tqB;
+            if constexpr (!Q_MEM) {
+                int fx, fy;
+                pk_freq(V, min(kx0 + c, nxh - 1), ky, fx, fy);
+                tq.init(V, fx, fy);
+                if (dcol) tqB.init(V, V.nx / 2, ky);
+            }
 #pragma unroll
             for (int t = 0; t < EMAX; ++t) {
                 const int e = threadIdx.x + t * kThreads;
                 if (e < total) {
+                    const int kz = e >> kLog2Tile;
                     const float2 sv = lds[e];
-                    float q;
-                    if constexpr (Q_MEM) q = qr[t];
-                    else q = tq.at(e >> kLog2Tile, nz, twl[e >> kLog2Tile]);
+                    float2 qs;  // q * spectrum
+                    if (dcol) {
+                        float qA, qB;
+                        if constexpr (Q_MEM) {
+                            qA = rts_q(V, 0, ky, kz);
+                            qB = rts_q(V, V.nx / 2, ky, kz);
+                        } else {
+                            qA = tq.at(kz, nz, twl[kz]);
+                            qB = tqB.at(kz, nz, twl[kz]);
+                        }
+                        qs = pk_apply(sv, zmir[t], qA, qB);
+                    } else {
+                        float q;
+                        if constexpr (Q_MEM) q = qr[t];
+                        else q = tq.at(kz, nz, twl[kz]);
+                        qs = make_float2(q * sv.x, q * sv.y);
+                    }
                     if constexpr (HAS_F)
-                        lds[e] = make_float2(fr[t].x - q * sv.x, fr[t].y - q * sv.y);
+                        lds[e] = make_float2(fr[t].x - qs.x, fr[t].y - qs.y);
                     else
-                        lds[e] = make_float2(-q * sv.x, -q * sv.y);
+                        lds[e] = make_float2(-qs.x, -qs.y);
                 }
             }
         }
@@ -1172,14 +1406,18 @@ __device__ __forceinline__ void c2r_load(float2* lds, int nx, int nxh, int RB,
         const int k = e - p * nx;
         const bool neg = (2 * k > nx);
         const int kk = neg ? nx - k : k;
+        // even nx: the Nyquist bin is the imaginary part of column 0
+        const bool nyq = 2 * kk == nx;
         // rows past the volume (or past the block for e >= total) read its
         // last row: in bounds, and their outputs are never stored (each
         // half of the packed row is Hermitian, so they do not mix)
         const int ra = min(row0 + step * p, nrows - 1);
         const int rb = min((step == 2) ? row0 + step * p + 1 : row0 + step * p, nrows - 1);
-        float2 A = ldg2(sa + kk + (size_t)ra * nxh);
-        float2 B = ldg2(sb + kk + (size_t)rb * nxh);
-        if (kk == 0 || 2 * kk == nx) { A.y = 0.0f; B.y = 0.0f; }
+        const int col = nyq ? 0 : kk;
+        float2 A = ldg2(sa + col + (size_t)ra * nxh);
+        float2 B = ldg2(sb + col + (size_t)rb * nxh);
+        if (nyq) { A.x = A.y; B.x = B.y; }
+        if (kk == 0 || nyq) { A.y = 0.0f; B.y = 0.0f; }
         if (neg) { A.y = -A.y; B.y = -B.y; }
         r[t] = make_float2(A.x - B.y, A.y + B.x);
     }
@@ -1216,7 +1454,7 @@ __global__ __launch_bounds__(kThreads) void k_c2r_tv(const VolDesc* __restrict__
     const VolDesc& V = vols[gv];
     if (status && status[gv].done) return;
     const int nx = N > 0 ? N : V.nx;
-    const int nxh = nx / 2 + 1;
+    const int nxh = (nx + 1) / 2;
     const int RB = N > 0 ? RBF : V.rows_c;
     const int nrows = V.ny * V.nz;
     const int row0 = lb * RB * 2;
@@ -1349,72 +1587,108 @@ __global__ __launch_bounds__(kThreads) void k_z_vs(const VolDesc* __restrict__ v
     run_fft<N, EMAX, true, kLog2Tile, false>(lds, V.pz, kTile, kTile, 1, twl);
     if constexpr (OP == ZOP_VS_FWD) {
         col_store<EMAX>(lds, buf, zs, nz, kx0, nxh);
-    } else if constexpr (OP == ZOP_VS_RADIUS) {
-        const float inv_count = (float)(1.0 / V.vs_counts[ridx]);
-        const float invN = (float)(1.0 / ((double)V.nx * V.ny * V.nz));
-        const QSM_GLOBAL float2* __restrict__ FH = glb(V.spec[SPEC_A]) + base;
-        const QSM_GLOBAL float2* __restrict__ MH = glb(V.spec[SPEC_B]) + base;
+    } else {
+        // column 0 of tile 0 at ky = 0, ny/2 holds two z spectra (the DC and
+        // Nyquist planes, even nx); the sphere's spectrum there is SA + i SB,
+        // both real, and the S^ of the Nyquist plane is kept after the half
+        // spectrum in vs_sfirst (2 nz entries)
+        const bool dpk = pk_double(V, tx, ky);
+        const int c = threadIdx.x & (kTile - 1);
+        const bool dcol = dpk && c == 0;
         QSM_GLOBAL float* sfirst = glb(V.vs_sfirst) + base;
-        float sreg[EMAX];
-        float2 fh[EMAX];
+        QSM_GLOBAL float* sfirst_nyq =
+            glb(V.vs_sfirst) + (size_t)nxh * V.ny * V.nz + (ky == 0 ? 0 : (size_t)nz);
+        if constexpr (OP == ZOP_VS_RADIUS) {
+            const float inv_count = (float)(1.0 / V.vs_counts[ridx]);
+            const float invN = (float)(1.0 / ((double)V.nx * V.ny * V.nz));
+            const QSM_GLOBAL float2* __restrict__ FH = glb(V.spec[SPEC_A]) + base;
+            const QSM_GLOBAL float2* __restrict__ MH = glb(V.spec[SPEC_B]) + base;
+            float2 qv[EMAX];  // F^ (1 - S^) / N, transformed after P
 #pragma unroll
-        for (int t = 0; t < EMAX; ++t) {
-            const int e = threadIdx.x + t * kThreads;
-            if (e < total) {
+            for (int t = 0; t < EMAX; ++t) {
+                const int e = threadIdx.x + t * kThreads;
+                if (e < total) {
+                    const int kz = e >> kLog2Tile;
+                    const int kx = kx0 + c;
+                    const float2 z = lds[e];
+                    if (kx < nxh) {
+                        const float2 m = ldg2(MH + kx + kz * zs);
+                        const float2 fh = ldg2(FH + kx + kz * zs);
+                        if (dcol) {
+                            const int mz = mirror_idx(kz, nz);
+                            const float sA = z.x * inv_count, sB = z.y * inv_count;
+                            if (flag) {
+                                sfirst[kx + kz * zs] = sA;
+                                sfirst_nyq[kz] = sB;
+                            }
+                            const float2 mm = ldg2(MH + kx + mz * zs), fm = ldg2(FH + kx + mz * zs);
+                            lds[e] = pk_apply(m, mm, sA * invN, sB * invN);
+                            qv[t] = pk_apply(fh, fm, (1.0f - sA) * invN, (1.0f - sB) * invN);
+                        } else {
+                            const float sv = z.x * inv_count;
+                            if (flag) sfirst[kx + kz * zs] = sv;
+                            lds[e] = make_float2(m.x * sv * invN, m.y * sv * invN);
+                            const float hp = (1.0f - sv) * invN;
+                            qv[t] = make_float2(fh.x * hp, fh.y * hp);
+                        }
+                    } else {
+                        lds[e] = make_float2(0.0f, 0.0f);
+                        qv[t] = make_float2(0.0f, 0.0f);
+                    }
+                }
+            }
+            __syncthreads();
+            run_fft<N, EMAX, true, kLog2Tile, true>(lds, V.pz, kTile, kTile, 1, twl);
+            col_store<EMAX>(lds, buf, zs, nz, kx0, nxh);
+            __syncthreads();
+#pragma unroll
+            for (int t = 0; t < EMAX; ++t) {
+                const int e = threadIdx.x + t * kThreads;
+                if (e < total) lds[e] = qv[t];
+            }
+            __syncthreads();
+            run_fft<N, EMAX, true, kLog2Tile, true>(lds, V.pz, kTile, kTile, 1, twl);
+            col_store<EMAX>(lds, glb(V.spec[SPEC_C]) + base, zs, nz, kx0, nxh);
+        } else {  // ZOP_VS_DECONV
+            const double invN = 1.0 / ((double)V.nx * V.ny * V.nz);
+            // X / (1 - S^), 0 where |1 - S^| < threshold
+            auto deconv = [&](double oms) {
+                if (fabs(oms) < V.vs_threshold) return 0.0;
+                return flag ? invN / oms : (1.0 / oms) * invN;
+            };
+            float2 zmir[EMAX];
+            if (dpk) {
+                if (dcol) {
+#pragma unroll
+                    for (int t = 0; t < EMAX; ++t) {
+                        const int kz = min((int)(threadIdx.x + t * kThreads) >> kLog2Tile, nz - 1);
+                        zmir[t] = lds[mirror_idx(kz, nz) * kTile];
+                    }
+                }
+                __syncthreads();
+            }
+#pragma unroll
+            for (int t = 0; t < EMAX; ++t) {
+                const int e = threadIdx.x + t * kThreads;
                 const int kz = e >> kLog2Tile;
-                const int kx = kx0 + (e & (kTile - 1));
-                const float sv = lds[e].x * inv_count;
-                sreg[t] = sv;
-                if (kx < nxh) {
-                    if (flag) sfirst[kx + kz * zs] = sv;
-                    const float2 m = ldg2(MH + kx + kz * zs);
-                    fh[t] = ldg2(FH + kx + kz * zs);
-                    lds[e] = make_float2(m.x * sv * invN, m.y * sv * invN);
-                } else {
-                    fh[t] = make_float2(0.0f, 0.0f);
-                    lds[e] = make_float2(0.0f, 0.0f);
+                const int kx = kx0 + c;
+                if (e < total && kx < nxh) {
+                    const float2 xv = lds[e];
+                    const double oms = 1.0 - (double)sfirst[kx + kz * zs];
+                    float2 y;
+                    if (dcol) {
+                        y = pk_apply_d(xv, zmir[t], deconv(oms), deconv(1.0 - (double)sfirst_nyq[kz]));
+                    } else {
+                        const double sc = deconv(oms);
+                        y = make_float2((float)(xv.x * sc), (float)(xv.y * sc));
+                    }
+                    lds[e] = y;
                 }
             }
+            __syncthreads();
+            run_fft<N, EMAX, true, kLog2Tile, true>(lds, V.pz, kTile, kTile, 1, twl);
+            col_store<EMAX>(lds, buf, zs, nz, kx0, nxh);
         }
-        __syncthreads();
-        run_fft<N, EMAX, true, kLog2Tile, true>(lds, V.pz, kTile, kTile, 1, twl);
-        col_store<EMAX>(lds, buf, zs, nz, kx0, nxh);
-        __syncthreads();
-#pragma unroll
-        for (int t = 0; t < EMAX; ++t) {
-            const int e = threadIdx.x + t * kThreads;
-            if (e < total) {
-                const float hp = (1.0f - sreg[t]) * invN;
-                lds[e] = make_float2(fh[t].x * hp, fh[t].y * hp);
-            }
-        }
-        __syncthreads();
-        run_fft<N, EMAX, true, kLog2Tile, true>(lds, V.pz, kTile, kTile, 1, twl);
-        col_store<EMAX>(lds, glb(V.spec[SPEC_C]) + base, zs, nz, kx0, nxh);
-    } else {  // ZOP_VS_DECONV
-        const double invN = 1.0 / ((double)V.nx * V.ny * V.nz);
-        for (int e = threadIdx.x; e < total; e += kThreads) {
-            const int kz = e >> kLog2Tile, c = e & (kTile - 1);
-            const int kx = kx0 + c;
-            if (kx < nxh) {
-                const double oms = 1.0 - (double)glb(V.vs_sfirst)[base + kx + kz * zs];
-                const float2 xv = lds[e];
-                float2 y;
-                if (fabs(oms) < V.vs_threshold) {
-                    y = make_float2(0.0f, 0.0f);
-                } else if (flag) {
-                    const double sc = invN / oms;
-                    y = make_float2((float)(xv.x * sc), (float)(xv.y * sc));
-                } else {
-                    const double sc = (1.0 / oms) * invN;
-                    y = make_float2((float)(xv.x * sc), (float)(xv.y * sc));
-                }
-                lds[e] = y;
-            }
-        }
-        __syncthreads();
-        run_fft<N, EMAX, true, kLog2Tile, true>(lds, V.pz, kTile, kTile, 1, twl);
-        col_store<EMAX>(lds, buf, zs, nz, kx0, nxh);
     }
 }
 
@@ -1439,7 +1713,7 @@ __global__ __launch_bounds__(kThreads) void k_c2r_vs_select(const VolDesc* __res
     const int gv = locate(lt, bid, lb);
     const VolDesc& V = vols[gv];
     const int nx = N > 0 ? N : V.nx;
-    const int nxh = nx / 2 + 1;
+    const int nxh = (nx + 1) / 2;
     const int RB = N > 0 ? qsmfft::fixed_rows(N) : V.rows_x;
     const int nrows = V.ny * V.nz;
     const int row0 = lb * RB;
@@ -1486,7 +1760,7 @@ __global__ __launch_bounds__(kThreads) void k_c2r_vs_out(const VolDesc* __restri
     const int gv = locate(lt, bid, lb);
     const VolDesc& V = vols[gv];
     const int nx = N > 0 ? N : V.nx;
-    const int nxh = nx / 2 + 1;
+    const int nxh = (nx + 1) / 2;
     const int RB = N > 0 ? qsmfft::fixed_rows(N) : V.rows_x;
     const int nrows = V.ny * V.nz;
     const int row0 = lb * RB * 2;

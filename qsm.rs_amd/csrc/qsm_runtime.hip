@@ -158,7 +158,7 @@ static int default_rows_x(int nx) { return std::max(1, std::min(32, (qsmfft::kTh
 template <class TB>
 PassBlocks fill_geometry(VolDesc& V, const qsm_grid& g, TB& tb, int xshift = 0, int cshift = 0) {
     V.nx = (int)g.nx; V.ny = (int)g.ny; V.nz = (int)g.nz;
-    V.nxh = V.nx / 2 + 1;
+    V.nxh = (V.nx + 1) / 2;  // even nx: DC and Nyquist share column 0
     V.rows_x = default_rows_x(V.nx) >> xshift;
     V.rows_c = std::max(1, (default_rows_x(V.nx) >> cshift) / 2);
     V.ntx = (V.nxh + kTile - 1) / kTile;
@@ -193,17 +193,22 @@ using Groups = std::vector<PassGroup>;
 
 void launch_tv_reduce(hipStream_t st, const VolDesc* d, int nvol, VolStatus* status, VolStatus* mirror);
 
-Groups make_groups(const std::vector<int>& len, const std::vector<int>& blocks, bool column) {
+// shift (x passes): each volume's row shift; volumes of one length with
+// different shifts run in separate groups.
+Groups make_groups(const std::vector<int>& len, const std::vector<int>& blocks, bool column,
+                   const std::vector<int>* shift = nullptr) {
     Groups gs;
     for (size_t v = 0; v < len.size(); ++v) {
+        const int sh = shift ? (*shift)[v] : 0;
         PassGroup* g = nullptr;
         for (auto& x : gs)
-            if (x.n == len[v]) g = &x;
+            if (x.n == len[v] && x.xshift == sh) g = &x;
         if (!g) {
             gs.push_back(PassGroup{});
             g = &gs.back();
             g->n = len[v];
             g->emax = column ? emax_for_col(len[v]) : 16;
+            g->xshift = sh;
             g->lt.nvol = 0;
             g->lt.start[0] = 0;
         }
@@ -453,33 +458,25 @@ static int tv_batch_create_impl(const qsm_tv_volume* vols, size_t n_vols, const 
     TwiddleBank tb;
     std::vector<int> lx, ly, lz, bx, bc, by, bz, bo;
     auto cleanup = [&](int rc) { batch_free(B); return rc; };
-    // A compiled x length whose R2C launch over the batch would have fewer
-    // than 1536 blocks (one small volume: 128 blocks for 128x128x64 against
-    // ~768 resident) runs that pass with a quarter of the rows per block, and
-    // its TV C2R pass likewise below 1024 blocks, where either keeps at least
-    // 4 pair-rows per block (fewer made the C2R pass slower, measured).
-    std::map<int, int> xshift, cshift;
-    {
-        std::map<int, long> xblocks, cblocks;
-        for (size_t v = 0; v < n_vols; ++v) {
-            const int nx = (int)vols[v].grid.nx;
-            const long npairs = ((long)vols[v].grid.ny * (long)vols[v].grid.nz + 1) / 2;
-            const int rx = default_rows_x(nx), rc = std::max(1, rx / 2);
-            xblocks[nx] += (npairs + rx - 1) / rx;
-            cblocks[nx] += (npairs + rc - 1) / rc;
-        }
-        for (const auto& kv : xblocks) {
-            const int nx = kv.first, r0 = default_rows_x(nx);
-            const bool fixed = has_fixed_kernels(nx);
-            xshift[nx] = (fixed && kv.second < 1536 && (r0 >> 2) >= 4) ? 2 : 0;
-            cshift[nx] = (fixed && cblocks[nx] < 1024 && (r0 >> 2) / 2 >= 4) ? 2 : 0;
-        }
+    // A small volume of a compiled x length, whose R2C pass would launch
+    // fewer than 1024 blocks (128 for 128x128x64 against ~768 resident), runs
+    // that pass with a quarter of the rows per block, and its TV C2R pass
+    // likewise, where either keeps at least 4 pair-rows per block (fewer made
+    // the C2R pass slower, measured).  The choice depends on the volume's own
+    // shape only: the two forms round differently in the last bits, and a
+    // volume must give the same bits alone and in any batch.
+    std::vector<int> xshift(n_vols), cshift(n_vols);
+    for (size_t v = 0; v < n_vols; ++v) {
+        const int nx = (int)vols[v].grid.nx, r0 = default_rows_x(nx), rc = std::max(1, r0 / 2);
+        const long npairs = ((long)vols[v].grid.ny * (long)vols[v].grid.nz + 1) / 2;
+        const bool fixed = has_fixed_kernels(nx);
+        xshift[v] = (fixed && (npairs + r0 - 1) / r0 < 1024 && (r0 >> 2) >= 4) ? 2 : 0;
+        cshift[v] = (fixed && (npairs + rc - 1) / rc < 1024 && (r0 >> 2) / 2 >= 4) ? 2 : 0;
     }
     for (size_t v = 0; v < n_vols; ++v) {
         VolDesc& V = B->h[v];
         std::memset(&V, 0, sizeof V);
-        const int nxv = (int)vols[v].grid.nx;
-        PassBlocks pb = fill_geometry(V, vols[v].grid, tb, xshift[nxv], cshift[nxv]);
+        PassBlocks pb = fill_geometry(V, vols[v].grid, tb, xshift[v], cshift[v]);
         const double* b = vols[v].bdir;
         double bn = std::sqrt(b[0] * b[0] + b[1] * b[1] + b[2] * b[2]);
         V.bx = b[0] / bn; V.by = b[1] / bn; V.bz = b[2] / bn;  // src/kernels/dipole.rs:39-43
@@ -519,10 +516,8 @@ static int tv_batch_create_impl(const qsm_tv_volume* vols, size_t n_vols, const 
         if ((rc = dev_alloc(&V.partial, 2 * (size_t)pb.cpass))) return cleanup(rc);
         B->allocs.push_back(V.partial);
     }
-    B->gx = make_groups(lx, bx, false);
-    B->gxc = make_groups(lx, bc, false);
-    for (auto& g : B->gx) g.xshift = xshift[g.n];
-    for (auto& g : B->gxc) g.xshift = cshift[g.n];
+    B->gx = make_groups(lx, bx, false, &xshift);
+    B->gxc = make_groups(lx, bc, false, &cshift);
     B->gy = make_groups(ly, by, true);
     B->gz = make_groups(lz, bz, true);
     B->lt_out = identity_table(bo);
@@ -1187,7 +1182,8 @@ static int vs_create_impl(const qsm_vsharp_volume* vols, size_t n_vols, const do
             if ((rc = dev_alloc(&V.spec[sidx], Nh))) return cleanup(rc);
             B->allocs.push_back(V.spec[sidx]);
         }
-        if ((rc = dev_alloc(&V.vs_sfirst, Nh))) return cleanup(rc);
+        // S^ of the largest sphere; 2 nz more for the Nyquist plane's doubly packed columns
+        if ((rc = dev_alloc(&V.vs_sfirst, Nh + 2 * (size_t)V.nz))) return cleanup(rc);
         B->allocs.push_back(V.vs_sfirst);
         if ((rc = dev_alloc(&V.vs_local, N))) return cleanup(rc);
         B->allocs.push_back(V.vs_local);

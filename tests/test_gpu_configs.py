@@ -21,6 +21,14 @@ from conftest import nrmse_dc
 pytestmark = pytest.mark.gpu
 
 
+def _progress(tag, every):
+    """Progress callback for a long oracle run: prints every `every` steps."""
+    def cb(cur, total):
+        if cur % every == 0 or cur == total:
+            print(f"{tag}: oracle {cur}/{total}", flush=True)
+    return cb
+
+
 def _report(tag, x, xo, mask, oracle):
     m = np.asarray(mask).ravel() > 0
     raw = float(oracle.nrmse(x, xo, mask))
@@ -62,7 +70,7 @@ def test_tv_200_iterations_256x256x160(qsm, oracle, phantom):
     x, it, _ = qsm.tv_admm(ph.local_field, ph.mask, grid, ph.bdir, p, return_info=True)
     print("gpu solve done; oracle running", flush=True)
     xo, oit, _ = oracle.tv_admm(ph.local_field, ph.mask, ph.dims, ph.voxel_size, ph.bdir, 1e-3, 0.1,
-                                0.0, 200, return_info=True)
+                                0.0, 200, progress=_progress("256x256x160", 25), return_info=True)
     assert it == oit == 200
     raw, dc, _ = _report("256x256x160 x200", x, xo, ph.mask, oracle)
     assert raw <= 1e-3
@@ -104,7 +112,7 @@ def test_vsharp_then_tv_384x384x256(qsm, oracle, chain_phantom):
     local, emask = tl.numpy(), te.numpy()
     print("gpu V-SHARP done; oracle running", flush=True)
     lo, emo = oracle.vsharp(ph.total_field, ph.mask, CHAIN_DIMS, CHAIN_VS, vp.threshold,
-                            vp.max_radius, vp.min_radius)
+                            vp.max_radius, vp.min_radius, progress=_progress("V-SHARP radius", 1))
     assert np.array_equal(emask, emo), int(np.sum(emask != emo))
     assert int(emo.sum()) > 0
     raw, _, _ = _report("V-SHARP 384x384x256", local, lo, emo, oracle)
@@ -114,7 +122,8 @@ def test_vsharp_then_tv_384x384x256(qsm, oracle, chain_phantom):
     p = qsm.TvParams(lambda_=1e-3, rho=0.1, tol=0.0, max_iter=20)
     x = qsm.tv_admm(lo, emo, grid, ph.bdir, p)
     print("gpu TV done; oracle running", flush=True)
-    xo = oracle.tv_admm(lo, emo, CHAIN_DIMS, CHAIN_VS, ph.bdir, 1e-3, 0.1, 0.0, 20)
+    xo = oracle.tv_admm(lo, emo, CHAIN_DIMS, CHAIN_VS, ph.bdir, 1e-3, 0.1, 0.0, 20,
+                        progress=_progress("TV 384x384x256", 5))
     raw, dc, _ = _report("TV x20 on V-SHARP output 384x384x256", x, xo, emo, oracle)
     assert raw <= 1e-3
     assert dc <= 1e-4
