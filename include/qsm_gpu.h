@@ -345,6 +345,17 @@ int qsm_dipole_forward(const double* d_chi, const qsm_grid* grid, const double b
                        void* stream);
 int qsm_dipole_adjoint(const double* d_field, const qsm_grid* grid, const double bdir[3], double* d_chi,
                        void* stream);
+/* qsm_dipole_forward without any device allocation, for callers that cannot
+ * allocate (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc).  No counterpart in the reference.  The workspace (one
+ * half spectrum, at most 72 MiB: e.g. 256x256x256) is a fixed device array
+ * that the library holds on each device until the process ends; larger
+ * volumes return QSM_EPARAM.  Every call on a device shares that array, so
+ * calls that overlap on one device race; each call blocks until its passes
+ * are done.  Same kernels and results as qsm_dipole_forward; stream may be
+ * null (the null stream). */
+int qsm_dipole_forward_noalloc(const double* d_chi, const qsm_grid* grid, const double bdir[3],
+                               double* d_field, void* stream);
 
 /* ---- NIfTI-1 input/output, src/io.rs ---------------------------------- */
 /* NiftiData, src/io.rs:13-26 (dims, voxel size, affine, scaling) plus the

@@ -194,6 +194,8 @@ _SIGS = {
     "qsm_tv_unrolled_destroy": (None, [C.c_void_p]),
     "qsm_dipole_forward": (C.c_int, [C.c_void_p, C.POINTER(_CGrid), _D, C.c_void_p, C.c_void_p]),
     "qsm_dipole_adjoint": (C.c_int, [C.c_void_p, C.POINTER(_CGrid), _D, C.c_void_p, C.c_void_p]),
+    "qsm_dipole_forward_noalloc": (C.c_int, [C.c_void_p, C.POINTER(_CGrid), _D, C.c_void_p,
+                                             C.c_void_p]),
     "qsm_device_alloc": (C.c_int, [_SZ, C.POINTER(C.c_void_p)]),
     "qsm_device_free": (C.c_int, [C.c_void_p]),
     "qsm_memcpy_htod": (C.c_int, [C.c_void_p, C.c_void_p, _SZ]),

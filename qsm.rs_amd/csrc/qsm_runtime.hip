@@ -1532,6 +1532,83 @@ int qsm_dipole_adjoint(const double* d_field, const qsm_grid* grid, const double
     return spectral_apply(P_Z_DIPOLE, 0, 0.0, d_field, nullptr, grid, bdir, d_chi, stream);
 }
 
+// ------------------------------------- dipole operator, no allocation ----
+// Workspace of qsm_dipole_forward_noalloc in a file-scope device array: the
+// volume descriptor, the twiddle tables (three lengths <= 512) and one half
+// spectrum.  Each device holds its own copy until the process ends.  Host
+// code never passes `scratch` itself to a kernel or a copy: its device
+// address comes from hipGetSymbolAddress, once per device.
+namespace {
+constexpr size_t kScratchDescBytes = 4096;
+constexpr size_t kScratchTwBytes = sizeof(float2) * FixedTwiddleBank::kCap;
+constexpr size_t kScratchSpecOff = 65536;
+constexpr size_t kScratchSpecBytes = (size_t)72 << 20;
+constexpr size_t kScratchBytes = kScratchSpecOff + kScratchSpecBytes;
+static_assert(sizeof(VolDesc) <= kScratchDescBytes, "descriptor does not fit its scratch slot");
+static_assert(kScratchDescBytes + kScratchTwBytes <= kScratchSpecOff, "twiddles overlap the spectrum");
+constexpr int kScratchMaxDevices = 64;
+std::mutex g_scratch_mu;
+void* g_scratch_ptr[kScratchMaxDevices] = {};
+}  // namespace
+
+static __device__ __align__(256) unsigned char scratch[kScratchBytes];
+
+// Device address of `scratch` on the current device (looked up on the first
+// call on each device).
+static int scratch_address(unsigned char** out) {
+    int dev = 0;
+    QSM_HIP(hipGetDevice(&dev));
+    if (dev < 0 || dev >= kScratchMaxDevices) return fail(QSM_EHIP, "device index %d out of range", dev);
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (!g_scratch_ptr[dev]) {
+        void* p = nullptr;
+        hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p)
+            return fail(QSM_EHIP, "hipGetSymbolAddress(scratch): %s", hipGetErrorString(e));
+        g_scratch_ptr[dev] = p;
+    }
+    *out = static_cast<unsigned char*>(g_scratch_ptr[dev]);
+    return QSM_OK;
+}
+
+int qsm_dipole_forward_noalloc(const double* d_chi, const qsm_grid* grid, const double bdir[3],
+                               double* d_field, void* stream) {
+    g_err.clear();
+    if (!d_chi || !grid || !bdir || !d_field) return fail(QSM_EPARAM, "null argument");
+    int rc = validate_grid(grid);
+    if (rc) return rc;
+    const double bn = std::sqrt(bdir[0] * bdir[0] + bdir[1] * bdir[1] + bdir[2] * bdir[2]);
+    if (!(bn > 0)) return fail(QSM_EPARAM, "B0 direction has zero length");
+    VolDesc V;
+    std::memset(&V, 0, sizeof V);
+    FixedTwiddleBank tb;
+    PassBlocks pb = fill_geometry(V, *grid, tb);
+    const size_t spec_bytes = (size_t)V.nxh * V.ny * V.nz * sizeof(float2);
+    if (spec_bytes > kScratchSpecBytes)
+        return fail(QSM_EPARAM,
+                    "volume %zux%zux%zu needs %zu bytes of spectrum workspace; the fixed scratch of "
+                    "qsm_dipole_forward_noalloc holds %zu (use qsm_dipole_forward)",
+                    grid->nx, grid->ny, grid->nz, spec_bytes, kScratchSpecBytes);
+    unsigned char* base = nullptr;
+    if ((rc = scratch_address(&base))) return rc;
+    VolDesc* d_vols = reinterpret_cast<VolDesc*>(base);
+    float2* d_tw = reinterpret_cast<float2*>(base + kScratchDescBytes);
+    V.bx = bdir[0] / bn; V.by = bdir[1] / bn; V.bz = bdir[2] / bn;
+    V.field = d_chi;
+    V.adj_out = d_field;
+    V.out = d_field;
+    V.spec[SPEC_WORK] = reinterpret_cast<float2*>(base + kScratchSpecOff);
+    hipStream_t st = (hipStream_t)stream;
+    // The R2C pass writes the whole spectrum, so nothing needs clearing.
+    QSM_HIP(hipMemcpyAsync(d_vols, &V, sizeof V, hipMemcpyHostToDevice, st));
+    QSM_HIP(hipMemcpyAsync(d_tw, tb.host, tb.count * sizeof(float2), hipMemcpyHostToDevice, st));
+    enqueue_round_trip(P_R2C_F64, P_Z_DIPOLE, V, pb, d_vols, d_tw, st);
+    QSM_CHECK_LAUNCH();
+    // the host-side tables above live on this stack frame: wait before returning
+    QSM_HIP(hipStreamSynchronize(st));
+    return QSM_OK;
+}
+
 // Host-array wrapper of a closed-form inversion (filter kind, parameter).
 static int filter_host(int filt, double fa, const double* field, const uint8_t* mask,
                        const qsm_grid* grid, const double bdir[3], double* chi_out,
