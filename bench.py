@@ -87,27 +87,53 @@ def traffic_for(kernel_index, nvol, dims):
     """Measured HBM bytes per launch of one iteration kernel, from the
     committed rocprofv3 PMC summary (256x256x160, one volume; scaled by the
     batch)."""
-    names = ["k_r2c_x (TVDIV)", "k_y_pass", "k_z_tv (SOLVE)", "k_y_pass", "k_c2r_tv"]
+    names = ["k_r2c_x", "k_y_pass", "k_z_tv", "k_y_pass", "k_c2r_tv"]
     if tuple(dims) != (256, 256, 160) or kernel_index >= len(names) or not os.path.exists(TRAFFIC_FILE):
         return None
     for line in open(TRAFFIC_FILE):
-        if line.startswith(names[kernel_index]):
-            hbm_mb = float(line[len(names[kernel_index]):].split()[3])
-            return int(hbm_mb * 1e6 * nvol)
+        f = line.split()
+        if f and f[0] == names[kernel_index]:
+            return int(float(f[4]) * 1e6 * nvol)   # columns: name, dispatches, fetch, write, HBM MB
     return None
 
 
-def vsharp_alg_bytes(dims, R):
-    """Algorithmic bytes of one V-SHARP run with R radii: FFT(field),
-    FFT(mask), per radius FFT(kernel) and two C2R with the fused multiply and
-    select, then the deconvolution R2C + C2R and the mask pass:
-    (3R+4) U + R (8Nh + 18N) + 4Nh + 9N with U = 4N + 40Nh (one 3-pass
-    real transform)."""
+def vsharp_kernel_bytes(dims):
+    """Algorithmic HBM bytes per volume of each V-SHARP launch as built
+    (every input read once, every output written once; Nh = ceil(nx/2) ny nz
+    complex values of 8 bytes).  Keys are the launch kinds; "radius" kinds
+    run once per radius."""
     nx, ny, nz = dims
     N = nx * ny * nz
     Nh = ((nx + 1) // 2) * ny * nz
-    U = 4 * N + 40 * Nh
-    return (3 * R + 4) * U + R * (8 * Nh + 18 * N) + 4 * Nh + 9 * N
+    return {
+        # setup: F^ = FFT(field) and M^ = FFT(mask), each R2C + y + z
+        "r2c_f64": 8 * N + 8 * Nh, "r2c_u8": N + 8 * Nh,
+        "y_fwd": 16 * Nh, "z_fwd": 16 * Nh,
+        "vs_init": 5 * N,                      # clear the selected field (f32) and mask (u8)
+        # per radius: sphere generated in the R2C pass (nothing read)
+        "radius_r2c_sphere": 8 * Nh,
+        "radius_y_fwd": 16 * Nh,
+        # z: S^ in, F^ and M^ read, P = M^ S^ and Q = F^ (1 - S^) written
+        "radius_z": 40 * Nh,
+        "radius_y_inv2": 32 * Nh,              # inverse y of P and Q in one launch
+        "radius_c2r_select": 16 * Nh + N,      # P + iQ rows, the final-mask flags
+        "select_writes": 5 * N,                # each voxel's value and flag written once over all radii
+        "sfirst": 4 * Nh,                      # S^ of the largest radius kept (first radius)
+        # deconvolution with the largest kernel
+        "r2c_f32": 4 * N + 8 * Nh, "deconv_y_fwd": 16 * Nh, "z_deconv": 20 * Nh,
+        "deconv_y_inv": 16 * Nh, "c2r_out": 8 * Nh + 10 * N,
+    }
+
+
+def vsharp_alg_bytes(dims, R):
+    """Algorithmic bytes of one V-SHARP run with R radii, as built:
+    FFT(field) and FFT(mask) once, per radius the sphere's R2C / y / z
+    (P and Q) / inverse y of both / one C2R of P + iQ with the selection,
+    then the deconvolution round trip and the output pass."""
+    k = vsharp_kernel_bytes(dims)
+    per_radius = sum(v for n, v in k.items() if n.startswith("radius_"))
+    once = sum(v for n, v in k.items() if not n.startswith("radius_"))
+    return once + R * per_radius
 
 
 def host_cpu_info():
