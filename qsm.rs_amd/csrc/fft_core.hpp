@@ -26,6 +26,10 @@ constexpr int kMaxStages = 10;
 constexpr int kThreads = 256;  // threads per workgroup for every FFT kernel
 
 // Per-length plan.  Stored in device memory; read with uniform loads.
+// A length with a prime factor above 13 runs as a chirp-z (Bluestein)
+// transform: bl_n is then the axis length, n the convolution length M the
+// stages run at, bl_chirp / bl_filt the offsets of its chirp and filter
+// tables (see bluestein_fft); bl_n = 0 for a direct transform.
 struct Plan {
     int n;
     int nstages;
@@ -34,6 +38,7 @@ struct Plan {
     uint64_t m_nb[kMaxStages];   // magic for division by n / radix[s]
     uint64_t m_ns[kMaxStages];   // magic for division by ns[s]
     int tw_off;  // offset (in float2) of this length's table exp(-2 pi i q/n), q<n
+    int bl_n, bl_chirp, bl_filt;
 };
 
 __host__ __device__ __forceinline__ float2 cmul(float2 a, float2 b) {
@@ -358,6 +363,114 @@ __device__ __forceinline__ void lds_fft(float2* lds, const Plan& P, int ncols, i
         }
 #undef QSM_STAGE
     }
+}
+#endif
+
+// ------------------------------------------------------------ Bluestein --
+// Chirp-z transform of length n through a circular convolution of length M
+// (M >= 2n - 1, a length the stages handle).  With c[j] = exp(-i pi j^2/n),
+// j k = (j^2 + k^2 - (k - j)^2) / 2 gives
+//   X[k] = sum_j x[j] exp(-2 pi i j k / n) = c[k] sum_j (x[j] c[j]) conj(c[k - j]),
+// so X = c . IFFT_M(FFT_M(x c, zero-padded) . B) / M with B = FFT_M of
+// conj(c[m]) placed at m and M - m (Bluestein 1970).  The bank holds c (n
+// entries) and B / M (M entries); the inverse transform is
+// conj(forward(conj(x))).  Element (e, c) is at lds[e*es + c*cs] in the
+// caller's layout (e < n) and at lds[e*wes + c*wcs] while working (e < M):
+// column tiles keep their layout with M rows, rows are respaced to stride M.
+// Each phase is a per-thread function so the host emulation can run it.
+struct BlGeom {
+    int n, M, ncols, es, cs, wes, wcs;
+};
+__host__ __device__ inline BlGeom bl_geom(const Plan& P, int ncols, int es, int cs, bool colfast) {
+    BlGeom g;
+    g.n = P.bl_n; g.M = P.n; g.ncols = ncols; g.es = es; g.cs = cs;
+    g.wes = colfast ? es : 1;
+    g.wcs = colfast ? cs : P.n;
+    return g;
+}
+constexpr int kBlRegs = 16;  // n * ncols <= kBlRegs * kThreads (n <= M / 2, ncols * M <= 8192)
+
+__host__ __device__ inline void bl_gather(const float2* lds, int tid, const BlGeom& g, bool working,
+                                          float2 (&r)[kBlRegs]) {
+    const int es = working ? g.wes : g.es, cs = working ? g.wcs : g.cs;
+    for (int q = 0; q < kBlRegs; ++q) {
+        const int idx = tid + q * kThreads;
+        if (idx < g.n * g.ncols) {
+            const int c = idx / g.n, e = idx - c * g.n;
+            r[q] = lds[e * es + c * cs];
+        }
+    }
+}
+// x c (conj(x) c for the inverse) into the working layout, zeros at e >= n
+template <bool INV>
+__host__ __device__ inline void bl_scatter_in(float2* lds, int tid, const BlGeom& g,
+                                              const float2 (&r)[kBlRegs], const float2* chirp) {
+    for (int q = 0; q < kBlRegs; ++q) {
+        const int idx = tid + q * kThreads;
+        if (idx < g.n * g.ncols) {
+            const int c = idx / g.n, e = idx - c * g.n;
+            lds[e * g.wes + c * g.wcs] = cmul(INV ? cconj(r[q]) : r[q], chirp[e]);
+        }
+    }
+    const int pad = g.M - g.n;
+    for (int idx = tid; idx < pad * g.ncols; idx += kThreads) {
+        const int c = idx / pad, e = g.n + (idx - c * pad);
+        lds[e * g.wes + c * g.wcs] = make_float2(0.0f, 0.0f);
+    }
+}
+__host__ __device__ inline void bl_filter(float2* lds, int tid, const BlGeom& g, const float2* filt) {
+    for (int idx = tid; idx < g.M * g.ncols; idx += kThreads) {
+        const int c = idx / g.M, e = idx - c * g.M;
+        float2* p = lds + e * g.wes + c * g.wcs;
+        *p = cmul(*p, filt[e]);
+    }
+}
+// c . conv (conjugated for the inverse) back to the caller's layout
+template <bool INV>
+__host__ __device__ inline void bl_scatter_out(float2* lds, int tid, const BlGeom& g,
+                                               const float2 (&r)[kBlRegs], const float2* chirp) {
+    for (int q = 0; q < kBlRegs; ++q) {
+        const int idx = tid + q * kThreads;
+        if (idx < g.n * g.ncols) {
+            const int c = idx / g.n, e = idx - c * g.n;
+            const float2 v = cmul(r[q], chirp[e]);
+            lds[e * g.es + c * g.cs] = INV ? cconj(v) : v;
+        }
+    }
+}
+
+#ifdef __HIPCC__
+// The chirp-z transform on the device.  The LDS array must hold ncols * M
+// elements; twl holds the length-M twiddle table, twg is the bank in global
+// memory (chirp and filter tables).  Returns after a final barrier.
+template <int EMAX, bool COLFAST, int LOG2C, bool INV>
+__device__ __noinline__ void bluestein_fft(float2* lds, const Plan& P, int ncols, int es, int cs,
+                                           const float2* twl, const float2* twg) {
+    const BlGeom g = bl_geom(P, ncols, es, cs, COLFAST);
+    const float2* chirp = twg + P.bl_chirp;
+    const float2* filt = twg + P.bl_filt;
+    const int tid = threadIdx.x;
+    float2 r[kBlRegs];
+    bl_gather(lds, tid, g, false, r);
+    __syncthreads();
+    bl_scatter_in<INV>(lds, tid, g, r, chirp);
+    __syncthreads();
+    lds_fft<EMAX, COLFAST, LOG2C, false>(lds, P, ncols, g.wes, g.wcs, twl);
+    bl_filter(lds, tid, g, filt);
+    __syncthreads();
+    lds_fft<EMAX, COLFAST, LOG2C, true>(lds, P, ncols, g.wes, g.wcs, twl);
+    bl_gather(lds, tid, g, true, r);
+    __syncthreads();
+    bl_scatter_out<INV>(lds, tid, g, r, chirp);
+    __syncthreads();
+}
+
+// Any runtime-planned length: direct stages or the chirp-z transform.
+template <int EMAX, bool COLFAST, int LOG2C, bool INV>
+__device__ __forceinline__ void lds_fft_any(float2* lds, const Plan& P, int ncols, int es, int cs,
+                                            const float2* twl, const float2* twg) {
+    if (P.bl_n) bluestein_fft<EMAX, COLFAST, LOG2C, INV>(lds, P, ncols, es, cs, twl, twg);
+    else lds_fft<EMAX, COLFAST, LOG2C, INV>(lds, P, ncols, es, cs, twl);
 }
 #endif
 

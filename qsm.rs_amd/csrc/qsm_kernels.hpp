@@ -68,34 +68,52 @@ __host__ __device__ constexpr int col_lds2(int n, int eg) {
     return qsmfft::col2_ok(n) ? 16 * n + 256 : col_lds(n, eg);
 }
 
+// twl: the pass's twiddle table in LDS; twg: the twiddle bank in global
+// memory (the chirp-z tables of runtime-planned lengths).
 template <int N, int EMAX, bool COLFAST, int LOG2C, bool INV, int NC = 0>
 __device__ __forceinline__ void run_fft(float2* lds, const qsmfft::Plan& P, int ncols, int es, int cs,
-                                        const float2* twl) {
+                                        const float2* twl, const float2* twg) {
     if constexpr (N > 0)
         qsmfft::fixed_fft<N, NC ? NC : (COLFAST ? kTile : qsmfft::fixed_rows(N)), COLFAST, INV>(lds, twl);
     else
-        qsmfft::lds_fft<EMAX, COLFAST, LOG2C, INV>(lds, P, ncols, es, cs, twl);
+        qsmfft::lds_fft_any<EMAX, COLFAST, LOG2C, INV>(lds, P, ncols, es, cs, twl, twg);
 }
 
 // A pass's twiddle table (exp(-2 pi i q / n), q < n) is fetched into
 // registers before the pass issues its tile loads and written to LDS after
 // them: a table load whose value had to reach LDS first would hold every block
 // for one memory round trip before its tile loads went out.
+// The table of plan P (its length P.n: the axis length, or the chirp-z
+// convolution length).
 template <int N>
 struct TwStage {
     static constexpr int K = (tw_lds(N) + kThreads - 1) / kThreads;
     float2 r[K];
-    __device__ __forceinline__ void fetch(const float2* __restrict__ tw, int n) {
-        const QSM_GLOBAL float2* g = glb(tw);
+    int n;
+    __device__ __forceinline__ void fetch(const float2* __restrict__ tw, const qsmfft::Plan& P) {
+        n = N > 0 ? N : P.n;
+        const QSM_GLOBAL float2* g = glb(tw + P.tw_off);
 #pragma unroll
         for (int t = 0; t < K; ++t) r[t] = ldg2(g + min((int)threadIdx.x + t * kThreads, n - 1));
     }
     // threads past n hold entry n - 1 and write it again (no branch)
-    __device__ __forceinline__ void commit(float2* twl, int n) const {
+    __device__ __forceinline__ void commit(float2* twl) const {
 #pragma unroll
         for (int t = 0; t < K; ++t) twl[min((int)threadIdx.x + t * kThreads, n - 1)] = r[t];
     }
 };
+
+// exp(-2 pi i k / n) of a z pass: its twiddle table, except for a chirp-z
+// length, whose table is that of the convolution length.
+template <int N>
+__device__ __forceinline__ float2 ztw(const VolDesc& V, const float2* twl, int kz, int nz) {
+    if (N == 0 && V.pz.bl_n) {
+        float sn, cs;
+        sincospif(-2.0f * (float)kz / (float)nz, &sn, &cs);
+        return make_float2(cs, sn);
+    }
+    return twl[kz];
+}
 
 __device__ __forceinline__ float shrinkf(float x, float t) {
     // src/utils/mod.rs:44-52
@@ -762,7 +780,7 @@ __global__ __launch_bounds__(kThreads) void k_r2c_x(const VolDesc* __restrict__ 
     const int nrows = V.ny * V.nz;
     const int row0 = lb * RB * 2;
     TwStage<N> tws;
-    tws.fetch(tw + V.px.tw_off, nx);
+    tws.fetch(tw, V.px);
     if constexpr (SRC == SRC_TVDIV) {
         fill_tvdiv<EMAX, false>(lds, V, nx, RB, glb(V.x[aux]), glb(V.u[aux ^ 1]), glb(V.u[aux]), row0,
                                 nrows, nullptr);
@@ -789,9 +807,9 @@ __global__ __launch_bounds__(kThreads) void k_r2c_x(const VolDesc* __restrict__ 
     } else {
         fill_simple<EMAX, SRC>(lds, V, nx, RB, row0, nrows, aux);
     }
-    tws.commit(twl, nx);
+    tws.commit(twl);
     __syncthreads();
-    run_fft<N, EMAX, false, 0, false, RBX>(lds, V.px, RB, 1, nx, twl);
+    run_fft<N, EMAX, false, 0, false, RBX>(lds, V.px, RB, 1, nx, twl, tw);
     QSM_GLOBAL float2* __restrict__ out = glb(V.spec[out_sel]);
     const int total = RB * nxh;
 #pragma unroll
@@ -845,7 +863,7 @@ __global__ __launch_bounds__(kThreads) void k_y_pass(const VolDesc* __restrict__
     const int ny = N > 0 ? N : V.ny;
     QSM_GLOBAL float2* buf = glb(V.spec[s]) + (size_t)kz * nxh * ny;
     TwStage<N> tws;
-    tws.fetch(tw + V.py.tw_off, ny);
+    tws.fetch(tw, V.py);
     // tile 0 of an even-nx volume: column 0 is packed (DC and Nyquist planes)
     const bool pkc = tx == 0 && packed_x(V);
     if constexpr (TWO) {
@@ -853,7 +871,7 @@ __global__ __launch_bounds__(kThreads) void k_y_pass(const VolDesc* __restrict__
         const int c = threadIdx.x & 15, j = threadIdx.x >> 4;
         float2 v[16];
         l16_load<R0>(v, buf, nxh, kx0, nxh);
-        tws.commit(twl, ny);
+        tws.commit(twl);
         // column 0 of a packed tile goes through LDS (free before the
         // first exchange and after the last one)
         if constexpr (INV) {
@@ -894,10 +912,10 @@ __global__ __launch_bounds__(kThreads) void k_y_pass(const VolDesc* __restrict__
         lr_store<R0>(v, buf, nxh, kx0, nxh);
     } else {
         col_load<EMAX>(lds, buf, nxh, ny, kx0, nxh);
-        tws.commit(twl, ny);
+        tws.commit(twl);
         __syncthreads();
         if (INV && pkc) ypk_lds<true>(lds, ny);
-        run_fft<N, EMAX, true, kLog2Tile, INV>(lds, V.py, kTile, kTile, 1, twl);
+        run_fft<N, EMAX, true, kLog2Tile, INV>(lds, V.py, kTile, kTile, 1, twl, tw);
         if (!INV && pkc) ypk_lds<false>(lds, ny);
         col_store<EMAX>(lds, buf, nxh, ny, kx0, nxh);
     }
@@ -1133,7 +1151,7 @@ __device__ __forceinline__ void z_tv_col2(const VolDesc& V, float2* lds, float2*
     if constexpr (OP == ZOP_TV_SOLVE0) {
         // g = 0 before the first step (AdmmBuffers::new), so div = 0 and x^ = f^
         l16_load<R0>(v, glb(V.spec[SPEC_A]) + base, zs, kx0, nxh);
-        tws.commit(twl, N);
+        tws.commit(twl);
         C::template l16_to_lr<true>(v, lds, twl);
         lr_store<R0>(v, buf, zs, kx0, nxh);
         return;
@@ -1158,7 +1176,7 @@ __device__ __forceinline__ void z_tv_col2(const VolDesc& V, float2* lds, float2*
             if constexpr (Q_MEM) qr[r] = Q[(size_t)(R0 * r) * zs];
         }
     }
-    tws.commit(twl, N);
+    tws.commit(twl);
     C::template a_fwd<false>(threadIdx.x, v, lds);
     __syncthreads();
     if constexpr (!Q_MEM) {
@@ -1240,7 +1258,7 @@ __global__ __launch_bounds__(kThreads) void k_z_tv(const VolDesc* __restrict__ v
     const bool dpk = pk_double(V, tx, ky);
     QSM_GLOBAL float2* buf = glb(V.spec[SPEC_WORK]) + base;
     TwStage<N> tws;
-    tws.fetch(tw + V.pz.tw_off, nz);
+    tws.fetch(tw, V.pz);
     if constexpr (TWO) {
         z_tv_col2<N, OP>(V, lds, twl, tws, ky, kx0, dpk);
         return;
@@ -1248,9 +1266,9 @@ __global__ __launch_bounds__(kThreads) void k_z_tv(const VolDesc* __restrict__ v
     if constexpr (OP == ZOP_TV_SOLVE0) {
         // g = 0 before the first step (AdmmBuffers::new), so div = 0 and x^ = f^
         col_load<EMAX>(lds, glb(V.spec[SPEC_A]) + base, zs, nz, kx0, nxh);
-        tws.commit(twl, nz);
+        tws.commit(twl);
         __syncthreads();
-        run_fft<N, EMAX, true, kLog2Tile, true>(lds, V.pz, kTile, kTile, 1, twl);
+        run_fft<N, EMAX, true, kLog2Tile, true>(lds, V.pz, kTile, kTile, 1, twl, tw);
         col_store<EMAX>(lds, buf, zs, nz, kx0, nxh);
         return;
     }
@@ -1273,9 +1291,9 @@ __global__ __launch_bounds__(kThreads) void k_z_tv(const VolDesc* __restrict__ v
             if constexpr (Q_MEM) qr[t] = in ? Q[kx + kz * zs] : 0.0f;
         }
     }
-    tws.commit(twl, nz);
+    tws.commit(twl);
     __syncthreads();
-    run_fft<N, EMAX, true, kLog2Tile, false>(lds, V.pz, kTile, kTile, 1, twl);
+    run_fft<N, EMAX, true, kLog2Tile, false>(lds, V.pz, kTile, kTile, 1, twl, tw);
     // a thread's elements all lie in column c
     const int c = threadIdx.x & (kTile - 1);
     const bool dcol = dpk && c == 0;
@@ -1364,14 +1382,15 @@ __global__ __launch_bounds__(kThreads) void k_z_tv(const VolDesc* __restrict__ v
                             qA = rts_q(V, 0, ky, kz);
                             qB = rts_q(V, V.nx / 2, ky, kz);
                         } else {
-                            qA = tq.at(kz, nz, twl[kz]);
-                            qB = tqB.at(kz, nz, twl[kz]);
+                            const float2 tz = ztw<N>(V, twl, kz, nz);
+                            qA = tq.at(kz, nz, tz);
+                            qB = tqB.at(kz, nz, tz);
                         }
                         qs = pk_apply(sv, zmir[t], qA, qB);
                     } else {
                         float q;
                         if constexpr (Q_MEM) q = qr[t];
-                        else q = tq.at(kz, nz, twl[kz]);
+                        else q = tq.at(kz, nz, ztw<N>(V, twl, kz, nz));
                         qs = make_float2(q * sv.x, q * sv.y);
                     }
                     if constexpr (HAS_F)
@@ -1382,7 +1401,7 @@ __global__ __launch_bounds__(kThreads) void k_z_tv(const VolDesc* __restrict__ v
             }
         }
         __syncthreads();
-        run_fft<N, EMAX, true, kLog2Tile, true>(lds, V.pz, kTile, kTile, 1, twl);
+        run_fft<N, EMAX, true, kLog2Tile, true>(lds, V.pz, kTile, kTile, 1, twl, tw);
         col_store<EMAX>(lds, buf, zs, nz, kx0, nxh);
     }
 }
@@ -1461,12 +1480,12 @@ __global__ __launch_bounds__(kThreads) void k_c2r_tv(const VolDesc* __restrict__
     const int total = RB * nx;
     const QSM_GLOBAL float2* s = glb(V.spec[SPEC_WORK]);
     TwStage<N> tws;
-    tws.fetch(tw + V.px.tw_off, nx);
+    tws.fetch(tw, V.px);
     c2r_load<EMAX>(lds, nx, nxh, RB, s, s, row0, 2, nrows);
     if constexpr (MODE != C2R_TV) {
-        tws.commit(twl, nx);
+        tws.commit(twl);
         __syncthreads();
-        run_fft<N, EMAX, false, 0, true, RBF>(lds, V.px, RB, 1, nx, twl);
+        run_fft<N, EMAX, false, 0, true, RBF>(lds, V.px, RB, 1, nx, twl, tw);
 #pragma unroll
         for (int t = 0; t < EMAX; ++t) {
             const int e = threadIdx.x + t * kThreads;
@@ -1513,9 +1532,9 @@ __global__ __launch_bounds__(kThreads) void k_c2r_tv(const VolDesc* __restrict__
         xp[t].x = xo[i + (size_t)ra * nx];
         xp[t].y = xo[i + (size_t)rb * nx];
     }
-    tws.commit(twl, nx);
+    tws.commit(twl);
     __syncthreads();
-    run_fft<N, EMAX, false, 0, true, RBF>(lds, V.px, RB, 1, nx, twl);
+    run_fft<N, EMAX, false, 0, true, RBF>(lds, V.px, RB, 1, nx, twl, tw);
     double sd = 0.0, sx = 0.0;
 #pragma unroll
     for (int t = 0; t < EMAX; ++t) {
@@ -1580,11 +1599,11 @@ __global__ __launch_bounds__(kThreads) void k_z_vs(const VolDesc* __restrict__ v
     const int total = nz * kTile;
     QSM_GLOBAL float2* buf = glb(V.spec[OP == ZOP_VS_FWD ? sel : SPEC_WORK]) + base;
     TwStage<N> tws;
-    tws.fetch(tw + V.pz.tw_off, nz);
+    tws.fetch(tw, V.pz);
     col_load<EMAX>(lds, buf, zs, nz, kx0, nxh);
-    tws.commit(twl, nz);
+    tws.commit(twl);
     __syncthreads();
-    run_fft<N, EMAX, true, kLog2Tile, false>(lds, V.pz, kTile, kTile, 1, twl);
+    run_fft<N, EMAX, true, kLog2Tile, false>(lds, V.pz, kTile, kTile, 1, twl, tw);
     if constexpr (OP == ZOP_VS_FWD) {
         col_store<EMAX>(lds, buf, zs, nz, kx0, nxh);
     } else {
@@ -1638,7 +1657,7 @@ __global__ __launch_bounds__(kThreads) void k_z_vs(const VolDesc* __restrict__ v
                 }
             }
             __syncthreads();
-            run_fft<N, EMAX, true, kLog2Tile, true>(lds, V.pz, kTile, kTile, 1, twl);
+            run_fft<N, EMAX, true, kLog2Tile, true>(lds, V.pz, kTile, kTile, 1, twl, tw);
             col_store<EMAX>(lds, buf, zs, nz, kx0, nxh);
             __syncthreads();
 #pragma unroll
@@ -1647,7 +1666,7 @@ __global__ __launch_bounds__(kThreads) void k_z_vs(const VolDesc* __restrict__ v
                 if (e < total) lds[e] = qv[t];
             }
             __syncthreads();
-            run_fft<N, EMAX, true, kLog2Tile, true>(lds, V.pz, kTile, kTile, 1, twl);
+            run_fft<N, EMAX, true, kLog2Tile, true>(lds, V.pz, kTile, kTile, 1, twl, tw);
             col_store<EMAX>(lds, glb(V.spec[SPEC_C]) + base, zs, nz, kx0, nxh);
         } else {  // ZOP_VS_DECONV
             const double invN = 1.0 / ((double)V.nx * V.ny * V.nz);
@@ -1686,7 +1705,7 @@ __global__ __launch_bounds__(kThreads) void k_z_vs(const VolDesc* __restrict__ v
                 }
             }
             __syncthreads();
-            run_fft<N, EMAX, true, kLog2Tile, true>(lds, V.pz, kTile, kTile, 1, twl);
+            run_fft<N, EMAX, true, kLog2Tile, true>(lds, V.pz, kTile, kTile, 1, twl, tw);
             col_store<EMAX>(lds, buf, zs, nz, kx0, nxh);
         }
     }
@@ -1718,13 +1737,13 @@ __global__ __launch_bounds__(kThreads) void k_c2r_vs_select(const VolDesc* __res
     const int nrows = V.ny * V.nz;
     const int row0 = lb * RB;
     TwStage<N> tws;
-    tws.fetch(tw + V.px.tw_off, nx);
+    tws.fetch(tw, V.px);
     c2r_load<EMAX>(lds, nx, nxh, RB, glb(V.spec[SPEC_WORK]), glb(V.spec[SPEC_C]), row0, 1, nrows);
-    tws.commit(twl, nx);
+    tws.commit(twl);
     QSM_GLOBAL uint8_t* fin = glb(V.vs_final);
     QSM_GLOBAL float* loc = glb(V.vs_local);
     __syncthreads();
-    run_fft<N, EMAX, false, 0, true>(lds, V.px, RB, 1, nx, twl);
+    run_fft<N, EMAX, false, 0, true>(lds, V.px, RB, 1, nx, twl, tw);
     const double count = V.vs_counts[ridx];
     const float countf = (float)count;
     const double norm = 1.0 / count;
@@ -1766,11 +1785,11 @@ __global__ __launch_bounds__(kThreads) void k_c2r_vs_out(const VolDesc* __restri
     const int row0 = lb * RB * 2;
     const QSM_GLOBAL float2* s = glb(V.spec[SPEC_WORK]);
     TwStage<N> tws;
-    tws.fetch(tw + V.px.tw_off, nx);
+    tws.fetch(tw, V.px);
     c2r_load<EMAX>(lds, nx, nxh, RB, s, s, row0, 2, nrows);
-    tws.commit(twl, nx);
+    tws.commit(twl);
     __syncthreads();
-    run_fft<N, EMAX, false, 0, true>(lds, V.px, RB, 1, nx, twl);
+    run_fft<N, EMAX, false, 0, true>(lds, V.px, RB, 1, nx, twl, tw);
     const int total = RB * nx;
     const QSM_GLOBAL uint8_t* fin = glb(V.vs_final);
     QSM_GLOBAL double* out = glb(V.out);

@@ -3,6 +3,7 @@
 // entry points declared in include/qsm_gpu.h.
 #include "qsm_internal.hpp"
 #include "fft_col2.hpp"
+#include "fft_chirp.hpp"
 #include "../../include/qsm_gpu.h"
 
 #include <algorithm>
@@ -62,15 +63,24 @@ int fail(int code, const char* fmt, ...) {
 
 // -------------------------------------------------------------- planning --
 constexpr int kMaxLen = 512;
-
+using qsmfft::direct_length;
+using qsmfft::chirp_length;
+using qsmfft::chirp_tables;
+using qsmfft::kMaxChirpLen;
 bool length_ok(size_t n) {
     if (n < 1 || n > (size_t)kMaxLen) return false;
-    int r[qsmfft::kMaxStages], s;
-    return qsmfft::factorize((int)n, r, &s);
+    return direct_length((int)n) || ((int)n <= kMaxChirpLen && chirp_length((int)n) > 0);
 }
 
+// Length the FFT stages of an axis of length n run at (n, or the chirp-z
+// convolution length), which sizes the LDS tiles.
+int work_length(int n) { return direct_length(n) ? n : chirp_length(n); }
+
 // EMAX (elements per thread) needed for a column pass of length n.
-int emax_for_col(int n) { return n <= 256 ? 16 : (n <= 384 ? 24 : 32); }
+int emax_for_col(int n) {
+    const int w = work_length(n);
+    return w <= 256 ? 16 : (w <= 384 ? 24 : 32);
+}
 
 // exp(-2 pi i q/n) from an exactly reduced angle, rounded once
 inline float2 twiddle_entry(int q, int n) {
@@ -80,7 +90,7 @@ inline float2 twiddle_entry(int q, int n) {
 
 struct TwiddleBank {
     std::vector<float2> host;
-    std::map<int, int> off;
+    std::map<int, int> off, chirp_off, filt_off;
     int add(int n) {
         auto it = off.find(n);
         if (it != off.end()) return it->second;
@@ -89,14 +99,32 @@ struct TwiddleBank {
         off[n] = o;
         return o;
     }
+    // chirp (n entries) and filter (M entries) offsets of length n
+    void add_chirp(int n, int M, int& oc, int& of) {
+        auto it = chirp_off.find(n);
+        if (it != chirp_off.end()) {
+            oc = it->second;
+            of = filt_off[n];
+            return;
+        }
+        std::vector<float2> c, f;
+        chirp_tables(n, M, c, f);
+        oc = (int)host.size();
+        host.insert(host.end(), c.begin(), c.end());
+        of = (int)host.size();
+        host.insert(host.end(), f.begin(), f.end());
+        chirp_off[n] = oc;
+        filt_off[n] = of;
+    }
 };
 
-// The same tables for one volume (at most three lengths <= kMaxLen) in
-// fixed storage, for the entry point that allocates nothing.
+// The same tables for one volume (three axes, each at most a length-512
+// table, or a length-M table plus a chirp of n <= 256 and a filter of M)
+// in fixed storage, for the entry point that allocates nothing.
 struct FixedTwiddleBank {
-    static constexpr int kCap = 3 * kMaxLen;
+    static constexpr int kCap = 3 * (2 * kMaxLen + kMaxChirpLen);
     float2 host[kCap];
-    int len[3] = {0, 0, 0}, off[3] = {0, 0, 0};
+    int len[6] = {0, 0, 0, 0, 0, 0}, off[6] = {0, 0, 0, 0, 0, 0};
     int nlen = 0, count = 0;
     int add(int n) {
         for (int i = 0; i < nlen; ++i)
@@ -107,13 +135,38 @@ struct FixedTwiddleBank {
         off[nlen++] = o;
         return o;
     }
+    void add_chirp(int n, int M, int& oc, int& of) {
+        for (int i = 0; i < nlen; ++i)
+            if (len[i] == -n) {  // chirp tables are keyed by -n
+                oc = off[i];
+                of = oc + n;
+                return;
+            }
+        std::vector<float2> c, f;
+        chirp_tables(n, M, c, f);
+        oc = count;
+        for (const float2& v : c) host[count++] = v;
+        of = count;
+        for (const float2& v : f) host[count++] = v;
+        len[nlen] = -n;
+        off[nlen++] = oc;
+    }
 };
 
 template <class TB>
 qsmfft::Plan make_plan(int n, TB& tb) {
     qsmfft::Plan p{};
-    qsmfft::make_plan_stages(n, p);
-    p.tw_off = tb.add(n);
+    if (direct_length(n)) {
+        qsmfft::make_plan_stages(n, p);
+        p.tw_off = tb.add(n);
+        return p;
+    }
+    // chirp-z: the stages run at the convolution length
+    const int M = chirp_length(n);
+    qsmfft::make_plan_stages(M, p);
+    p.tw_off = tb.add(M);
+    p.bl_n = n;
+    tb.add_chirp(n, M, p.bl_chirp, p.bl_filt);
     return p;
 }
 
@@ -136,8 +189,9 @@ int validate_grid(const qsm_grid* g) {
     for (size_t d : dims)
         if (!length_ok(d))
             return fail(QSM_EUNSUPPORTED,
-                        "axis length %zu not supported by the GPU FFT (needs 2^a3^b5^c7^d11^e13^f <= %d)",
-                        d, kMaxLen);
+                        "axis length %zu not supported by the GPU FFT (2^a3^b5^c7^d11^e13^f <= %d, "
+                        "or any length <= %d)",
+                        d, kMaxLen, kMaxChirpLen);
     if (g->nx * g->ny * g->nz > (size_t)1 << 29)
         return fail(QSM_EUNSUPPORTED, "volume of %zu voxels exceeds 2^29", g->nx * g->ny * g->nz);
     return QSM_OK;
@@ -150,7 +204,7 @@ struct PassBlocks {
 
 // Pair-rows per block of the x passes: as many as 16 elements per thread
 // hold, at most 32.
-static int default_rows_x(int nx) { return std::max(1, std::min(32, (qsmfft::kThreads * 16) / nx)); }
+static int default_rows_x(int nx) { return std::max(1, std::min(32, (qsmfft::kThreads * 16) / work_length(nx))); }
 
 // xshift / cshift 2 (compiled lengths only): the R2C / TV C2R passes take a
 // quarter of their default rows per block, as the kernels launched with

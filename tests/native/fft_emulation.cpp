@@ -7,6 +7,7 @@
 // Built with hipcc as host code; prints one line per case and exits non-zero
 // on any mismatch.
 #include "../../qsm.rs_amd/csrc/fft_col2.hpp"
+#include "../../qsm.rs_amd/csrc/fft_chirp.hpp"
 
 #include <cmath>
 #include <complex>
@@ -88,6 +89,92 @@ static double run_case(int n, int ncols) {
             maxref = std::fmax(maxref, std::abs(acc));
         }
     return maxerr / maxref;
+}
+
+// ---- chirp-z (Bluestein) lengths: the device phases of bluestein_fft ----
+template <int EMAX, bool COLFAST, int LOG2C, bool INV>
+static void emu_bluestein(std::vector<float2>& lds, const Plan& P, int ncols, int es, int cs,
+                          const float2* tw, const float2* chirp, const float2* filt) {
+    const BlGeom g = bl_geom(P, ncols, es, cs, COLFAST);
+    std::vector<std::array<float2, kBlRegs>> regs(kThreads);
+    for (int tid = 0; tid < kThreads; ++tid) {
+        float2 r[kBlRegs];
+        bl_gather(lds.data(), tid, g, false, r);
+        for (int q = 0; q < kBlRegs; ++q) regs[tid][q] = r[q];
+    }
+    for (int tid = 0; tid < kThreads; ++tid) {
+        float2 r[kBlRegs];
+        for (int q = 0; q < kBlRegs; ++q) r[q] = regs[tid][q];
+        bl_scatter_in<INV>(lds.data(), tid, g, r, chirp);
+    }
+    emu_fft<EMAX, COLFAST, LOG2C, false>(lds, P, ncols, g.wes, g.wcs, tw);
+    for (int tid = 0; tid < kThreads; ++tid) bl_filter(lds.data(), tid, g, filt);
+    emu_fft<EMAX, COLFAST, LOG2C, true>(lds, P, ncols, g.wes, g.wcs, tw);
+    for (int tid = 0; tid < kThreads; ++tid) {
+        float2 r[kBlRegs];
+        bl_gather(lds.data(), tid, g, true, r);
+        for (int q = 0; q < kBlRegs; ++q) regs[tid][q] = r[q];
+    }
+    for (int tid = 0; tid < kThreads; ++tid) {
+        float2 r[kBlRegs];
+        for (int q = 0; q < kBlRegs; ++q) r[q] = regs[tid][q];
+        bl_scatter_out<INV>(lds.data(), tid, g, r, chirp);
+    }
+}
+
+// Length n through the chirp-z path with ncols sequences; the LDS array is
+// sized as the kernels size it (ncols * M elements).
+template <int EMAX, bool COLFAST, int LOG2C, bool INV>
+static double run_bluestein(int n, int ncols) {
+    const int M = chirp_length(n);
+    if (M <= 0) return -1.0;
+    Plan P{};
+    if (!make_plan_stages(M, P)) return -1.0;
+    P.bl_n = n;
+    std::vector<float2> tw(M), chirp, filt;
+    for (int q = 0; q < M; ++q) {
+        double a = -2.0 * M_PI * q / M;
+        tw[q] = make_float2((float)std::cos(a), (float)std::sin(a));
+    }
+    chirp_tables(n, M, chirp, filt);
+    const int es = COLFAST ? ncols : 1, cs = COLFAST ? 1 : n;
+    std::vector<float2> lds((size_t)ncols * M, make_float2(NAN, NAN));
+    std::vector<std::complex<double>> in(n * ncols);
+    srand(n * 977 + ncols);
+    for (int c = 0; c < ncols; ++c)
+        for (int e = 0; e < n; ++e) {
+            double re = rand() / (double)RAND_MAX - 0.5, im = rand() / (double)RAND_MAX - 0.5;
+            in[c * n + e] = {re, im};
+            lds[e * es + c * cs] = make_float2((float)re, (float)im);
+        }
+    emu_bluestein<EMAX, COLFAST, LOG2C, INV>(lds, P, ncols, es, cs, tw.data(), chirp.data(), filt.data());
+    double maxerr = 0.0, maxref = 0.0;
+    const double sgn = INV ? 1.0 : -1.0;
+    for (int c = 0; c < ncols; ++c)
+        for (int k = 0; k < n; ++k) {
+            std::complex<double> acc = 0.0;
+            for (int e = 0; e < n; ++e)
+                acc += in[c * n + e] * std::polar(1.0, sgn * 2.0 * M_PI * ((long)k * e % n) / n);
+            float2 g = lds[k * es + c * cs];
+            maxerr = std::fmax(maxerr, std::abs(acc - std::complex<double>(g.x, g.y)));
+            maxref = std::fmax(maxref, std::abs(acc));
+        }
+    return maxerr / maxref;
+}
+
+static int check_bluestein(int n) {
+    const int M = chirp_length(n);
+    // column tiles of 16 (EMAX 32: 16 M <= 8192); rows as the x passes size them
+    double e1 = run_bluestein<32, true, 4, false>(n, 16);
+    double e2 = run_bluestein<32, true, 4, true>(n, 16);
+    int rows = (kThreads * 16) / M; if (rows < 1) rows = 1; if (rows > 32) rows = 32;
+    double e3 = run_bluestein<16, false, 0, false>(n, rows);
+    double e4 = run_bluestein<16, false, 0, true>(n, rows);
+    double worst = std::fmax(std::fmax(e1, e2), std::fmax(e3, e4));
+    bool ok = e1 >= 0 && e2 >= 0 && e3 >= 0 && e4 >= 0 && worst < 4e-6;
+    printf("chirp-z n=%4d M=%4d colfwd=%.2e colinv=%.2e rowfwd=%.2e rowinv=%.2e %s\n", n, M, e1, e2,
+           e3, e4, ok ? "ok" : "FAIL");
+    return ok ? 0 : 1;
 }
 
 // ---- compile-time-length path (fft_fixed.hpp) ----
@@ -272,6 +359,10 @@ int main() {
                ok ? "ok" : "FAIL");
         if (!ok) ++bad;
     }
+    // lengths with a prime factor above 13 (17, 19, 23, 68 = 4 x 17, 97,
+    // 164 = 4 x 41, 232 = 8 x 29, 251, 256 - 2 = 254 = 2 x 127) and a direct
+    // length run through the same path
+    for (int n : {17, 19, 23, 68, 97, 164, 232, 251, 254, 30}) bad += check_bluestein(n);
     bad += bad_fixed + bad_col2;
     printf("%s\n", bad ? "FAILED" : "ALL OK");
     return bad ? 1 : 0;

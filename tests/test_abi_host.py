@@ -124,8 +124,12 @@ def test_bad_parameters_are_errors_not_hangs():
     with pytest.raises(qsm.QsmError) as e:  # min_radius <= 0 never terminates in the reference
         qsm.vsharp(np.zeros(512), np.ones(512, np.uint8), g, qsm.VsharpParams(0.05, 12.0, 0.0))
     assert e.value.code == 2
-    assert not qsm.fft_length_supported(17) and qsm.fft_length_supported(384)
-    assert not qsm.fft_length_supported(1024)
+    # any length up to 256 (prime factors above 13 through the chirp-z
+    # transform), and lengths up to 512 that factor into 2, 3, 5, 7, 11, 13
+    assert qsm.fft_length_supported(17) and qsm.fft_length_supported(384)
+    assert qsm.fft_length_supported(68) and qsm.fft_length_supported(251)
+    assert not qsm.fft_length_supported(257 * 2) and not qsm.fft_length_supported(1024)
+    assert not qsm.fft_length_supported(0)
 
 
 def test_phantom_is_deterministic_and_masked():
