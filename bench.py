@@ -108,7 +108,7 @@ def vsharp_kernel_bytes(dims):
     return {
         # setup: F^ = FFT(field) and M^ = FFT(mask), each R2C + y + z
         "r2c_f64": 8 * N + 8 * Nh, "r2c_u8": N + 8 * Nh,
-        "y_fwd": 16 * Nh, "z_fwd": 16 * Nh,
+        "y_fwd_x2": 2 * 16 * Nh, "z_fwd_x2": 2 * 16 * Nh,   # one y and one z launch per spectrum
         "vs_init": 5 * N,                      # clear the selected field (f32) and mask (u8)
         # per radius: sphere generated in the R2C pass (nothing read)
         "radius_r2c_sphere": 8 * Nh,
