@@ -586,6 +586,218 @@ __device__ __forceinline__ void fill_tvdiv(float2* lds, const VolDesc& V, int nx
     }
 }
 
+// Lane rotations across the 64-wide wavefront (DPP wave_rol / wave_ror):
+// lane l receives the value of lane l + 1 (lane_next) or l - 1 (lane_prev),
+// modulo 64.
+__device__ __forceinline__ float lane_next(float v) {
+    return __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, v), 0x134, 0xF, 0xF,
+                                                                 false));
+}
+__device__ __forceinline__ float lane_prev(float v) {
+    return __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, v), 0x13C, 0xF, 0xF,
+                                                                 false));
+}
+
+// Row lengths and pair-rows per block (RB) the wave-row form takes: whole
+// 64-column groups, at most 8 per lane, and RB / 2 rows per wave even.
+__host__ __device__ constexpr bool tvdiv_rows_form(int n, int rb) {
+    return n > 0 && n % 64 == 0 && n / 64 <= 8 && rb % 4 == 0;
+}
+
+// One row of the wave-row form below: the values at the row's own voxels.
+template <int C>
+struct TvRow {
+    float xc[C], ux[C], uy[C], uz[C], xzp[C], xzm[C], uzm[C];
+};
+
+// Row r of the volume (r = j + k ny, k the plane): its own x and u, x on the
+// planes k +- 1 and u_z on plane k - 1 (periodic), columns lane + 64 m.
+template <int C>
+__device__ __forceinline__ void tvrow_load(TvRow<C>& R, int r, int k, int nz, int nxy, int N, int lane,
+                                           const QSM_GLOBAL float* __restrict__ x,
+                                           const QSM_GLOBAL float* __restrict__ u) {
+    constexpr int nx = 64 * C;
+    const int base = r * nx;
+    const int dzp = (k + 1 < nz) ? nxy : nxy - N;
+    const int dzm = (k > 0) ? -nxy : N - nxy;
+    const QSM_GLOBAL float* xr = x + base;
+    const QSM_GLOBAL float* ur = u + base;
+#pragma unroll
+    for (int m = 0; m < C; ++m) R.xc[m] = xr[lane + 64 * m];
+#pragma unroll
+    for (int m = 0; m < C; ++m) {
+        R.ux[m] = ur[lane + 64 * m];
+        R.uy[m] = ur[N + lane + 64 * m];
+        R.uz[m] = ur[2 * N + lane + 64 * m];
+        R.xzp[m] = xr[dzp + lane + 64 * m];
+        R.xzm[m] = xr[dzm + lane + 64 * m];
+        R.uzm[m] = ur[2 * N + dzm + lane + 64 * m];
+    }
+}
+
+// fill_tvdiv for rows of nx = 64 C with the block's rows split over its
+// waves: wave w takes RPW consecutive rows (RPW even, so a pair-row never
+// spans two waves), lane l the columns l + 64 m.  Row indices are then the
+// same in every lane (scalar), the in-row neighbours (x at i +- 1, u_x at
+// i - 1) come from lane rotations and the y neighbours from the wave's
+// neighbouring rows in registers; only the rows above and below the wave's
+// rows, and y-wrapping rows, are loaded again.  Loads per voxel: 7 plus
+// 3 / RPW (against 13).  The arithmetic per voxel is that of tvdiv_compute.
+template <int C, int RPW>
+__device__ __forceinline__ void fill_tvdiv_rows(float2* lds, const VolDesc& V, int row0, int nrows,
+                                                const QSM_GLOBAL float* __restrict__ x,
+                                                const QSM_GLOBAL float* __restrict__ u,
+                                                QSM_GLOBAL float* __restrict__ uo) {
+    constexpr int nx = 64 * C;
+    static_assert(RPW % 2 == 0, "pair-rows must not span waves");
+    static_assert(kThreads == 256, "four waves per block");
+    const int ny = V.ny, nz = V.nz;
+    const int nxy = nx * ny, N = nxy * nz;
+    const float t = V.thr, hx = V.hx, hy = V.hy, hz = V.hz;
+    const int lane = (int)(threadIdx.x & 63);
+    const int wv = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
+    const int r0 = row0 + wv * RPW;
+    // plane and in-plane index of the wave's first row, then stepped
+    int k = r0 / ny;
+    int j = r0 - k * ny;
+    TvRow<C> A, B;
+    float hym_x[C], hym_uy[C], hyp_x[C];  // rows above / below the wave's rows
+    if (r0 < nrows) {
+        tvrow_load<C>(A, r0, k, nz, nxy, N, lane, x, u);
+        // the row before the wave's first row, periodic in y
+        const int ra = j > 0 ? r0 - 1 : r0 + ny - 1;
+        const QSM_GLOBAL float* xr = x + ra * nx;
+        const QSM_GLOBAL float* ur = u + N + ra * nx;
+#pragma unroll
+        for (int m = 0; m < C; ++m) {
+            hym_x[m] = xr[lane + 64 * m];
+            hym_uy[m] = ur[lane + 64 * m];
+        }
+    } else {
+#pragma unroll
+        for (int m = 0; m < C; ++m) hym_x[m] = hym_uy[m] = 0.0f;
+    }
+    float pxc[C], puy[C], dev[C];
+#pragma unroll
+    for (int q = 0; q < RPW; ++q) {
+        TvRow<C>& cur = (q & 1) ? B : A;
+        TvRow<C>& nxt = (q & 1) ? A : B;
+        const int r = r0 + q;
+        const bool ok = r < nrows;
+        // plane / in-plane index of row r + 1
+        int jn = j + 1, kn = k;
+        if (jn == ny) {
+            jn = 0;
+            ++kn;
+        }
+        if (q + 1 < RPW && r + 1 < nrows) tvrow_load<C>(nxt, r + 1, kn, nz, nxy, N, lane, x, u);
+        if (q + 2 == RPW && ok) {
+            // the row after the wave's last row rl = r + 1 (in-plane index
+            // jn), periodic in y; issued one row ahead
+            const int rl = r + 1;
+            const bool okl = rl < nrows;
+            const int rb = (jn + 1 < ny) ? rl + 1 : rl - ny + 1;
+            const QSM_GLOBAL float* xr = x + (okl ? rb : 0) * nx;
+#pragma unroll
+            for (int m = 0; m < C; ++m) hyp_x[m] = okl ? xr[lane + 64 * m] : 0.0f;
+        }
+        float xym[C], uym[C], xyp[C];
+        if (!ok) {
+            // rows past the volume (the last block): d = 0, nothing stored
+#pragma unroll
+            for (int m = 0; m < C; ++m) {
+                xym[m] = uym[m] = xyp[m] = 0.0f;
+                cur.xc[m] = cur.ux[m] = cur.uy[m] = cur.uz[m] = 0.0f;
+                cur.xzp[m] = cur.xzm[m] = cur.uzm[m] = 0.0f;
+            }
+        } else {
+            // y - 1
+            if (q == 0) {
+#pragma unroll
+                for (int m = 0; m < C; ++m) {
+                    xym[m] = hym_x[m];
+                    uym[m] = hym_uy[m];
+                }
+            } else if (j == 0) {
+                const int ra = r + ny - 1;
+#pragma unroll
+                for (int m = 0; m < C; ++m) {
+                    xym[m] = x[ra * nx + lane + 64 * m];
+                    uym[m] = u[N + ra * nx + lane + 64 * m];
+                }
+            } else {
+#pragma unroll
+                for (int m = 0; m < C; ++m) {
+                    xym[m] = pxc[m];
+                    uym[m] = puy[m];
+                }
+            }
+            // y + 1
+            if (q + 1 == RPW) {
+#pragma unroll
+                for (int m = 0; m < C; ++m) xyp[m] = hyp_x[m];
+            } else if (j + 1 == ny) {
+                const int rb = r - ny + 1;
+#pragma unroll
+                for (int m = 0; m < C; ++m) xyp[m] = x[rb * nx + lane + 64 * m];
+            } else {
+#pragma unroll
+                for (int m = 0; m < C; ++m) xyp[m] = nxt.xc[m];
+            }
+        }
+        // in-row neighbours: lane l + 1 (lane 63: lane 0 of the next
+        // column group, periodic) and lane l - 1
+        float rn[C], rp[C], rpu[C];
+#pragma unroll
+        for (int m = 0; m < C; ++m) {
+            rn[m] = lane_next(cur.xc[m]);
+            rp[m] = lane_prev(cur.xc[m]);
+            rpu[m] = lane_prev(cur.ux[m]);
+        }
+        float d[C];
+#pragma unroll
+        for (int m = 0; m < C; ++m) {
+            const float xp = lane == 63 ? rn[(m + 1) % C] : rn[m];
+            const float xm = lane == 0 ? rp[(m + C - 1) % C] : rp[m];
+            const float uxm = lane == 0 ? rpu[(m + C - 1) % C] : rpu[m];
+            const float xc = cur.xc[m];
+            const float vx = (xp - xc) * hx + cur.ux[m];
+            const float vy = (xyp[m] - xc) * hy + cur.uy[m];
+            const float vz = (cur.xzp[m] - xc) * hz + cur.uz[m];
+            const float vxm = (xc - xm) * hx + uxm;
+            const float vym = (xc - xym[m]) * hy + uym[m];
+            const float vzm = (xc - cur.xzm[m]) * hz + cur.uzm[m];
+            const float zx = shrinkf(vx, t), zy = shrinkf(vy, t), zz = shrinkf(vz, t);
+            if (ok) {
+                QSM_GLOBAL float* o = uo + r * nx + lane + 64 * m;
+                o[0] = vx - zx;
+                o[N] = vy - zy;
+                o[2 * N] = vz - zz;
+            }
+            const float gx = 2.0f * zx - vx, gy = 2.0f * zy - vy, gz = 2.0f * zz - vz;
+            const float gxm = 2.0f * shrinkf(vxm, t) - vxm;
+            const float gym = 2.0f * shrinkf(vym, t) - vym;
+            const float gzm = 2.0f * shrinkf(vzm, t) - vzm;
+            d[m] = ok ? (gx - gxm) * hx + (gy - gym) * hy + (gz - gzm) * hz : 0.0f;
+        }
+        if (q & 1) {
+            const int p = (wv * RPW + q) >> 1;
+#pragma unroll
+            for (int m = 0; m < C; ++m) lds[p * nx + lane + 64 * m] = make_float2(dev[m], d[m]);
+        } else {
+#pragma unroll
+            for (int m = 0; m < C; ++m) dev[m] = d[m];
+        }
+#pragma unroll
+        for (int m = 0; m < C; ++m) {
+            pxc[m] = cur.xc[m];
+            puy[m] = cur.uy[m];
+        }
+        j = jn;
+        k = kn;
+    }
+}
+
 // First step of an NLTV Newton round after the first (src/inversion/
 // nltv.rs:86-120): the previous round ended either at convergence (no z/u
 // update) or after max_iter steps (a z/u update with the previous weights is
@@ -781,7 +993,10 @@ __global__ __launch_bounds__(kThreads) void k_r2c_x(const VolDesc* __restrict__ 
     const int row0 = lb * RB * 2;
     TwStage<N> tws;
     tws.fetch(tw, V.px);
-    if constexpr (SRC == SRC_TVDIV) {
+    if constexpr (SRC == SRC_TVDIV && tvdiv_rows_form(N, RBX)) {
+        fill_tvdiv_rows<N / 64, RBX / 2>(lds, V, row0, nrows, glb(V.x[aux]), glb(V.u[aux ^ 1]),
+                                         glb(V.u[aux]));
+    } else if constexpr (SRC == SRC_TVDIV) {
         fill_tvdiv<EMAX, false>(lds, V, nx, RB, glb(V.x[aux]), glb(V.u[aux ^ 1]), glb(V.u[aux]), row0,
                                 nrows, nullptr);
     } else if constexpr (SRC == SRC_TVDIV_REC) {
