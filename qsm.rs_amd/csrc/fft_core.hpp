@@ -277,94 +277,63 @@ __host__ __device__ __forceinline__ void split_b(int b, const StageGeom& g, int&
     }
 }
 
-// Load phase of a stage: thread tid gathers the inputs of its butterflies.
-template <int R, int BMAX, bool COLFAST, int LOG2C>
-__host__ __device__ __forceinline__ void stage_load(const float2* lds, int tid, const StageGeom& g,
-                                           float2 (&v)[BMAX][R]) {
+// One butterfly of a stage, out of place: reads its R inputs from src,
+// applies the twiddles and a radix-R DFT, writes its R outputs to their
+// Stockham positions in dst.  Registers: the R values only.
+template <int R, bool COLFAST, int LOG2C, bool INV>
+__host__ __device__ __forceinline__ void stage_bfly(const float2* src, float2* dst, int b, const StageGeom& g,
+                                                    const float2* tw) {
+    int j, c;
+    split_b<COLFAST, LOG2C>(b, g, j, c);
+    const float2* p = src + j * g.es + c * g.cs;
+    float2 v[R];
+#pragma unroll
+    for (int r = 0; r < R; ++r) v[r] = p[r * g.nb * g.es];
+    const int grp = (int)fastdiv((uint32_t)j, g.m_Ns);
+    const int k = j - grp * g.Ns;
+    if (k != 0) {
+        const int ts = k * g.tstep;
+#pragma unroll
+        for (int r = 1; r < R; ++r) {
+            float2 w = tw[r * ts];
+            if (INV) w = cconj(w);
+            v[r] = cmul(v[r], w);
+        }
+    }
+    Dft<R, INV>::run(v, tw, g.n);
+    float2* q = dst + (grp * g.Ns * R + k) * g.es + c * g.cs;
+#pragma unroll
+    for (int r = 0; r < R; ++r) q[r * g.Ns * g.es] = v[r];
+}
+
+// Thread tid's share of a stage: butterflies tid, tid + kThreads, ...  The
+// stage reads only src and writes only dst, so the threads need no barrier
+// inside it, one after it.
+template <int R, bool COLFAST, int LOG2C, bool INV>
+__host__ __device__ __forceinline__ void stage_pp(const float2* src, float2* dst, int tid, const StageGeom& g,
+                                                  const float2* tw) {
     const int total = g.nb * g.ncols;
-#pragma unroll
-    for (int t = 0; t < BMAX; ++t) {
-        int b = tid + t * kThreads;
-        if (b < total) {
-            int j, c;
-            split_b<COLFAST, LOG2C>(b, g, j, c);
-            const float2* p = lds + j * g.es + c * g.cs;
-#pragma unroll
-            for (int r = 0; r < R; ++r) v[t][r] = p[r * g.nb * g.es];
-        }
-    }
+#pragma unroll 2
+    for (int b = tid; b < total; b += kThreads) stage_bfly<R, COLFAST, LOG2C, INV>(src, dst, b, g, tw);
 }
 
-// Compute + store phase of a stage.
-template <int R, int BMAX, bool COLFAST, int LOG2C, bool INV>
-__host__ __device__ __forceinline__ void stage_store(float2* lds, int tid, const StageGeom& g,
-                                            float2 (&v)[BMAX][R], const float2* tw) {
-    const int total = g.nb * g.ncols;
-#pragma unroll
-    for (int t = 0; t < BMAX; ++t) {
-        int b = tid + t * kThreads;
-        if (b < total) {
-            int j, c;
-            split_b<COLFAST, LOG2C>(b, g, j, c);
-            int grp = (int)fastdiv((uint32_t)j, g.m_Ns);
-            int k = j - grp * g.Ns;
-            if (k != 0) {
-                int ts = k * g.tstep;
-#pragma unroll
-                for (int r = 1; r < R; ++r) {
-                    float2 w = tw[r * ts];
-                    if (INV) w = cconj(w);
-                    v[t][r] = cmul(v[t][r], w);
-                }
-            }
-            Dft<R, INV>::run(v[t], tw, g.n);
-            float2* p = lds + (grp * g.Ns * R + k) * g.es + c * g.cs;
-#pragma unroll
-            for (int r = 0; r < R; ++r) p[r * g.Ns * g.es] = v[t][r];
-        }
+// Stage s of plan P from src to dst (the radix chosen at run time).
+template <bool COLFAST, int LOG2C, bool INV>
+__host__ __device__ inline void plan_stage(const float2* src, float2* dst, int tid, const Plan& P, int s, int ncols,
+                                           int es, int cs, const float2* tw) {
+    const StageGeom g = plan_geom(P, s, ncols, es, cs);
+    switch (P.radix[s]) {
+        case 2: stage_pp<2, COLFAST, LOG2C, INV>(src, dst, tid, g, tw); break;
+        case 3: stage_pp<3, COLFAST, LOG2C, INV>(src, dst, tid, g, tw); break;
+        case 4: stage_pp<4, COLFAST, LOG2C, INV>(src, dst, tid, g, tw); break;
+        case 5: stage_pp<5, COLFAST, LOG2C, INV>(src, dst, tid, g, tw); break;
+        case 7: stage_pp<7, COLFAST, LOG2C, INV>(src, dst, tid, g, tw); break;
+        case 8: stage_pp<8, COLFAST, LOG2C, INV>(src, dst, tid, g, tw); break;
+        case 11: stage_pp<11, COLFAST, LOG2C, INV>(src, dst, tid, g, tw); break;
+        case 13: stage_pp<13, COLFAST, LOG2C, INV>(src, dst, tid, g, tw); break;
+        default: break;
     }
 }
-
-// Elements per thread the register staging is sized for.
-template <int R, int EMAX>
-struct Bmax {
-    static constexpr int value = (EMAX + R - 1) / R;
-};
-
-#ifdef __HIPCC__
-// Full in-LDS transform on the device.  Caller must have synchronised after
-// filling LDS; returns after a final barrier.  n*ncols <= kThreads*EMAX.
-template <int EMAX, bool COLFAST, int LOG2C, bool INV>
-__device__ __forceinline__ void lds_fft(float2* lds, const Plan& P, int ncols, int es, int cs,
-                               const float2* tw) {
-    const int tid = threadIdx.x;
-    const int nst = P.nstages;
-    for (int s = 0; s < nst; ++s) {
-        const int R = P.radix[s];
-        const StageGeom g = plan_geom(P, s, ncols, es, cs);
-#define QSM_STAGE(RR)                                                         \
-    case RR: {                                                                \
-        float2 v[Bmax<RR, EMAX>::value][RR];                                  \
-        stage_load<RR, Bmax<RR, EMAX>::value, COLFAST, LOG2C>(lds, tid, g, v); \
-        __syncthreads();                                                      \
-        stage_store<RR, Bmax<RR, EMAX>::value, COLFAST, LOG2C, INV>(lds, tid, g, v, tw); \
-        __syncthreads();                                                      \
-    } break;
-        switch (R) {
-            QSM_STAGE(2)
-            QSM_STAGE(3)
-            QSM_STAGE(4)
-            QSM_STAGE(5)
-            QSM_STAGE(7)
-            QSM_STAGE(8)
-            QSM_STAGE(11)
-            QSM_STAGE(13)
-            default: break;
-        }
-#undef QSM_STAGE
-    }
-}
-#endif
 
 // ------------------------------------------------------------ Bluestein --
 // Chirp-z transform of length n through a circular convolution of length M
@@ -374,10 +343,11 @@ __device__ __forceinline__ void lds_fft(float2* lds, const Plan& P, int ncols, i
 // so X = c . IFFT_M(FFT_M(x c, zero-padded) . B) / M with B = FFT_M of
 // conj(c[m]) placed at m and M - m (Bluestein 1970).  The bank holds c (n
 // entries) and B / M (M entries); the inverse transform is
-// conj(forward(conj(x))).  Element (e, c) is at lds[e*es + c*cs] in the
-// caller's layout (e < n) and at lds[e*wes + c*wcs] while working (e < M):
+// conj(forward(conj(x))).  Element (e, c) is at [e*es + c*cs] in the
+// caller's layout (e < n) and at [e*wes + c*wcs] while working (e < M):
 // column tiles keep their layout with M rows, rows are respaced to stride M.
-// Each phase is a per-thread function so the host emulation can run it.
+// Each phase is element-wise from one LDS buffer to the other (or in place),
+// a per-thread function the host emulation runs as well.
 struct BlGeom {
     int n, M, ncols, es, cs, wes, wcs;
 };
@@ -388,89 +358,86 @@ __host__ __device__ inline BlGeom bl_geom(const Plan& P, int ncols, int es, int 
     g.wcs = colfast ? cs : P.n;
     return g;
 }
-constexpr int kBlRegs = 16;  // n * ncols <= kBlRegs * kThreads (n <= M / 2, ncols * M <= 8192)
-
-__host__ __device__ inline void bl_gather(const float2* lds, int tid, const BlGeom& g, bool working,
-                                          float2 (&r)[kBlRegs]) {
-    const int es = working ? g.wes : g.es, cs = working ? g.wcs : g.cs;
-    for (int q = 0; q < kBlRegs; ++q) {
-        const int idx = tid + q * kThreads;
-        if (idx < g.n * g.ncols) {
-            const int c = idx / g.n, e = idx - c * g.n;
-            r[q] = lds[e * es + c * cs];
-        }
-    }
-}
-// x c (conj(x) c for the inverse) into the working layout, zeros at e >= n
+// x c (conj(x) c for the inverse) from src (caller's layout) into dst
+// (working layout), zeros at n <= e < M
 template <bool INV>
-__host__ __device__ inline void bl_scatter_in(float2* lds, int tid, const BlGeom& g,
-                                              const float2 (&r)[kBlRegs], const float2* chirp) {
-    for (int q = 0; q < kBlRegs; ++q) {
-        const int idx = tid + q * kThreads;
-        if (idx < g.n * g.ncols) {
-            const int c = idx / g.n, e = idx - c * g.n;
-            lds[e * g.wes + c * g.wcs] = cmul(INV ? cconj(r[q]) : r[q], chirp[e]);
-        }
-    }
-    const int pad = g.M - g.n;
-    for (int idx = tid; idx < pad * g.ncols; idx += kThreads) {
-        const int c = idx / pad, e = g.n + (idx - c * pad);
-        lds[e * g.wes + c * g.wcs] = make_float2(0.0f, 0.0f);
-    }
-}
-__host__ __device__ inline void bl_filter(float2* lds, int tid, const BlGeom& g, const float2* filt) {
+__host__ __device__ inline void bl_in(const float2* src, float2* dst, int tid, const BlGeom& g, const float2* chirp) {
     for (int idx = tid; idx < g.M * g.ncols; idx += kThreads) {
         const int c = idx / g.M, e = idx - c * g.M;
-        float2* p = lds + e * g.wes + c * g.wcs;
+        float2 v = make_float2(0.0f, 0.0f);
+        if (e < g.n) {
+            const float2 x = src[e * g.es + c * g.cs];
+            v = cmul(INV ? cconj(x) : x, chirp[e]);
+        }
+        dst[e * g.wes + c * g.wcs] = v;
+    }
+}
+// times the filter B / M, in place (working layout)
+__host__ __device__ inline void bl_filter(float2* buf, int tid, const BlGeom& g, const float2* filt) {
+    for (int idx = tid; idx < g.M * g.ncols; idx += kThreads) {
+        const int c = idx / g.M, e = idx - c * g.M;
+        float2* p = buf + e * g.wes + c * g.wcs;
         *p = cmul(*p, filt[e]);
     }
 }
-// c . conv (conjugated for the inverse) back to the caller's layout
+// c . conv (conjugated for the inverse) from src (working layout) into dst
+// (caller's layout)
 template <bool INV>
-__host__ __device__ inline void bl_scatter_out(float2* lds, int tid, const BlGeom& g,
-                                               const float2 (&r)[kBlRegs], const float2* chirp) {
-    for (int q = 0; q < kBlRegs; ++q) {
-        const int idx = tid + q * kThreads;
-        if (idx < g.n * g.ncols) {
-            const int c = idx / g.n, e = idx - c * g.n;
-            const float2 v = cmul(r[q], chirp[e]);
-            lds[e * g.es + c * g.cs] = INV ? cconj(v) : v;
-        }
+__host__ __device__ inline void bl_out(const float2* src, float2* dst, int tid, const BlGeom& g,
+                                       const float2* chirp) {
+    for (int idx = tid; idx < g.n * g.ncols; idx += kThreads) {
+        const int c = idx / g.n, e = idx - c * g.n;
+        const float2 v = cmul(src[e * g.wes + c * g.wcs], chirp[e]);
+        dst[e * g.es + c * g.cs] = INV ? cconj(v) : v;
     }
 }
 
 #ifdef __HIPCC__
-// The chirp-z transform on the device.  The LDS array must hold ncols * M
-// elements; twl holds the length-M twiddle table, twg is the bank in global
-// memory (chirp and filter tables).  Returns after a final barrier.
-template <int EMAX, bool COLFAST, int LOG2C, bool INV>
-__device__ __noinline__ void bluestein_fft(float2* lds, const Plan& P, int ncols, int es, int cs,
-                                           const float2* twl, const float2* twg) {
-    const BlGeom g = bl_geom(P, ncols, es, cs, COLFAST);
-    const float2* chirp = twg + P.bl_chirp;
-    const float2* filt = twg + P.bl_filt;
-    const int tid = threadIdx.x;
-    float2 r[kBlRegs];
-    bl_gather(lds, tid, g, false, r);
-    __syncthreads();
-    bl_scatter_in<INV>(lds, tid, g, r, chirp);
-    __syncthreads();
-    lds_fft<EMAX, COLFAST, LOG2C, false>(lds, P, ncols, g.wes, g.wcs, twl);
-    bl_filter(lds, tid, g, filt);
-    __syncthreads();
-    lds_fft<EMAX, COLFAST, LOG2C, true>(lds, P, ncols, g.wes, g.wcs, twl);
-    bl_gather(lds, tid, g, true, r);
-    __syncthreads();
-    bl_scatter_out<INV>(lds, tid, g, r, chirp);
-    __syncthreads();
-}
-
-// Any runtime-planned length: direct stages or the chirp-z transform.
-template <int EMAX, bool COLFAST, int LOG2C, bool INV>
-__device__ __forceinline__ void lds_fft_any(float2* lds, const Plan& P, int ncols, int es, int cs,
+// Full transform of the ncols sequences in buffer a on the device, buffer b
+// as the other half of each stage (both hold ncols * n, or ncols * M for a
+// chirp-z length, elements).  Caller must have synchronised after filling a;
+// returns after a final barrier with the result in a.
+template <bool COLFAST, int LOG2C, bool INV>
+__device__ __forceinline__ void lds_fft_any(float2* a, float2* b, const Plan& P, int ncols, int es, int cs,
                                             const float2* twl, const float2* twg) {
-    if (P.bl_n) bluestein_fft<EMAX, COLFAST, LOG2C, INV>(lds, P, ncols, es, cs, twl, twg);
-    else lds_fft<EMAX, COLFAST, LOG2C, INV>(lds, P, ncols, es, cs, twl);
+    const int tid = threadIdx.x;
+    float2* src = a;
+    float2* dst = b;
+    auto stages = [&](int wes, int wcs, bool inv) {
+        for (int s = 0; s < P.nstages; ++s) {
+            if (inv) plan_stage<COLFAST, LOG2C, true>(src, dst, tid, P, s, ncols, wes, wcs, twl);
+            else plan_stage<COLFAST, LOG2C, false>(src, dst, tid, P, s, ncols, wes, wcs, twl);
+            __syncthreads();
+            float2* t = src;
+            src = dst;
+            dst = t;
+        }
+    };
+    int n = P.n;
+    if (P.bl_n) {
+        const BlGeom g = bl_geom(P, ncols, es, cs, COLFAST);
+        const float2* chirp = twg + P.bl_chirp;
+        n = g.n;
+        bl_in<INV>(a, b, tid, g, chirp);
+        __syncthreads();
+        src = b;
+        dst = a;
+        stages(g.wes, g.wcs, false);
+        bl_filter(src, tid, g, twg + P.bl_filt);
+        __syncthreads();
+        stages(g.wes, g.wcs, true);
+        bl_out<INV>(src, dst, tid, g, chirp);
+        __syncthreads();
+        src = dst;
+    } else {
+        stages(es, cs, INV);
+    }
+    if (src != a) {
+        // the caller's region is contiguous: ncols rows of n (es 1, cs n)
+        // or n rows of ncols (es ncols, cs 1)
+        for (int i = tid; i < n * ncols; i += kThreads) a[i] = src[i];
+        __syncthreads();
+    }
 }
 #endif
 

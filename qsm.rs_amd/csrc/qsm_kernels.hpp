@@ -68,15 +68,27 @@ __host__ __device__ constexpr int col_lds2(int n, int eg) {
     return qsmfft::col2_ok(n) ? 16 * n + 256 : col_lds(n, eg);
 }
 
+// The second LDS buffer of the runtime-plan transforms: one per kernel and
+// size, shared by every transform the kernel runs.
+template <int SIZE>
+__device__ __forceinline__ float2* pp_buffer() {
+    __shared__ float2 pp[SIZE];
+    return pp;
+}
+
 // twl: the pass's twiddle table in LDS; twg: the twiddle bank in global
 // memory (the chirp-z tables of runtime-planned lengths).
 template <int N, int EMAX, bool COLFAST, int LOG2C, bool INV, int NC = 0>
 __device__ __forceinline__ void run_fft(float2* lds, const qsmfft::Plan& P, int ncols, int es, int cs,
                                         const float2* twl, const float2* twg) {
-    if constexpr (N > 0)
+    if constexpr (N > 0) {
         qsmfft::fixed_fft<N, NC ? NC : (COLFAST ? kTile : qsmfft::fixed_rows(N)), COLFAST, INV>(lds, twl);
-    else
-        qsmfft::lds_fft_any<EMAX, COLFAST, LOG2C, INV>(lds, P, ncols, es, cs, twl, twg);
+    } else {
+        // runtime-planned lengths: out-of-place stages between the kernel's
+        // tile (kThreads * EMAX elements in every such kernel) and a second
+        // buffer of that size
+        qsmfft::lds_fft_any<COLFAST, LOG2C, INV>(lds, pp_buffer<kThreads * EMAX>(), P, ncols, es, cs, twl, twg);
+    }
 }
 
 // A pass's twiddle table (exp(-2 pi i q / n), q < n) is fetched into

@@ -18,42 +18,26 @@
 
 using namespace qsmfft;
 
-template <int R, int EMAX, bool COLFAST, int LOG2C, bool INV>
-static void emu_stage(std::vector<float2>& lds, const StageGeom& g, const float2* tw) {
-    constexpr int B = Bmax<R, EMAX>::value;
-    std::vector<std::array<std::array<float2, R>, B>> regs(kThreads);
-    for (int tid = 0; tid < kThreads; ++tid) {
-        float2 v[B][R];
-        stage_load<R, B, COLFAST, LOG2C>(lds.data(), tid, g, v);
-        for (int t = 0; t < B; ++t)
-            for (int r = 0; r < R; ++r) regs[tid][t][r] = v[t][r];
+// The out-of-place stages of lds_fft_any: every simulated thread runs its
+// share of a stage from src to dst, then the buffers swap (the barrier).
+// Returns the buffer holding the result.
+template <bool COLFAST, int LOG2C, bool INV>
+static std::vector<float2>* emu_stages(std::vector<float2>* src, std::vector<float2>* dst, const Plan& P,
+                                       int ncols, int es, int cs, const float2* tw) {
+    for (int s = 0; s < P.nstages; ++s) {
+        for (int tid = 0; tid < kThreads; ++tid)
+            plan_stage<COLFAST, LOG2C, INV>(src->data(), dst->data(), tid, P, s, ncols, es, cs, tw);
+        std::swap(src, dst);
     }
-    for (int tid = 0; tid < kThreads; ++tid) {
-        float2 v[B][R];
-        for (int t = 0; t < B; ++t)
-            for (int r = 0; r < R; ++r) v[t][r] = regs[tid][t][r];
-        stage_store<R, B, COLFAST, LOG2C, INV>(lds.data(), tid, g, v, tw);
-    }
+    return src;
 }
 
 template <int EMAX, bool COLFAST, int LOG2C, bool INV>
 static void emu_fft(std::vector<float2>& lds, const Plan& P, int ncols, int es, int cs,
                     const float2* tw) {
-    for (int s = 0; s < P.nstages; ++s) {
-        int R = P.radix[s];
-        StageGeom g = plan_geom(P, s, ncols, es, cs);
-        switch (R) {
-            case 2: emu_stage<2, EMAX, COLFAST, LOG2C, INV>(lds, g, tw); break;
-            case 3: emu_stage<3, EMAX, COLFAST, LOG2C, INV>(lds, g, tw); break;
-            case 4: emu_stage<4, EMAX, COLFAST, LOG2C, INV>(lds, g, tw); break;
-            case 5: emu_stage<5, EMAX, COLFAST, LOG2C, INV>(lds, g, tw); break;
-            case 7: emu_stage<7, EMAX, COLFAST, LOG2C, INV>(lds, g, tw); break;
-            case 8: emu_stage<8, EMAX, COLFAST, LOG2C, INV>(lds, g, tw); break;
-            case 11: emu_stage<11, EMAX, COLFAST, LOG2C, INV>(lds, g, tw); break;
-            case 13: emu_stage<13, EMAX, COLFAST, LOG2C, INV>(lds, g, tw); break;
-            default: std::abort();
-        }
-    }
+    std::vector<float2> other(lds.size(), make_float2(NAN, NAN));
+    std::vector<float2>* r = emu_stages<COLFAST, LOG2C, INV>(&lds, &other, P, ncols, es, cs, tw);
+    if (r != &lds) lds = *r;
 }
 
 template <int EMAX, bool COLFAST, int LOG2C, bool INV>
@@ -91,35 +75,20 @@ static double run_case(int n, int ncols) {
     return maxerr / maxref;
 }
 
-// ---- chirp-z (Bluestein) lengths: the device phases of bluestein_fft ----
+// ---- chirp-z (Bluestein) lengths: the device phases of lds_fft_any ----
 template <int EMAX, bool COLFAST, int LOG2C, bool INV>
 static void emu_bluestein(std::vector<float2>& lds, const Plan& P, int ncols, int es, int cs,
                           const float2* tw, const float2* chirp, const float2* filt) {
     const BlGeom g = bl_geom(P, ncols, es, cs, COLFAST);
-    std::vector<std::array<float2, kBlRegs>> regs(kThreads);
-    for (int tid = 0; tid < kThreads; ++tid) {
-        float2 r[kBlRegs];
-        bl_gather(lds.data(), tid, g, false, r);
-        for (int q = 0; q < kBlRegs; ++q) regs[tid][q] = r[q];
-    }
-    for (int tid = 0; tid < kThreads; ++tid) {
-        float2 r[kBlRegs];
-        for (int q = 0; q < kBlRegs; ++q) r[q] = regs[tid][q];
-        bl_scatter_in<INV>(lds.data(), tid, g, r, chirp);
-    }
-    emu_fft<EMAX, COLFAST, LOG2C, false>(lds, P, ncols, g.wes, g.wcs, tw);
-    for (int tid = 0; tid < kThreads; ++tid) bl_filter(lds.data(), tid, g, filt);
-    emu_fft<EMAX, COLFAST, LOG2C, true>(lds, P, ncols, g.wes, g.wcs, tw);
-    for (int tid = 0; tid < kThreads; ++tid) {
-        float2 r[kBlRegs];
-        bl_gather(lds.data(), tid, g, true, r);
-        for (int q = 0; q < kBlRegs; ++q) regs[tid][q] = r[q];
-    }
-    for (int tid = 0; tid < kThreads; ++tid) {
-        float2 r[kBlRegs];
-        for (int q = 0; q < kBlRegs; ++q) r[q] = regs[tid][q];
-        bl_scatter_out<INV>(lds.data(), tid, g, r, chirp);
-    }
+    std::vector<float2> other(lds.size(), make_float2(NAN, NAN));
+    for (int tid = 0; tid < kThreads; ++tid) bl_in<INV>(lds.data(), other.data(), tid, g, chirp);
+    std::vector<float2>* r = emu_stages<COLFAST, LOG2C, false>(&other, &lds, P, ncols, g.wes, g.wcs, tw);
+    for (int tid = 0; tid < kThreads; ++tid) bl_filter(r->data(), tid, g, filt);
+    std::vector<float2>* o = (r == &lds) ? &other : &lds;
+    r = emu_stages<COLFAST, LOG2C, true>(r, o, P, ncols, g.wes, g.wcs, tw);
+    o = (r == &lds) ? &other : &lds;
+    for (int tid = 0; tid < kThreads; ++tid) bl_out<INV>(r->data(), o->data(), tid, g, chirp);
+    if (o != &lds) lds = *o;
 }
 
 // Length n through the chirp-z path with ncols sequences; the LDS array is
