@@ -81,13 +81,13 @@ static void emu_bluestein(std::vector<float2>& lds, const Plan& P, int ncols, in
                           const float2* tw, const float2* chirp, const float2* filt) {
     const BlGeom g = bl_geom(P, ncols, es, cs, COLFAST);
     std::vector<float2> other(lds.size(), make_float2(NAN, NAN));
-    for (int tid = 0; tid < kThreads; ++tid) bl_in<INV>(lds.data(), other.data(), tid, g, chirp);
+    for (int tid = 0; tid < kThreads; ++tid) bl_in<COLFAST, LOG2C, INV>(lds.data(), other.data(), tid, g, chirp);
     std::vector<float2>* r = emu_stages<COLFAST, LOG2C, false>(&other, &lds, P, ncols, g.wes, g.wcs, tw);
-    for (int tid = 0; tid < kThreads; ++tid) bl_filter(r->data(), tid, g, filt);
+    for (int tid = 0; tid < kThreads; ++tid) bl_filter<COLFAST, LOG2C>(r->data(), tid, g, filt);
     std::vector<float2>* o = (r == &lds) ? &other : &lds;
     r = emu_stages<COLFAST, LOG2C, true>(r, o, P, ncols, g.wes, g.wcs, tw);
     o = (r == &lds) ? &other : &lds;
-    for (int tid = 0; tid < kThreads; ++tid) bl_out<INV>(r->data(), o->data(), tid, g, chirp);
+    for (int tid = 0; tid < kThreads; ++tid) bl_out<COLFAST, LOG2C, INV>(r->data(), o->data(), tid, g, chirp);
     if (o != &lds) lds = *o;
 }
 
