@@ -160,8 +160,8 @@ void qsm_vsharp_params_default(qsm_vsharp_params* p);
 const char* qsm_last_error_message(void);
 /* Device name, architecture and library build info into buf. */
 int qsm_backend_info(char* buf, size_t len);
-/* 1 if n is an axis length the GPU FFT supports, else 0: n <= 512 factoring
- * into 2, 3, 5, 7, 11, 13, or any n <= 256 (a prime factor above 13 runs as a
+/* 1 if n is an axis length the GPU FFT supports, else 0: n <= 1024 factoring
+ * into 2, 3, 5, 7, 11, 13, or any n <= 512 (a prime factor above 13 runs as a
  * chirp-z transform).  The reference (rustfft) transforms any length. */
 int qsm_fft_length_supported(size_t n);
 

@@ -1,6 +1,6 @@
 // Host-side planning of chirp-z (Bluestein) transforms: which lengths run
 // directly, the convolution length of the others and their tables (see
-// bluestein_fft in fft_core.hpp).  Host code only.
+// lds_fft_any in fft_core.hpp).  Host code only.
 #pragma once
 
 #include <cmath>
@@ -10,7 +10,7 @@
 
 namespace qsmfft {
 
-constexpr int kMaxDirectLen = 512;
+constexpr int kMaxDirectLen = 1024;
 // Lengths with a prime factor above 13 run as chirp-z transforms through a
 // convolution of length M >= 2n - 1 <= kMaxDirectLen, so up to this length.
 constexpr int kMaxChirpLen = kMaxDirectLen / 2;

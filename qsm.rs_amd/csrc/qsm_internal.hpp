@@ -21,6 +21,11 @@ namespace qsm {
 constexpr int kMaxVolumes = 64;
 constexpr int kTile = 16;      // columns per y/z-pass tile (16 * 8 B = 128 B rows)
 constexpr int kLog2Tile = 4;
+// Runtime-planned column lengths above this run in tiles of kTileLong
+// columns (two LDS buffers of kTileLong * 1024 elements fit a workgroup).
+constexpr int kTileLong = 8;
+constexpr int kWideTileMaxLen = 512;
+constexpr int tile_log2(int tw) { return tw == 16 ? 4 : (tw == 8 ? 3 : -1); }
 
 struct LaunchTable {
     int nvol;
@@ -43,6 +48,7 @@ struct VolDesc {
     int rows_x;                  // pair-rows per block in the x passes
     int rows_c;                  // pair-rows per block in the TV C2R pass
     int ntx;                     // y/z tiles along kx: ceil(nxh / kTile)
+    int ntx_long;                // the same in tiles of kTileLong columns
     uint64_t m_nx, m_nxh, m_ny;  // magic divisors
     qsmfft::Plan px, py, pz;
     float hx, hy, hz;            // 1/voxel size (gradient / divergence)

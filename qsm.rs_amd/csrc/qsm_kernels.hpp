@@ -25,7 +25,7 @@ namespace qsm {
 using qsmfft::kThreads;
 using qsmfft::fastdiv;
 
-constexpr int kMaxLen = 512;
+constexpr int kMaxLen = 1024;
 
 // Device pointers held in the volume descriptor are generic (flat) to the
 // compiler; accessed through global-address-space types they compile to
@@ -241,22 +241,23 @@ __device__ __forceinline__ float2 ypk_inv(int s, int ny, float2 own, float2 mir)
     return make_float2(mir.x + own.y, own.x - mir.y);                  // conj P0 + i conj PN
 }
 
-// The same on column 0 of a tile held in LDS (lds[row * 16 + c]); every
-// thread of the block calls it.
-template <bool INV>
+// The same on column 0 of a tile of TW columns held in LDS (lds[row * TW +
+// c], ny <= kMaxLen); every thread of the block calls it.
+template <bool INV, int TW = kTile>
 __device__ __forceinline__ void ypk_lds(float2* lds, int ny) {
-    float2 own[2], mir[2];
+    constexpr int Q = (TW == kTile ? kWideTileMaxLen : kMaxLen) / kThreads;
+    float2 own[Q], mir[Q];
 #pragma unroll
-    for (int q = 0; q < 2; ++q) {
+    for (int q = 0; q < Q; ++q) {
         const int s = min((int)threadIdx.x + q * kThreads, ny - 1);
-        own[q] = lds[s * kTile];
-        mir[q] = lds[mirror_idx(s, ny) * kTile];
+        own[q] = lds[s * TW];
+        mir[q] = lds[mirror_idx(s, ny) * TW];
     }
     __syncthreads();
 #pragma unroll
-    for (int q = 0; q < 2; ++q) {
+    for (int q = 0; q < Q; ++q) {
         const int s = threadIdx.x + q * kThreads;
-        if (s < ny) lds[s * kTile] = INV ? ypk_inv(s, ny, own[q], mir[q]) : ypk_fwd(s, ny, own[q], mir[q]);
+        if (s < ny) lds[s * TW] = INV ? ypk_inv(s, ny, own[q], mir[q]) : ypk_fwd(s, ny, own[q], mir[q]);
     }
     __syncthreads();
 }
@@ -285,20 +286,21 @@ __device__ __forceinline__ float sphere_element(const VolDesc& V, int i, int r, 
 }
 
 // ---------------------------------------------------------- tile helpers --
-// Column tile: element (row, c) at lds[row*kTile + c] <-> base[kx0 + c + row*stride];
+// Column tile: element (row, c) at lds[row*TW + c] <-> base[kx0 + c + row*stride];
 // columns kx >= nxh read as zero and are not written.
-template <int EMAX>
+template <int EMAX, int TW = kTile>
 __device__ __forceinline__ void col_load(float2* lds, const QSM_GLOBAL float2* __restrict__ base,
                                          size_t stride, int n, int kx0, int nxh) {
+    constexpr int L2 = tile_log2(TW);
     float2 r[EMAX];
-    const int total = n * kTile;
+    const int total = n * TW;
     // unconditional loads: rows past n and columns past nxh read in-bounds
     // elements that are never stored
 #pragma unroll
     for (int t = 0; t < EMAX; ++t) {
         const int e = threadIdx.x + t * kThreads;
-        const int row = min(e >> kLog2Tile, n - 1);
-        const int kx = min(kx0 + (e & (kTile - 1)), nxh - 1);
+        const int row = min(e >> L2, n - 1);
+        const int kx = min(kx0 + (e & (TW - 1)), nxh - 1);
         r[t] = ldg2(base + kx + row * stride);
     }
 #pragma unroll
@@ -308,15 +310,16 @@ __device__ __forceinline__ void col_load(float2* lds, const QSM_GLOBAL float2* _
     }
 }
 
-template <int EMAX>
+template <int EMAX, int TW = kTile>
 __device__ __forceinline__ void col_store(const float2* lds, QSM_GLOBAL float2* __restrict__ base,
                                           size_t stride, int n, int kx0, int nxh) {
-    const int total = n * kTile;
+    constexpr int L2 = tile_log2(TW);
+    const int total = n * TW;
 #pragma unroll
     for (int t = 0; t < EMAX; ++t) {
         const int e = threadIdx.x + t * kThreads;
-        const int row = e >> kLog2Tile;
-        const int kx = kx0 + (e & (kTile - 1));
+        const int row = e >> L2;
+        const int kx = kx0 + (e & (TW - 1));
         if (e < total && kx < nxh) stg2(base + kx + row * stride, lds[e]);
     }
 }
@@ -1063,14 +1066,16 @@ __global__ __launch_bounds__(kThreads) void k_r2c_x(const VolDesc* __restrict__ 
 }
 
 // ------------------------------------------------------------- C2C y pass --
-// Block = one tile of kTile consecutive kx columns at one kz, full ny.
+// Block = one tile of TW consecutive kx columns at one kz, full ny (TW =
+// kTile; kTileLong for runtime-planned lengths above kWideTileMaxLen).
 // sel2 >= 0 transforms a second spectrum with a second set of blocks.
 // SCHED 0: two-stage column FFT (fft_col2.hpp) where the length allows;
 // SCHED 1: the LDS-staged Stockham stages (less LDS per block).
-template <int N, int EG, bool INV, int SCHED>
+template <int N, int EG, bool INV, int SCHED, int TW = kTile>
 __global__ __launch_bounds__(kThreads) void k_y_pass(const VolDesc* __restrict__ vols, LaunchTable lt,
                                                      const float2* __restrict__ tw, int sel, int sel2,
                                                      const VolStatus* __restrict__ status) {
+    static_assert(TW == kTile || N == 0, "compiled lengths use tiles of kTile columns");
     constexpr int EMAX = col_emax(N, EG);
     constexpr bool TWO = SCHED == 0 && qsmfft::col2_ok(N);
     __shared__ float2 lds[TWO ? col_lds2(N, EG) : col_lds(N, EG)];
@@ -1080,12 +1085,13 @@ __global__ __launch_bounds__(kThreads) void k_y_pass(const VolDesc* __restrict__
     const int gv = locate(lt, bid, lb);
     const VolDesc& V = vols[gv];
     if (status && status[gv].done) return;
-    const int per = V.ntx * V.nz;
+    const int ntx = TW == kTile ? V.ntx : V.ntx_long;
+    const int per = ntx * V.nz;
     int s = sel;
     if (lb >= per) { lb -= per; s = sel2; }
-    const int kz = lb / V.ntx;
-    const int tx = lb - kz * V.ntx;
-    const int kx0 = tx * kTile;
+    const int kz = lb / ntx;
+    const int tx = lb - kz * ntx;
+    const int kx0 = tx * TW;
     const int nxh = V.nxh;
     const int ny = N > 0 ? N : V.ny;
     QSM_GLOBAL float2* buf = glb(V.spec[s]) + (size_t)kz * nxh * ny;
@@ -1138,13 +1144,13 @@ __global__ __launch_bounds__(kThreads) void k_y_pass(const VolDesc* __restrict__
         }
         lr_store<R0>(v, buf, nxh, kx0, nxh);
     } else {
-        col_load<EMAX>(lds, buf, nxh, ny, kx0, nxh);
+        col_load<EMAX, TW>(lds, buf, nxh, ny, kx0, nxh);
         tws.commit(twl);
         __syncthreads();
-        if (INV && pkc) ypk_lds<true>(lds, ny);
-        run_fft<N, EMAX, true, kLog2Tile, INV>(lds, V.py, kTile, kTile, 1, twl, tw);
-        if (!INV && pkc) ypk_lds<false>(lds, ny);
-        col_store<EMAX>(lds, buf, nxh, ny, kx0, nxh);
+        if (INV && pkc) ypk_lds<true, TW>(lds, ny);
+        run_fft<N, EMAX, true, tile_log2(TW), INV>(lds, V.py, TW, TW, 1, twl, tw);
+        if (!INV && pkc) ypk_lds<false, TW>(lds, ny);
+        col_store<EMAX, TW>(lds, buf, nxh, ny, kx0, nxh);
     }
 }
 
@@ -1459,7 +1465,7 @@ __device__ __forceinline__ void z_tv_col2(const VolDesc& V, float2* lds, float2*
     l16_store<R0>(v, buf, zs, kx0, nxh);
 }
 
-template <int N, int EG, int OP, int SCHED>
+template <int N, int EG, int OP, int SCHED, int TW = kTile>
 __global__ __launch_bounds__(kThreads) void k_z_tv(const VolDesc* __restrict__ vols, LaunchTable lt,
                                                    const float2* __restrict__ tw,
                                                    const VolStatus* __restrict__ status) {
@@ -1473,14 +1479,17 @@ __global__ __launch_bounds__(kThreads) void k_z_tv(const VolDesc* __restrict__ v
     const int gv = locate(lt, bid, lb);
     const VolDesc& V = vols[gv];
     if (status && status[gv].done) return;
-    const int ky = lb / V.ntx;
-    const int tx = lb - ky * V.ntx;
-    const int kx0 = tx * kTile;
+    static_assert(TW == kTile || N == 0, "compiled lengths use tiles of kTile columns");
+    constexpr int L2 = tile_log2(TW);
+    const int ntx = TW == kTile ? V.ntx : V.ntx_long;
+    const int ky = lb / ntx;
+    const int tx = lb - ky * ntx;
+    const int kx0 = tx * TW;
     const int nxh = V.nxh;
     const int nz = N > 0 ? N : V.nz;
     const size_t zs = (size_t)nxh * V.ny;
     const size_t base = (size_t)ky * nxh;
-    const int total = nz * kTile;
+    const int total = nz * TW;
     // column 0 holds two z spectra (the DC and Nyquist planes at ky = 0, ny/2)
     const bool dpk = pk_double(V, tx, ky);
     QSM_GLOBAL float2* buf = glb(V.spec[SPEC_WORK]) + base;
@@ -1492,14 +1501,14 @@ __global__ __launch_bounds__(kThreads) void k_z_tv(const VolDesc* __restrict__ v
     }
     if constexpr (OP == ZOP_TV_SOLVE0) {
         // g = 0 before the first step (AdmmBuffers::new), so div = 0 and x^ = f^
-        col_load<EMAX>(lds, glb(V.spec[SPEC_A]) + base, zs, nz, kx0, nxh);
+        col_load<EMAX, TW>(lds, glb(V.spec[SPEC_A]) + base, zs, nz, kx0, nxh);
         tws.commit(twl);
         __syncthreads();
-        run_fft<N, EMAX, true, kLog2Tile, true>(lds, V.pz, kTile, kTile, 1, twl, tw);
-        col_store<EMAX>(lds, buf, zs, nz, kx0, nxh);
+        run_fft<N, EMAX, true, L2, true>(lds, V.pz, TW, TW, 1, twl, tw);
+        col_store<EMAX, TW>(lds, buf, zs, nz, kx0, nxh);
         return;
     }
-    col_load<EMAX>(lds, buf, zs, nz, kx0, nxh);
+    col_load<EMAX, TW>(lds, buf, zs, nz, kx0, nxh);
     // x-update operands, in flight during the forward FFT
     constexpr bool HAS_F = OP == ZOP_TV_SOLVE || OP == ZOP_RTS_SOLVE;
     constexpr bool Q_MEM = OP == ZOP_RTS_SOLVE;
@@ -1511,8 +1520,8 @@ __global__ __launch_bounds__(kThreads) void k_z_tv(const VolDesc* __restrict__ v
 #pragma unroll
         for (int t = 0; t < EMAX; ++t) {
             const int e = threadIdx.x + t * kThreads;
-            const int kz = e >> kLog2Tile;
-            const int kx = kx0 + (e & (kTile - 1));
+            const int kz = e >> L2;
+            const int kx = kx0 + (e & (TW - 1));
             const bool in = e < total && kx < nxh;
             if constexpr (HAS_F) fr[t] = in ? ldg2(F + kx + kz * zs) : make_float2(0.0f, 0.0f);
             if constexpr (Q_MEM) qr[t] = in ? Q[kx + kz * zs] : 0.0f;
@@ -1520,9 +1529,9 @@ __global__ __launch_bounds__(kThreads) void k_z_tv(const VolDesc* __restrict__ v
     }
     tws.commit(twl);
     __syncthreads();
-    run_fft<N, EMAX, true, kLog2Tile, false>(lds, V.pz, kTile, kTile, 1, twl, tw);
+    run_fft<N, EMAX, true, L2, false>(lds, V.pz, TW, TW, 1, twl, tw);
     // a thread's elements all lie in column c
-    const int c = threadIdx.x & (kTile - 1);
+    const int c = threadIdx.x & (TW - 1);
     const bool dcol = dpk && c == 0;
     // doubly packed column: Z(-kz) of each element, read before any is rewritten
     float2 zmir[EMAX];
@@ -1530,8 +1539,8 @@ __global__ __launch_bounds__(kThreads) void k_z_tv(const VolDesc* __restrict__ v
         if (dcol) {
 #pragma unroll
             for (int t = 0; t < EMAX; ++t) {
-                const int kz = min((int)(threadIdx.x + t * kThreads) >> kLog2Tile, nz - 1);
-                zmir[t] = lds[mirror_idx(kz, nz) * kTile];
+                const int kz = min((int)(threadIdx.x + t * kThreads) >> L2, nz - 1);
+                zmir[t] = lds[mirror_idx(kz, nz) * TW];
             }
         }
         __syncthreads();
@@ -1542,7 +1551,7 @@ __global__ __launch_bounds__(kThreads) void k_z_tv(const VolDesc* __restrict__ v
 #pragma unroll
         for (int t = 0; t < EMAX; ++t) {
             const int e = threadIdx.x + t * kThreads;
-            const int kz = e >> kLog2Tile;
+            const int kz = e >> L2;
             const int kx = kx0 + c;
             if (e < total && kx < nxh) {
                 const float2 sv = lds[e];
@@ -1569,7 +1578,7 @@ __global__ __launch_bounds__(kThreads) void k_z_tv(const VolDesc* __restrict__ v
 #pragma unroll
             for (int t = 0; t < EMAX; ++t) {
                 const int e = threadIdx.x + t * kThreads;
-                const int kz = e >> kLog2Tile;
+                const int kz = e >> L2;
                 const int kx = kx0 + c;
                 if (e < total) {
                     const float2 sv = lds[e];
@@ -1600,7 +1609,7 @@ __global__ __launch_bounds__(kThreads) void k_z_tv(const VolDesc* __restrict__ v
             for (int t = 0; t < EMAX; ++t) {
                 const int e = threadIdx.x + t * kThreads;
                 if (e < total) {
-                    const int kz = e >> kLog2Tile;
+                    const int kz = e >> L2;
                     const float2 sv = lds[e];
                     float2 qs;  // q * spectrum
                     if (dcol) {
@@ -1628,8 +1637,8 @@ __global__ __launch_bounds__(kThreads) void k_z_tv(const VolDesc* __restrict__ v
             }
         }
         __syncthreads();
-        run_fft<N, EMAX, true, kLog2Tile, true>(lds, V.pz, kTile, kTile, 1, twl, tw);
-        col_store<EMAX>(lds, buf, zs, nz, kx0, nxh);
+        run_fft<N, EMAX, true, L2, true>(lds, V.pz, TW, TW, 1, twl, tw);
+        col_store<EMAX, TW>(lds, buf, zs, nz, kx0, nxh);
     }
 }
 
@@ -1805,7 +1814,7 @@ __global__ __launch_bounds__(kThreads) void k_c2r_tv(const VolDesc* __restrict__
 // VS_DECONV: forward z of FFT(local), X / (1 - S^) with |1 - S^| < threshold
 //   -> 0 (src/bgremove/vsharp.rs:180-196; division form of
 //   src/bgremove/sharp.rs:113-123 when divide != 0), scaled by 1/N, inverse z.
-template <int N, int EG, int OP>
+template <int N, int EG, int OP, int TW = kTile>
 __global__ __launch_bounds__(kThreads) void k_z_vs(const VolDesc* __restrict__ vols, LaunchTable lt,
                                                    const float2* __restrict__ tw, int sel, int ridx,
                                                    int flag) {
@@ -1816,30 +1825,33 @@ __global__ __launch_bounds__(kThreads) void k_z_vs(const VolDesc* __restrict__ v
     int lb;
     const int gv = locate(lt, bid, lb);
     const VolDesc& V = vols[gv];
-    const int ky = lb / V.ntx;
-    const int tx = lb - ky * V.ntx;
-    const int kx0 = tx * kTile;
+    static_assert(TW == kTile || N == 0, "compiled lengths use tiles of kTile columns");
+    constexpr int L2 = tile_log2(TW);
+    const int ntx = TW == kTile ? V.ntx : V.ntx_long;
+    const int ky = lb / ntx;
+    const int tx = lb - ky * ntx;
+    const int kx0 = tx * TW;
     const int nxh = V.nxh;
     const int nz = N > 0 ? N : V.nz;
     const size_t zs = (size_t)nxh * V.ny;
     const size_t base = (size_t)ky * nxh;
-    const int total = nz * kTile;
+    const int total = nz * TW;
     QSM_GLOBAL float2* buf = glb(V.spec[OP == ZOP_VS_FWD ? sel : SPEC_WORK]) + base;
     TwStage<N> tws;
     tws.fetch(tw, V.pz);
-    col_load<EMAX>(lds, buf, zs, nz, kx0, nxh);
+    col_load<EMAX, TW>(lds, buf, zs, nz, kx0, nxh);
     tws.commit(twl);
     __syncthreads();
-    run_fft<N, EMAX, true, kLog2Tile, false>(lds, V.pz, kTile, kTile, 1, twl, tw);
+    run_fft<N, EMAX, true, L2, false>(lds, V.pz, TW, TW, 1, twl, tw);
     if constexpr (OP == ZOP_VS_FWD) {
-        col_store<EMAX>(lds, buf, zs, nz, kx0, nxh);
+        col_store<EMAX, TW>(lds, buf, zs, nz, kx0, nxh);
     } else {
         // column 0 of tile 0 at ky = 0, ny/2 holds two z spectra (the DC and
         // Nyquist planes, even nx); the sphere's spectrum there is SA + i SB,
         // both real, and the S^ of the Nyquist plane is kept after the half
         // spectrum in vs_sfirst (2 nz entries)
         const bool dpk = pk_double(V, tx, ky);
-        const int c = threadIdx.x & (kTile - 1);
+        const int c = threadIdx.x & (TW - 1);
         const bool dcol = dpk && c == 0;
         QSM_GLOBAL float* sfirst = glb(V.vs_sfirst) + base;
         QSM_GLOBAL float* sfirst_nyq =
@@ -1854,7 +1866,7 @@ __global__ __launch_bounds__(kThreads) void k_z_vs(const VolDesc* __restrict__ v
             for (int t = 0; t < EMAX; ++t) {
                 const int e = threadIdx.x + t * kThreads;
                 if (e < total) {
-                    const int kz = e >> kLog2Tile;
+                    const int kz = e >> L2;
                     const int kx = kx0 + c;
                     const float2 z = lds[e];
                     if (kx < nxh) {
@@ -1884,8 +1896,8 @@ __global__ __launch_bounds__(kThreads) void k_z_vs(const VolDesc* __restrict__ v
                 }
             }
             __syncthreads();
-            run_fft<N, EMAX, true, kLog2Tile, true>(lds, V.pz, kTile, kTile, 1, twl, tw);
-            col_store<EMAX>(lds, buf, zs, nz, kx0, nxh);
+            run_fft<N, EMAX, true, L2, true>(lds, V.pz, TW, TW, 1, twl, tw);
+            col_store<EMAX, TW>(lds, buf, zs, nz, kx0, nxh);
             __syncthreads();
 #pragma unroll
             for (int t = 0; t < EMAX; ++t) {
@@ -1893,8 +1905,8 @@ __global__ __launch_bounds__(kThreads) void k_z_vs(const VolDesc* __restrict__ v
                 if (e < total) lds[e] = qv[t];
             }
             __syncthreads();
-            run_fft<N, EMAX, true, kLog2Tile, true>(lds, V.pz, kTile, kTile, 1, twl, tw);
-            col_store<EMAX>(lds, glb(V.spec[SPEC_C]) + base, zs, nz, kx0, nxh);
+            run_fft<N, EMAX, true, L2, true>(lds, V.pz, TW, TW, 1, twl, tw);
+            col_store<EMAX, TW>(lds, glb(V.spec[SPEC_C]) + base, zs, nz, kx0, nxh);
         } else {  // ZOP_VS_DECONV
             const double invN = 1.0 / ((double)V.nx * V.ny * V.nz);
             // X / (1 - S^), 0 where |1 - S^| < threshold
@@ -1907,8 +1919,8 @@ __global__ __launch_bounds__(kThreads) void k_z_vs(const VolDesc* __restrict__ v
                 if (dcol) {
 #pragma unroll
                     for (int t = 0; t < EMAX; ++t) {
-                        const int kz = min((int)(threadIdx.x + t * kThreads) >> kLog2Tile, nz - 1);
-                        zmir[t] = lds[mirror_idx(kz, nz) * kTile];
+                        const int kz = min((int)(threadIdx.x + t * kThreads) >> L2, nz - 1);
+                        zmir[t] = lds[mirror_idx(kz, nz) * TW];
                     }
                 }
                 __syncthreads();
@@ -1916,7 +1928,7 @@ __global__ __launch_bounds__(kThreads) void k_z_vs(const VolDesc* __restrict__ v
 #pragma unroll
             for (int t = 0; t < EMAX; ++t) {
                 const int e = threadIdx.x + t * kThreads;
-                const int kz = e >> kLog2Tile;
+                const int kz = e >> L2;
                 const int kx = kx0 + c;
                 if (e < total && kx < nxh) {
                     const float2 xv = lds[e];
@@ -1932,8 +1944,8 @@ __global__ __launch_bounds__(kThreads) void k_z_vs(const VolDesc* __restrict__ v
                 }
             }
             __syncthreads();
-            run_fft<N, EMAX, true, kLog2Tile, true>(lds, V.pz, kTile, kTile, 1, twl, tw);
-            col_store<EMAX>(lds, buf, zs, nz, kx0, nxh);
+            run_fft<N, EMAX, true, L2, true>(lds, V.pz, TW, TW, 1, twl, tw);
+            col_store<EMAX, TW>(lds, buf, zs, nz, kx0, nxh);
         }
     }
 }

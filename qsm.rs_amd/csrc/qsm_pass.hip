@@ -78,7 +78,7 @@ static void launch_rows(int kind, const LaunchTable& lt, const PassArgs& a) {
     launch_rows_s<N, EG, 0>(kind, lt, a);
 }
 
-template <int N, int EG>
+template <int N, int EG, int TW = kTile>
 static void launch_cols(int kind, const LaunchTable& lt, const PassArgs& a) {
     // lengths without the two-stage transform compile only the staged one
     constexpr int S0 = qsmfft::col2_ok(N) ? 0 : 1;
@@ -86,37 +86,37 @@ static void launch_cols(int kind, const LaunchTable& lt, const PassArgs& a) {
     hipStream_t st = a.stream;
     switch (kind) {
         case P_Y_FWD:
-            if (a.sched) k_y_pass<N, EG, false, 1><<<g, b, 0, st>>>(a.vols, lt, a.tw, a.sel, a.sel2, a.status);
-            else k_y_pass<N, EG, false, S0><<<g, b, 0, st>>>(a.vols, lt, a.tw, a.sel, a.sel2, a.status);
+            if (a.sched) k_y_pass<N, EG, false, 1, TW><<<g, b, 0, st>>>(a.vols, lt, a.tw, a.sel, a.sel2, a.status);
+            else k_y_pass<N, EG, false, S0, TW><<<g, b, 0, st>>>(a.vols, lt, a.tw, a.sel, a.sel2, a.status);
             break;
         case P_Y_INV:
-            if (a.sched) k_y_pass<N, EG, true, 1><<<g, b, 0, st>>>(a.vols, lt, a.tw, a.sel, a.sel2, a.status);
-            else k_y_pass<N, EG, true, S0><<<g, b, 0, st>>>(a.vols, lt, a.tw, a.sel, a.sel2, a.status);
+            if (a.sched) k_y_pass<N, EG, true, 1, TW><<<g, b, 0, st>>>(a.vols, lt, a.tw, a.sel, a.sel2, a.status);
+            else k_y_pass<N, EG, true, S0, TW><<<g, b, 0, st>>>(a.vols, lt, a.tw, a.sel, a.sel2, a.status);
             break;
-        case P_Z_TV_SETUP: k_z_tv<N, EG, ZOP_TV_SETUP, 1><<<g, b, 0, st>>>(a.vols, lt, a.tw, a.status); break;
+        case P_Z_TV_SETUP: k_z_tv<N, EG, ZOP_TV_SETUP, 1, TW><<<g, b, 0, st>>>(a.vols, lt, a.tw, a.status); break;
         case P_Z_TV_SOLVE:
-            if (a.sched) k_z_tv<N, EG, ZOP_TV_SOLVE, 1><<<g, b, 0, st>>>(a.vols, lt, a.tw, a.status);
-            else k_z_tv<N, EG, ZOP_TV_SOLVE, S0><<<g, b, 0, st>>>(a.vols, lt, a.tw, a.status);
+            if (a.sched) k_z_tv<N, EG, ZOP_TV_SOLVE, 1, TW><<<g, b, 0, st>>>(a.vols, lt, a.tw, a.status);
+            else k_z_tv<N, EG, ZOP_TV_SOLVE, S0, TW><<<g, b, 0, st>>>(a.vols, lt, a.tw, a.status);
             break;
         case P_Z_TV_ADJ:
-            if (a.sched) k_z_tv<N, EG, ZOP_TV_ADJ, 1><<<g, b, 0, st>>>(a.vols, lt, a.tw, nullptr);
-            else k_z_tv<N, EG, ZOP_TV_ADJ, S0><<<g, b, 0, st>>>(a.vols, lt, a.tw, nullptr);
+            if (a.sched) k_z_tv<N, EG, ZOP_TV_ADJ, 1, TW><<<g, b, 0, st>>>(a.vols, lt, a.tw, nullptr);
+            else k_z_tv<N, EG, ZOP_TV_ADJ, S0, TW><<<g, b, 0, st>>>(a.vols, lt, a.tw, nullptr);
             break;
-        case P_Z_TV_APPLYA: k_z_tv<N, EG, ZOP_TV_APPLYA, 1><<<g, b, 0, st>>>(a.vols, lt, a.tw, nullptr); break;
-        case P_Z_DIPOLE: k_z_tv<N, EG, ZOP_DIPOLE, 1><<<g, b, 0, st>>>(a.vols, lt, a.tw, nullptr); break;
-        case P_Z_FILTER: k_z_tv<N, EG, ZOP_FILTER, 1><<<g, b, 0, st>>>(a.vols, lt, a.tw, nullptr); break;
-        case P_Z_RTS_SETUP: k_z_tv<N, EG, ZOP_RTS_SETUP, 1><<<g, b, 0, st>>>(a.vols, lt, a.tw, nullptr); break;
+        case P_Z_TV_APPLYA: k_z_tv<N, EG, ZOP_TV_APPLYA, 1, TW><<<g, b, 0, st>>>(a.vols, lt, a.tw, nullptr); break;
+        case P_Z_DIPOLE: k_z_tv<N, EG, ZOP_DIPOLE, 1, TW><<<g, b, 0, st>>>(a.vols, lt, a.tw, nullptr); break;
+        case P_Z_FILTER: k_z_tv<N, EG, ZOP_FILTER, 1, TW><<<g, b, 0, st>>>(a.vols, lt, a.tw, nullptr); break;
+        case P_Z_RTS_SETUP: k_z_tv<N, EG, ZOP_RTS_SETUP, 1, TW><<<g, b, 0, st>>>(a.vols, lt, a.tw, nullptr); break;
         case P_Z_TV_SOLVE0:
-            if (a.sched) k_z_tv<N, EG, ZOP_TV_SOLVE0, 1><<<g, b, 0, st>>>(a.vols, lt, a.tw, a.status);
-            else k_z_tv<N, EG, ZOP_TV_SOLVE0, S0><<<g, b, 0, st>>>(a.vols, lt, a.tw, a.status);
+            if (a.sched) k_z_tv<N, EG, ZOP_TV_SOLVE0, 1, TW><<<g, b, 0, st>>>(a.vols, lt, a.tw, a.status);
+            else k_z_tv<N, EG, ZOP_TV_SOLVE0, S0, TW><<<g, b, 0, st>>>(a.vols, lt, a.tw, a.status);
             break;
         case P_Z_RTS_SOLVE:
-            if (a.sched) k_z_tv<N, EG, ZOP_RTS_SOLVE, 1><<<g, b, 0, st>>>(a.vols, lt, a.tw, a.status);
-            else k_z_tv<N, EG, ZOP_RTS_SOLVE, S0><<<g, b, 0, st>>>(a.vols, lt, a.tw, a.status);
+            if (a.sched) k_z_tv<N, EG, ZOP_RTS_SOLVE, 1, TW><<<g, b, 0, st>>>(a.vols, lt, a.tw, a.status);
+            else k_z_tv<N, EG, ZOP_RTS_SOLVE, S0, TW><<<g, b, 0, st>>>(a.vols, lt, a.tw, a.status);
             break;
-        case P_Z_VS_FWD: k_z_vs<N, EG, ZOP_VS_FWD><<<g, b, 0, st>>>(a.vols, lt, a.tw, a.sel, 0, 0); break;
-        case P_Z_VS_RADIUS: k_z_vs<N, EG, ZOP_VS_RADIUS><<<g, b, 0, st>>>(a.vols, lt, a.tw, 0, a.ridx, a.first); break;
-        case P_Z_VS_DECONV: k_z_vs<N, EG, ZOP_VS_DECONV><<<g, b, 0, st>>>(a.vols, lt, a.tw, 0, 0, a.divide); break;
+        case P_Z_VS_FWD: k_z_vs<N, EG, ZOP_VS_FWD, TW><<<g, b, 0, st>>>(a.vols, lt, a.tw, a.sel, 0, 0); break;
+        case P_Z_VS_RADIUS: k_z_vs<N, EG, ZOP_VS_RADIUS, TW><<<g, b, 0, st>>>(a.vols, lt, a.tw, 0, a.ridx, a.first); break;
+        case P_Z_VS_DECONV: k_z_vs<N, EG, ZOP_VS_DECONV, TW><<<g, b, 0, st>>>(a.vols, lt, a.tw, 0, 0, a.divide); break;
         default: break;
     }
 }
@@ -133,7 +133,8 @@ void launch_generic(int kind, int emax, const LaunchTable& lt, const PassArgs& a
     if (is_row_kind(kind)) launch_rows<0, 16>(kind, lt, a);
     else if (emax <= 16) launch_cols<0, 16>(kind, lt, a);
     else if (emax <= 24) launch_cols<0, 24>(kind, lt, a);
-    else launch_cols<0, 32>(kind, lt, a);
+    else if (emax <= 32) launch_cols<0, 32>(kind, lt, a);
+    else launch_cols<0, 32, kTileLong>(kind, lt, a);  // lengths above kWideTileMaxLen
 }
 #endif
 

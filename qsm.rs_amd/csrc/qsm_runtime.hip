@@ -62,7 +62,7 @@ int fail(int code, const char* fmt, ...) {
     } while (0)
 
 // -------------------------------------------------------------- planning --
-constexpr int kMaxLen = 512;
+constexpr int kMaxLen = qsmfft::kMaxDirectLen;
 using qsmfft::direct_length;
 using qsmfft::chirp_length;
 using qsmfft::chirp_tables;
@@ -76,11 +76,16 @@ bool length_ok(size_t n) {
 // convolution length), which sizes the LDS tiles.
 int work_length(int n) { return direct_length(n) ? n : chirp_length(n); }
 
-// EMAX (elements per thread) needed for a column pass of length n.
+// EMAX (elements per thread) needed for a column pass of length n in tiles
+// of kTile columns; above kWideTileMaxLen the pass runs in tiles of
+// kTileLong columns (EMAX 32), marked by the value 64.
 int emax_for_col(int n) {
     const int w = work_length(n);
-    return w <= 256 ? 16 : (w <= 384 ? 24 : 32);
+    return w <= 256 ? 16 : (w <= 384 ? 24 : (w <= kWideTileMaxLen ? 32 : 64));
 }
+
+// Columns per tile of a column pass of length n.
+int col_tile(int n) { return work_length(n) <= kWideTileMaxLen ? kTile : kTileLong; }
 
 // exp(-2 pi i q/n) from an exactly reduced angle, rounded once
 inline float2 twiddle_entry(int q, int n) {
@@ -118,8 +123,8 @@ struct TwiddleBank {
     }
 };
 
-// The same tables for one volume (three axes, each at most a length-512
-// table, or a length-M table plus a chirp of n <= 256 and a filter of M)
+// The same tables for one volume (three axes, each at most a length-1024
+// table, or a length-M table plus a chirp of n <= 512 and a filter of M)
 // in fixed storage, for the entry point that allocates nothing.
 struct FixedTwiddleBank {
     static constexpr int kCap = 3 * (2 * kMaxLen + kMaxChirpLen);
@@ -216,6 +221,7 @@ PassBlocks fill_geometry(VolDesc& V, const qsm_grid& g, TB& tb, int xshift = 0, 
     V.rows_x = default_rows_x(V.nx) >> xshift;
     V.rows_c = std::max(1, (default_rows_x(V.nx) >> cshift) / 2);
     V.ntx = (V.nxh + kTile - 1) / kTile;
+    V.ntx_long = (V.nxh + kTileLong - 1) / kTileLong;
     V.m_nx = qsmfft::magic_for((uint32_t)V.nx);
     V.m_nxh = qsmfft::magic_for((uint32_t)V.nxh);
     V.m_ny = qsmfft::magic_for((uint32_t)V.ny);
@@ -228,8 +234,8 @@ PassBlocks fill_geometry(VolDesc& V, const qsm_grid& g, TB& tb, int xshift = 0, 
     const int npairs = (V.ny * V.nz + 1) / 2;
     pb.xpass = (npairs + V.rows_x - 1) / V.rows_x;
     pb.cpass = (npairs + V.rows_c - 1) / V.rows_c;
-    pb.ypass = V.ntx * V.nz;
-    pb.zpass = V.ntx * V.ny;
+    pb.ypass = (col_tile(V.ny) == kTile ? V.ntx : V.ntx_long) * V.nz;
+    pb.zpass = (col_tile(V.nz) == kTile ? V.ntx : V.ntx_long) * V.ny;
     const size_t N = (size_t)V.nx * V.ny * V.nz;
     pb.out = (int)((N + 1023) / 1024);
     return pb;
@@ -1588,7 +1594,7 @@ int qsm_dipole_adjoint(const double* d_field, const qsm_grid* grid, const double
 
 // ------------------------------------- dipole operator, no allocation ----
 // Workspace of qsm_dipole_forward_noalloc in a file-scope device array: the
-// volume descriptor, the twiddle tables (three lengths <= 512) and one half
+// volume descriptor, the twiddle tables (three lengths <= 1024) and one half
 // spectrum.  Each device holds its own copy until the process ends.  Host
 // code never passes `scratch` itself to a kernel or a copy: its device
 // address comes from hipGetSymbolAddress, once per device.

@@ -133,9 +133,11 @@ static double run_bluestein(int n, int ncols) {
 
 static int check_bluestein(int n) {
     const int M = chirp_length(n);
-    // column tiles of 16 (EMAX 32: 16 M <= 8192); rows as the x passes size them
-    double e1 = run_bluestein<32, true, 4, false>(n, 16);
-    double e2 = run_bluestein<32, true, 4, true>(n, 16);
+    // column tiles of 16 (EMAX 32: 16 M <= 8192), of 8 for M above 512; rows
+    // as the x passes size them
+    const bool wide = M <= 512;
+    double e1 = wide ? run_bluestein<32, true, 4, false>(n, 16) : run_bluestein<32, true, 3, false>(n, 8);
+    double e2 = wide ? run_bluestein<32, true, 4, true>(n, 16) : run_bluestein<32, true, 3, true>(n, 8);
     int rows = (kThreads * 16) / M; if (rows < 1) rows = 1; if (rows > 32) rows = 32;
     double e3 = run_bluestein<16, false, 0, false>(n, rows);
     double e4 = run_bluestein<16, false, 0, true>(n, rows);
@@ -331,7 +333,17 @@ int main() {
     // lengths with a prime factor above 13 (17, 19, 23, 68 = 4 x 17, 97,
     // 164 = 4 x 41, 232 = 8 x 29, 251, 256 - 2 = 254 = 2 x 127) and a direct
     // length run through the same path
-    for (int n : {17, 19, 23, 68, 97, 164, 232, 251, 254, 30}) bad += check_bluestein(n);
+    for (int n : {17, 19, 23, 68, 97, 164, 232, 251, 254, 30, 263, 293, 509}) bad += check_bluestein(n);
+    // direct lengths above 512: column tiles of 8 (EMAX 32: 8 n <= 8192), rows of 4
+    for (int n : {520, 576, 640, 729, 768, 1000, 1024}) {
+        double e1 = run_case<32, true, 3, false>(n, 8), e2 = run_case<32, true, 3, true>(n, 8);
+        double e3 = run_case<16, false, 0, false>(n, 4), e4 = run_case<16, false, 0, true>(n, 4);
+        double worst = std::fmax(std::fmax(e1, e2), std::fmax(e3, e4));
+        bool ok = e1 >= 0 && e2 >= 0 && e3 >= 0 && e4 >= 0 && worst < 2e-6;
+        printf("n=%4d col8fwd=%.2e col8inv=%.2e rowfwd=%.2e rowinv=%.2e %s\n", n, e1, e2, e3, e4,
+               ok ? "ok" : "FAIL");
+        if (!ok) ++bad;
+    }
     bad += bad_fixed + bad_col2;
     printf("%s\n", bad ? "FAILED" : "ALL OK");
     return bad ? 1 : 0;

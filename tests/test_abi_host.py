@@ -128,7 +128,13 @@ def test_bad_parameters_are_errors_not_hangs():
     # transform), and lengths up to 512 that factor into 2, 3, 5, 7, 11, 13
     assert qsm.fft_length_supported(17) and qsm.fft_length_supported(384)
     assert qsm.fft_length_supported(68) and qsm.fft_length_supported(251)
-    assert not qsm.fft_length_supported(257 * 2) and not qsm.fft_length_supported(1024)
+    # direct lengths up to 1024 (576 = 2^6 3^2, 640, 1000 = 2^3 5^3), chirp-z up to 512
+    assert qsm.fft_length_supported(576) and qsm.fft_length_supported(640)
+    assert qsm.fft_length_supported(1000) and qsm.fft_length_supported(1024)
+    assert qsm.fft_length_supported(509) and qsm.fft_length_supported(263)
+    # above 512 only 13-smooth lengths; nothing above 1024
+    assert not qsm.fft_length_supported(257 * 2) and not qsm.fft_length_supported(521)
+    assert not qsm.fft_length_supported(1026) and not qsm.fft_length_supported(2048)
     assert not qsm.fft_length_supported(0)
 
 

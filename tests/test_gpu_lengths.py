@@ -5,7 +5,9 @@ row passes (R2C / C2R), y and z through the column passes.
 
 Lengths: 17 and 97 (primes), 68 = 4 x 17, 164 = 4 x 41, 232 = 8 x 29 (the
 matrix sizes a drop-in must accept, where the reference computes a result),
-251 (the largest prime below the chirp-z limit of 256), 254 = 2 x 127."""
+251, 254 = 2 x 127; above 256 the chirp-z primes 263 and 509 (convolution
+length 1024), and the direct lengths above 512 (576, 640, 1000, 1024), whose
+column passes run in tiles of 8 columns."""
 import numpy as np
 import pytest
 
@@ -87,6 +89,83 @@ def test_vsharp_chirp_lengths(qsm, oracle, dims):
 def test_dipole_chirp_lengths(qsm, oracle, dims, bdir):
     # the dipole forward model Re IFFT(D FFT(chi)) (src/kernels/dipole.rs:24-72)
     rng = np.random.default_rng(7)
+    chi = rng.standard_normal(int(np.prod(dims))) * 0.05
+    vs = (0.9, 1.0, 1.1)
+    out = qsm.dipole_forward(chi, qsm.Grid(dims, vs), bdir)
+    ref = oracle.dipole_forward(chi, dims, vs, bdir)
+    err = float(np.max(np.abs(out - ref)) / np.max(np.abs(ref)))
+    print(dims, f"max rel err {err:.3e}", flush=True)
+    assert err < 2e-5, err
+
+
+LONG = [576, 640, 1000, 1024]
+CHIRP_LONG = [263, 509]
+
+
+@pytest.mark.parametrize("n", LONG + CHIRP_LONG)
+def test_long_lengths_supported(qsm, n):
+    assert qsm.fft_length_supported(n)
+
+
+@pytest.mark.parametrize("n", [576, 1024, 509])
+def test_tv_long_length_each_axis(qsm, oracle, n):
+    # TV-ADMM, 5 iterations, tol 0, tilted B0, isotropic voxels: raw NRMSE
+    rng = np.random.default_rng(n + 1)
+    for dims in [(n, 4, 3), (3, n, 4), (4, 3, n)]:
+        f, m = _case(rng, dims)
+        bdir = (0.1, -0.2, 1.0)
+        p = qsm.TvParams(lambda_=1e-3, rho=0.1, tol=0.0, max_iter=5)
+        x = qsm.tv_admm(f, m, qsm.Grid(dims, (1.0, 1.0, 1.0)), bdir, p)
+        xo = oracle.tv_admm(f, m, dims, (1.0, 1.0, 1.0), bdir, 1e-3, 0.1, 0.0, 5)
+        raw = float(oracle.nrmse(x, xo, m))
+        print(dims, f"nrmse raw {raw:.3e}", flush=True)
+        assert raw < 2e-5, (dims, raw)
+        assert np.all(x[m == 0] == 0.0)
+
+
+@pytest.mark.parametrize("dims", [(640, 12, 10), (10, 1000, 8), (12, 8, 576), (263, 20, 16)])
+def test_tv_long_lengths_mixed(qsm, oracle, dims):
+    # 20 iterations, anisotropic voxels: DC-removed NRMSE asserted, raw printed
+    rng = np.random.default_rng(sum(dims))
+    f, m = _case(rng, dims)
+    vs, bdir = (0.9, 1.0, 1.1), (0.0, 0.1, 1.0)
+    p = qsm.TvParams(lambda_=1e-3, rho=0.1, tol=0.0, max_iter=20)
+    x = qsm.tv_admm(f, m, qsm.Grid(dims, vs), bdir, p)
+    xo = oracle.tv_admm(f, m, dims, vs, bdir, 1e-3, 0.1, 0.0, 20)
+    raw = float(oracle.nrmse(x, xo, m))
+    dc = nrmse_dc(x, xo, m)
+    print(dims, f"nrmse raw {raw:.3e} dc-removed {dc:.3e}", flush=True)
+    assert dc < 2e-5, dc
+
+
+@pytest.mark.parametrize("dims", [(576, 32, 24), (24, 640, 20), (20, 24, 509)])
+def test_vsharp_long_lengths(qsm, oracle, dims):
+    # V-SHARP: eroded mask identical, local field within 1e-4 of its largest
+    # value.  The GPU works in FP32, so its error follows the range of the
+    # background it removes: the ramp spans +-0.32..0.64 rad along every axis
+    # (as over the 64-voxel axes of the other V-SHARP tests), not 0.02 rad
+    # per voxel over 640 voxels.
+    g = [np.arange(n) - n // 2 for n in dims]
+    X, Y, Z = np.meshgrid(*g, indexing="ij")
+    r = [0.35 * n for n in dims]
+    mask = (((X / r[0]) ** 2 + (Y / r[1]) ** 2 + (Z / r[2]) ** 2) <= 1.0).astype(np.uint8).ravel(order="F")
+    rng = np.random.default_rng(dims[1])
+    sx, sy, sz = (64.0 / n for n in dims)
+    field = (0.01 * sx * X + 0.02 * sy * Y - 0.01 * sz * Z).ravel(order="F") + rng.standard_normal(mask.size) * 0.01
+    vs = (1.0, 1.0, 1.0)
+    radii = [3.0, 2.0]
+    local, em = qsm.vsharp_with_radii(field, mask, qsm.Grid(dims, vs), radii, 0.05)
+    lo, emo = oracle.vsharp_with_radii(field, mask, dims, vs, radii, 0.05)
+    assert np.array_equal(em, emo), int(np.sum(em != emo))
+    sel = emo > 0
+    err = float(np.max(np.abs(local[sel] - lo[sel])) / np.max(np.abs(lo[sel])))
+    print(dims, f"max rel err {err:.3e}", flush=True)
+    assert err < 1e-4, err
+
+
+@pytest.mark.parametrize("dims,bdir", [((1024, 6, 4), (0.2, 0.1, 1.0)), ((6, 4, 768), (0.0, 0.3, 1.0))])
+def test_dipole_long_lengths(qsm, oracle, dims, bdir):
+    rng = np.random.default_rng(11)
     chi = rng.standard_normal(int(np.prod(dims))) * 0.05
     vs = (0.9, 1.0, 1.1)
     out = qsm.dipole_forward(chi, qsm.Grid(dims, vs), bdir)
