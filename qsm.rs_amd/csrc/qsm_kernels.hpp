@@ -330,6 +330,15 @@ __device__ __forceinline__ void col_store(const float2* lds, QSM_GLOBAL float2* 
 // their results are never stored) and are not written.
 static_assert(kTile == 16 && kThreads == 256, "fft_col2.hpp assumes 16-column tiles and 256 threads");
 
+// The TV / RTS z-pass operands f^ (SPEC_A) and q are written once at setup
+// and read at every iteration by the same z-pass block, so they are stored
+// tile-major: the tile of TW columns at (tx, ky) holds its nz rows of TW
+// elements contiguously.  Element (kx, kz) of that tile is at
+// zop_base(...) + kx + kz * TW.
+__host__ __device__ inline size_t zop_base(int ky, int tx, int ntx, int nz, int TW) {
+    return ((size_t)ky * ntx + tx) * (size_t)nz * TW - (size_t)tx * TW;
+}
+
 template <int R0>
 __device__ __forceinline__ void l16_load(float2 (&v)[16], const QSM_GLOBAL float2* __restrict__ base,
                                          size_t stride, int kx0, int nxh) {
@@ -1383,7 +1392,7 @@ __device__ __forceinline__ void z_tv_col2(const VolDesc& V, float2* lds, float2*
     float2 v[16];
     if constexpr (OP == ZOP_TV_SOLVE0) {
         // g = 0 before the first step (AdmmBuffers::new), so div = 0 and x^ = f^
-        l16_load<R0>(v, glb(V.spec[SPEC_A]) + base, zs, kx0, nxh);
+        l16_load<R0>(v, glb(V.spec[SPEC_A]) + zop_base(ky, kx0 / kTile, V.ntx, N, kTile), kTile, kx0, nxh);
         tws.commit(twl);
         C::template l16_to_lr<true>(v, lds, twl);
         lr_store<R0>(v, buf, zs, kx0, nxh);
@@ -1400,13 +1409,14 @@ __device__ __forceinline__ void z_tv_col2(const VolDesc& V, float2* lds, float2*
     {
         // threads j >= R0 load a copy of row j mod R0 (unused): loads under a
         // branch would make the compiler wait for every load before the FFT
-        const size_t off = base + min(kx0 + c, nxh - 1) + (size_t)(act ? j : j % R0) * zs;
+        const size_t off = zop_base(ky, kx0 / kTile, V.ntx, N, kTile) + min(kx0 + c, nxh - 1) +
+                           (size_t)(act ? j : j % R0) * kTile;
         const QSM_GLOBAL float2* __restrict__ F = glb(V.spec[SPEC_A]) + off;
         const QSM_GLOBAL float* __restrict__ Q = glb(V.q) + off;
 #pragma unroll
         for (int r = 0; r < 16; ++r) {
-            if constexpr (HAS_F) fr[r] = ldg2(F + (size_t)(R0 * r) * zs);
-            if constexpr (Q_MEM) qr[r] = Q[(size_t)(R0 * r) * zs];
+            if constexpr (HAS_F) fr[r] = ldg2(F + R0 * r * kTile);
+            if constexpr (Q_MEM) qr[r] = Q[R0 * r * kTile];
         }
     }
     tws.commit(twl);
@@ -1490,6 +1500,8 @@ __global__ __launch_bounds__(kThreads) void k_z_tv(const VolDesc* __restrict__ v
     const size_t zs = (size_t)nxh * V.ny;
     const size_t base = (size_t)ky * nxh;
     const int total = nz * TW;
+    // f^ / q operands (tile-major): element (kx, kz) at fb + kx + kz * TW
+    const size_t fb = zop_base(ky, tx, ntx, nz, TW);
     // column 0 holds two z spectra (the DC and Nyquist planes at ky = 0, ny/2)
     const bool dpk = pk_double(V, tx, ky);
     QSM_GLOBAL float2* buf = glb(V.spec[SPEC_WORK]) + base;
@@ -1501,7 +1513,7 @@ __global__ __launch_bounds__(kThreads) void k_z_tv(const VolDesc* __restrict__ v
     }
     if constexpr (OP == ZOP_TV_SOLVE0) {
         // g = 0 before the first step (AdmmBuffers::new), so div = 0 and x^ = f^
-        col_load<EMAX, TW>(lds, glb(V.spec[SPEC_A]) + base, zs, nz, kx0, nxh);
+        col_load<EMAX, TW>(lds, glb(V.spec[SPEC_A]) + fb, TW, nz, kx0, nxh);
         tws.commit(twl);
         __syncthreads();
         run_fft<N, EMAX, true, L2, true>(lds, V.pz, TW, TW, 1, twl, tw);
@@ -1515,16 +1527,16 @@ __global__ __launch_bounds__(kThreads) void k_z_tv(const VolDesc* __restrict__ v
     float2 fr[HAS_F ? EMAX : 1];
     float qr[Q_MEM ? EMAX : 1];
     if constexpr (HAS_F || Q_MEM) {
-        const QSM_GLOBAL float2* __restrict__ F = glb(V.spec[SPEC_A]) + base;
-        const QSM_GLOBAL float* __restrict__ Q = glb(V.q) + base;
+        const QSM_GLOBAL float2* __restrict__ F = glb(V.spec[SPEC_A]) + fb;
+        const QSM_GLOBAL float* __restrict__ Q = glb(V.q) + fb;
 #pragma unroll
         for (int t = 0; t < EMAX; ++t) {
             const int e = threadIdx.x + t * kThreads;
             const int kz = e >> L2;
             const int kx = kx0 + (e & (TW - 1));
             const bool in = e < total && kx < nxh;
-            if constexpr (HAS_F) fr[t] = in ? ldg2(F + kx + kz * zs) : make_float2(0.0f, 0.0f);
-            if constexpr (Q_MEM) qr[t] = in ? Q[kx + kz * zs] : 0.0f;
+            if constexpr (HAS_F) fr[t] = in ? ldg2(F + kx + kz * TW) : make_float2(0.0f, 0.0f);
+            if constexpr (Q_MEM) qr[t] = in ? Q[kx + kz * TW] : 0.0f;
         }
     }
     tws.commit(twl);
@@ -1546,8 +1558,8 @@ __global__ __launch_bounds__(kThreads) void k_z_tv(const VolDesc* __restrict__ v
         __syncthreads();
     }
     if constexpr (OP == ZOP_TV_SETUP || OP == ZOP_RTS_SETUP) {
-        QSM_GLOBAL float2* F = glb(V.spec[SPEC_A]) + base;
-        QSM_GLOBAL float* Q = glb(V.q) + base;
+        QSM_GLOBAL float2* F = glb(V.spec[SPEC_A]) + fb;
+        QSM_GLOBAL float* Q = glb(V.q) + fb;
 #pragma unroll
         for (int t = 0; t < EMAX; ++t) {
             const int e = threadIdx.x + t * kThreads;
@@ -1569,8 +1581,8 @@ __global__ __launch_bounds__(kThreads) void k_z_tv(const VolDesc* __restrict__ v
                     setup_mult<OP>(V, fx, fy, kz, m, qv);
                     fv = make_float2((float)((double)sv.x * m), (float)((double)sv.y * m));
                 }
-                stg2(F + kx + kz * zs, fv);
-                Q[kx + kz * zs] = (float)qv;  // (RTS recomputes q in the doubly packed column)
+                stg2(F + kx + kz * TW, fv);
+                Q[kx + kz * TW] = (float)qv;  // (RTS recomputes q in the doubly packed column)
             }
         }
     } else {

@@ -87,6 +87,14 @@ int emax_for_col(int n) {
 // Columns per tile of a column pass of length n.
 int col_tile(int n) { return work_length(n) <= kWideTileMaxLen ? kTile : kTileLong; }
 
+// Elements of the tile-major TV / RTS z-pass operands (f^, q) of a volume:
+// ny * nz rows of whole z-pass tiles (at least the half spectrum).
+size_t zop_elems(const VolDesc& V) {
+    const int tw = col_tile(V.nz);
+    const size_t cols = (size_t)(tw == kTile ? V.ntx : V.ntx_long) * tw;
+    return cols * V.ny * V.nz;
+}
+
 // exp(-2 pi i q/n) from an exactly reduced angle, rounded once
 inline float2 twiddle_entry(int q, int n) {
     long double a = -2.0L * 3.14159265358979323846264338327950288L * q / n;
@@ -431,10 +439,10 @@ static int tune_column_schedules(qsm_tv_batch* B) {
             return QSM_OK;
         }
     }
-    for (const auto& V : B->h)
-        for (int sidx : {SPEC_WORK, SPEC_A})
-            launch_fill_zero(B->stream, reinterpret_cast<float*>(V.spec[sidx]),
-                             2 * (size_t)V.nxh * V.ny * V.nz);
+    for (const auto& V : B->h) {
+        launch_fill_zero(B->stream, reinterpret_cast<float*>(V.spec[SPEC_WORK]), 2 * (size_t)V.nxh * V.ny * V.nz);
+        launch_fill_zero(B->stream, reinterpret_cast<float*>(V.spec[SPEC_A]), 2 * zop_elems(V));
+    }
     hipEvent_t e0 = nullptr, e1 = nullptr;
     hipError_t e = hipEventCreate(&e0);
     if (e == hipSuccess) e = hipEventCreate(&e1);
@@ -567,11 +575,12 @@ static int tv_batch_create_impl(const qsm_tv_volume* vols, size_t n_vols, const 
             if ((rc = dev_alloc(&V.u[p], 3 * N))) return cleanup(rc);
             B->allocs.push_back(V.u[p]);
         }
-        for (int s : {SPEC_WORK, SPEC_A}) {
-            if ((rc = dev_alloc(&V.spec[s], Nh))) return cleanup(rc);
-            B->allocs.push_back(V.spec[s]);
-        }
-        if ((rc = dev_alloc(&V.q, Nh))) return cleanup(rc);
+        if ((rc = dev_alloc(&V.spec[SPEC_WORK], Nh))) return cleanup(rc);
+        B->allocs.push_back(V.spec[SPEC_WORK]);
+        // f^ and q: tile-major z-pass operands (zop_base in the kernels)
+        if ((rc = dev_alloc(&V.spec[SPEC_A], zop_elems(V)))) return cleanup(rc);
+        B->allocs.push_back(V.spec[SPEC_A]);
+        if ((rc = dev_alloc(&V.q, zop_elems(V)))) return cleanup(rc);
         B->allocs.push_back(V.q);
         if ((rc = dev_alloc(&V.partial, 2 * (size_t)pb.cpass))) return cleanup(rc);
         B->allocs.push_back(V.partial);

@@ -250,11 +250,11 @@ def test_fft_schedules_agree_with_oracle(qsm, oracle):
 
 
 def test_tv_rejects_bad_input(qsm):
-    # 263 is a prime above the chirp-z limit (256): unsupported
-    grid = qsm.Grid((263, 4, 4), (1.0, 1.0, 1.0))
+    # 521 is a prime above the chirp-z limit (512): unsupported
+    grid = qsm.Grid((521, 4, 4), (1.0, 1.0, 1.0))
     p = qsm.TvParams()
     with pytest.raises(qsm.QsmError) as ei:
-        qsm.tv_admm(np.zeros(263 * 16), np.ones(263 * 16, np.uint8), grid, (0, 0, 1), p)
+        qsm.tv_admm(np.zeros(521 * 16), np.ones(521 * 16, np.uint8), grid, (0, 0, 1), p)
     assert ei.value.code == 6
     with pytest.raises(ValueError):
         qsm.tv_admm(np.zeros(10), np.ones(10, np.uint8), qsm.Grid((4, 4, 4)), (0, 0, 1), p)
