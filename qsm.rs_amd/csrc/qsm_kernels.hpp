@@ -17,6 +17,8 @@
 // over a compile-time count into registers); twiddles are staged in LDS.
 #pragma once
 
+#include <type_traits>
+
 #include "fft_col2.hpp"
 #include "qsm_internal.hpp"
 
@@ -622,90 +624,108 @@ __device__ __forceinline__ float lane_prev(float v) {
                                                                  false));
 }
 
-// Row lengths and pair-rows per block (RB) the wave-row form takes: whole
-// 64-column groups, at most 8 per lane, and RB / 2 rows per wave even.
+// Row lengths and pair-rows per block (RB) the wave-row form takes: at
+// least one pair-row per wave, at most 4 column groups of 64 per lane.
+// Measured: faster than the element-wise fill at 144 ... 256; at 384 (6
+// groups, 222 VGPRs) slower, and with 7 or 8 groups its two row buffers take
+// all 256 VGPRs (one wave per SIMD, where the element-wise fill keeps two).
 __host__ __device__ constexpr bool tvdiv_rows_form(int n, int rb) {
-    return n > 0 && n % 64 == 0 && n / 64 <= 8 && rb % 4 == 0;
+    return n > 0 && n <= 256 && rb >= 4;
 }
 
-// One row of the wave-row form below: the values at the row's own voxels.
-template <int C>
+// One row of the wave-row form below: the values at the row's own voxels,
+// columns lane + 64 m (m < C = ceil(NX / 64); in a partial last group the
+// lanes past NX hold column NX - 1 and are never stored).
+template <int NX>
 struct TvRow {
+    static constexpr int C = (NX + 63) / 64;
     float xc[C], ux[C], uy[C], uz[C], xzp[C], xzm[C], uzm[C];
 };
 
+template <int NX>
+__device__ __forceinline__ int tv_col(int lane, int m) {
+    return (NX % 64 != 0 && m == (NX + 63) / 64 - 1) ? min(lane + 64 * m, NX - 1) : lane + 64 * m;
+}
+
 // Row r of the volume (r = j + k ny, k the plane): its own x and u, x on the
-// planes k +- 1 and u_z on plane k - 1 (periodic), columns lane + 64 m.
-template <int C>
-__device__ __forceinline__ void tvrow_load(TvRow<C>& R, int r, int k, int nz, int nxy, int N, int lane,
+// planes k +- 1 and u_z on plane k - 1 (periodic).
+template <int NX>
+__device__ __forceinline__ void tvrow_load(TvRow<NX>& R, int r, int k, int nz, int nxy, int N, int lane,
                                            const QSM_GLOBAL float* __restrict__ x,
                                            const QSM_GLOBAL float* __restrict__ u) {
-    constexpr int nx = 64 * C;
-    const int base = r * nx;
+    constexpr int C = TvRow<NX>::C;
+    const int base = r * NX;
     const int dzp = (k + 1 < nz) ? nxy : nxy - N;
     const int dzm = (k > 0) ? -nxy : N - nxy;
     const QSM_GLOBAL float* xr = x + base;
     const QSM_GLOBAL float* ur = u + base;
 #pragma unroll
-    for (int m = 0; m < C; ++m) R.xc[m] = xr[lane + 64 * m];
+    for (int m = 0; m < C; ++m) R.xc[m] = xr[tv_col<NX>(lane, m)];
 #pragma unroll
     for (int m = 0; m < C; ++m) {
-        R.ux[m] = ur[lane + 64 * m];
-        R.uy[m] = ur[N + lane + 64 * m];
-        R.uz[m] = ur[2 * N + lane + 64 * m];
-        R.xzp[m] = xr[dzp + lane + 64 * m];
-        R.xzm[m] = xr[dzm + lane + 64 * m];
-        R.uzm[m] = ur[2 * N + dzm + lane + 64 * m];
+        const int i = tv_col<NX>(lane, m);
+        R.ux[m] = ur[i];
+        R.uy[m] = ur[N + i];
+        R.uz[m] = ur[2 * N + i];
+        R.xzp[m] = xr[dzp + i];
+        R.xzm[m] = xr[dzm + i];
+        R.uzm[m] = ur[2 * N + dzm + i];
     }
 }
 
-// fill_tvdiv for rows of nx = 64 C with the block's rows split over its
-// waves: wave w takes RPW consecutive rows (RPW even, so a pair-row never
-// spans two waves), lane l the columns l + 64 m.  Row indices are then the
-// same in every lane (scalar), the in-row neighbours (x at i +- 1, u_x at
-// i - 1) come from lane rotations and the y neighbours from the wave's
-// neighbouring rows in registers; only the rows above and below the wave's
-// rows, and y-wrapping rows, are loaded again.  Loads per voxel: 7 plus
-// 3 / RPW (against 13).  The arithmetic per voxel is that of tvdiv_compute.
-template <int C, int RPW>
+// fill_tvdiv with the block's RB pair-rows split over its four waves: wave w
+// takes pair-rows [w RB / 4, (w + 1) RB / 4) (at most PMAX), lane l the
+// columns l + 64 m.  Row indices are then the same in every lane (scalar),
+// the in-row neighbours (x at i +- 1, u_x at i - 1) come from lane rotations
+// (the periodic wrap between columns NX - 1 and 0 from lane broadcasts) and
+// the y neighbours from the wave's neighbouring rows in registers; only the
+// rows above and below the wave's rows, and y-wrapping rows, are loaded
+// again.  Loads per voxel: 7 plus 3 / (rows per wave), against 13.  The
+// arithmetic per voxel is that of tvdiv_compute.
+template <int NX, int RB>
 __device__ __forceinline__ void fill_tvdiv_rows(float2* lds, const VolDesc& V, int row0, int nrows,
                                                 const QSM_GLOBAL float* __restrict__ x,
                                                 const QSM_GLOBAL float* __restrict__ u,
                                                 QSM_GLOBAL float* __restrict__ uo) {
-    constexpr int nx = 64 * C;
-    static_assert(RPW % 2 == 0, "pair-rows must not span waves");
+    constexpr int C = TvRow<NX>::C;
+    constexpr int LV = NX - 64 * (C - 1);  // valid lanes of the last column group
+    constexpr int PMAX = (RB + 3) / 4;
     static_assert(kThreads == 256, "four waves per block");
     const int ny = V.ny, nz = V.nz;
-    const int nxy = nx * ny, N = nxy * nz;
+    const int nxy = NX * ny, N = nxy * nz;
     const float t = V.thr, hx = V.hx, hy = V.hy, hz = V.hz;
     const int lane = (int)(threadIdx.x & 63);
     const int wv = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
-    const int r0 = row0 + wv * RPW;
+    // pair-rows [p0, p0 + nrw / 2) of this wave; the same count for every
+    // wave (a compile-time constant, so the row loop unrolls) when 4 | RB
+    const int p0 = (RB % 4 == 0) ? wv * (RB / 4) : (wv * RB) >> 2;
+    const int nrw = (RB % 4 == 0) ? RB / 2 : 2 * ((((wv + 1) * RB) >> 2) - p0);
+    const int r0 = row0 + 2 * p0;
     // plane and in-plane index of the wave's first row, then stepped
     int k = r0 / ny;
     int j = r0 - k * ny;
-    TvRow<C> A, B;
+    TvRow<NX> A, B;
     float hym_x[C], hym_uy[C], hyp_x[C];  // rows above / below the wave's rows
     if (r0 < nrows) {
-        tvrow_load<C>(A, r0, k, nz, nxy, N, lane, x, u);
+        tvrow_load<NX>(A, r0, k, nz, nxy, N, lane, x, u);
         // the row before the wave's first row, periodic in y
         const int ra = j > 0 ? r0 - 1 : r0 + ny - 1;
-        const QSM_GLOBAL float* xr = x + ra * nx;
-        const QSM_GLOBAL float* ur = u + N + ra * nx;
+        const QSM_GLOBAL float* xr = x + ra * NX;
+        const QSM_GLOBAL float* ur = u + N + ra * NX;
 #pragma unroll
         for (int m = 0; m < C; ++m) {
-            hym_x[m] = xr[lane + 64 * m];
-            hym_uy[m] = ur[lane + 64 * m];
+            hym_x[m] = xr[tv_col<NX>(lane, m)];
+            hym_uy[m] = ur[tv_col<NX>(lane, m)];
         }
     } else {
 #pragma unroll
         for (int m = 0; m < C; ++m) hym_x[m] = hym_uy[m] = 0.0f;
     }
     float pxc[C], puy[C], dev[C];
-#pragma unroll
-    for (int q = 0; q < RPW; ++q) {
-        TvRow<C>& cur = (q & 1) ? B : A;
-        TvRow<C>& nxt = (q & 1) ? A : B;
+    // row q of the wave from cur, with the next row's loads issued into nxt;
+    // ODD: the second row of a pair-row (the pair goes to LDS)
+    auto step = [&](TvRow<NX>& cur, TvRow<NX>& nxt, const int q, auto odd) {
+        constexpr bool ODD = decltype(odd)::value;
         const int r = r0 + q;
         const bool ok = r < nrows;
         // plane / in-plane index of row r + 1
@@ -714,16 +734,16 @@ __device__ __forceinline__ void fill_tvdiv_rows(float2* lds, const VolDesc& V, i
             jn = 0;
             ++kn;
         }
-        if (q + 1 < RPW && r + 1 < nrows) tvrow_load<C>(nxt, r + 1, kn, nz, nxy, N, lane, x, u);
-        if (q + 2 == RPW && ok) {
+        if (q + 1 < nrw && r + 1 < nrows) tvrow_load<NX>(nxt, r + 1, kn, nz, nxy, N, lane, x, u);
+        if (q + 2 == nrw && ok) {
             // the row after the wave's last row rl = r + 1 (in-plane index
             // jn), periodic in y; issued one row ahead
             const int rl = r + 1;
             const bool okl = rl < nrows;
             const int rb = (jn + 1 < ny) ? rl + 1 : rl - ny + 1;
-            const QSM_GLOBAL float* xr = x + (okl ? rb : 0) * nx;
+            const QSM_GLOBAL float* xr = x + (okl ? rb : 0) * NX;
 #pragma unroll
-            for (int m = 0; m < C; ++m) hyp_x[m] = okl ? xr[lane + 64 * m] : 0.0f;
+            for (int m = 0; m < C; ++m) hyp_x[m] = okl ? xr[tv_col<NX>(lane, m)] : 0.0f;
         }
         float xym[C], uym[C], xyp[C];
         if (!ok) {
@@ -746,8 +766,8 @@ __device__ __forceinline__ void fill_tvdiv_rows(float2* lds, const VolDesc& V, i
                 const int ra = r + ny - 1;
 #pragma unroll
                 for (int m = 0; m < C; ++m) {
-                    xym[m] = x[ra * nx + lane + 64 * m];
-                    uym[m] = u[N + ra * nx + lane + 64 * m];
+                    xym[m] = x[ra * NX + tv_col<NX>(lane, m)];
+                    uym[m] = u[N + ra * NX + tv_col<NX>(lane, m)];
                 }
             } else {
 #pragma unroll
@@ -757,20 +777,22 @@ __device__ __forceinline__ void fill_tvdiv_rows(float2* lds, const VolDesc& V, i
                 }
             }
             // y + 1
-            if (q + 1 == RPW) {
+            if (q + 1 == nrw) {
 #pragma unroll
                 for (int m = 0; m < C; ++m) xyp[m] = hyp_x[m];
             } else if (j + 1 == ny) {
                 const int rb = r - ny + 1;
 #pragma unroll
-                for (int m = 0; m < C; ++m) xyp[m] = x[rb * nx + lane + 64 * m];
+                for (int m = 0; m < C; ++m) xyp[m] = x[rb * NX + tv_col<NX>(lane, m)];
             } else {
 #pragma unroll
                 for (int m = 0; m < C; ++m) xyp[m] = nxt.xc[m];
             }
         }
-        // in-row neighbours: lane l + 1 (lane 63: lane 0 of the next
-        // column group, periodic) and lane l - 1
+        // in-row neighbours: lane l + 1 (lane 63: lane 0 of the next column
+        // group) and lane l - 1 (lane 0: lane 63 of the previous group);
+        // columns NX - 1 and 0 are neighbours (periodic), through lane
+        // broadcasts when the last group is partial
         float rn[C], rp[C], rpu[C];
 #pragma unroll
         for (int m = 0; m < C; ++m) {
@@ -778,12 +800,26 @@ __device__ __forceinline__ void fill_tvdiv_rows(float2* lds, const VolDesc& V, i
             rp[m] = lane_prev(cur.xc[m]);
             rpu[m] = lane_prev(cur.ux[m]);
         }
+        // (whole groups: the rotations already wrap column NX - 1 to 0)
+        float x0 = 0.0f, xl = 0.0f, uxl = 0.0f;
+        if constexpr (NX % 64 != 0) {
+            x0 = __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, cur.xc[0]), 0));
+            xl = __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, cur.xc[C - 1]), LV - 1));
+            uxl = __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, cur.ux[C - 1]), LV - 1));
+        }
         float d[C];
 #pragma unroll
         for (int m = 0; m < C; ++m) {
-            const float xp = lane == 63 ? rn[(m + 1) % C] : rn[m];
-            const float xm = lane == 0 ? rp[(m + C - 1) % C] : rp[m];
-            const float uxm = lane == 0 ? rpu[(m + C - 1) % C] : rpu[m];
+            float xp = lane == 63 ? rn[(m + 1) % C] : rn[m];
+            float xm = lane == 0 ? rp[(m + C - 1) % C] : rp[m];
+            float uxm = lane == 0 ? rpu[(m + C - 1) % C] : rpu[m];
+            if constexpr (NX % 64 != 0) {
+                if (m == C - 1 && lane == LV - 1) xp = x0;  // column NX - 1
+                if (m == 0 && lane == 0) {                  // column 0
+                    xm = xl;
+                    uxm = uxl;
+                }
+            }
             const float xc = cur.xc[m];
             const float vx = (xp - xc) * hx + cur.ux[m];
             const float vy = (xyp[m] - xc) * hy + cur.uy[m];
@@ -792,8 +828,9 @@ __device__ __forceinline__ void fill_tvdiv_rows(float2* lds, const VolDesc& V, i
             const float vym = (xc - xym[m]) * hy + uym[m];
             const float vzm = (xc - cur.xzm[m]) * hz + cur.uzm[m];
             const float zx = shrinkf(vx, t), zy = shrinkf(vy, t), zz = shrinkf(vz, t);
-            if (ok) {
-                QSM_GLOBAL float* o = uo + r * nx + lane + 64 * m;
+            const bool col_ok = NX % 64 == 0 || m < C - 1 || lane < LV;
+            if (ok && col_ok) {
+                QSM_GLOBAL float* o = uo + r * NX + lane + 64 * m;
                 o[0] = vx - zx;
                 o[N] = vy - zy;
                 o[2 * N] = vz - zz;
@@ -804,10 +841,13 @@ __device__ __forceinline__ void fill_tvdiv_rows(float2* lds, const VolDesc& V, i
             const float gzm = 2.0f * shrinkf(vzm, t) - vzm;
             d[m] = ok ? (gx - gxm) * hx + (gy - gym) * hy + (gz - gzm) * hz : 0.0f;
         }
-        if (q & 1) {
-            const int p = (wv * RPW + q) >> 1;
+        if constexpr (ODD) {
+            const int p = p0 + (q >> 1);
 #pragma unroll
-            for (int m = 0; m < C; ++m) lds[p * nx + lane + 64 * m] = make_float2(dev[m], d[m]);
+            for (int m = 0; m < C; ++m) {
+                const bool col_ok = NX % 64 == 0 || m < C - 1 || lane < LV;
+                if (col_ok) lds[p * NX + lane + 64 * m] = make_float2(dev[m], d[m]);
+            }
         } else {
 #pragma unroll
             for (int m = 0; m < C; ++m) dev[m] = d[m];
@@ -819,7 +859,15 @@ __device__ __forceinline__ void fill_tvdiv_rows(float2* lds, const VolDesc& V, i
         }
         j = jn;
         k = kn;
+    };
+    // pair-rows of this wave (nrw / 2 <= PMAX, wave-uniform): the row
+    // buffers keep fixed roles, so nothing is indexed at run time
+#pragma unroll
+    for (int s = 0; s < (nrw >> 1); ++s) {
+        step(A, B, 2 * s, std::false_type{});
+        step(B, A, 2 * s + 1, std::true_type{});
     }
+    (void)PMAX;
 }
 
 // First step of an NLTV Newton round after the first (src/inversion/
@@ -1018,8 +1066,7 @@ __global__ __launch_bounds__(kThreads) void k_r2c_x(const VolDesc* __restrict__ 
     TwStage<N> tws;
     tws.fetch(tw, V.px);
     if constexpr (SRC == SRC_TVDIV && tvdiv_rows_form(N, RBX)) {
-        fill_tvdiv_rows<N / 64, RBX / 2>(lds, V, row0, nrows, glb(V.x[aux]), glb(V.u[aux ^ 1]),
-                                         glb(V.u[aux]));
+        fill_tvdiv_rows<N, RBX>(lds, V, row0, nrows, glb(V.x[aux]), glb(V.u[aux ^ 1]), glb(V.u[aux]));
     } else if constexpr (SRC == SRC_TVDIV) {
         fill_tvdiv<EMAX, false>(lds, V, nx, RB, glb(V.x[aux]), glb(V.u[aux ^ 1]), glb(V.u[aux]), row0,
                                 nrows, nullptr);
