@@ -19,8 +19,8 @@ LIB = os.path.join(HERE, "libqsm_gpu.so")
 # qsm_pass.hip is compiled once per specialised axis length (and once with
 # length 0 for the runtime-plan fallback); keep in step with
 # QSM_FIXED_LENGTHS in qsm_misc.hip.
-FIXED_LENGTHS = [64, 96, 112, 128, 144, 160, 176, 180, 192, 200, 208, 216, 220, 224, 240, 256, 288, 320,
-                 384, 416, 448, 480, 512]
+FIXED_LENGTHS = [48, 64, 72, 80, 96, 112, 120, 128, 144, 150, 160, 168, 176, 180, 192, 200, 208, 216, 220,
+                 224, 240, 256, 288, 320, 384, 416, 448, 480, 512]
 UNITS = ([("qsm_pass.hip", f"qsm_pass_{n}", [f"-DQSM_PASS_N={n}"]) for n in [0] + FIXED_LENGTHS]
          + [("qsm_misc.hip", "qsm_misc", []), ("qsm_runtime.hip", "qsm_runtime", []),
             ("qsm_nifti.hip", "qsm_nifti", [])])

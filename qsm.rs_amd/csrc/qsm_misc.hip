@@ -238,8 +238,9 @@ void launch_status_reset(hipStream_t st, VolStatus* s, int nvol) {
 }
 
 // ------------------------------------------------------------ dispatcher --
-#define QSM_FIXED_LENGTHS(X) X(64) X(96) X(112) X(128) X(144) X(160) X(176) X(180) X(192) X(200) X(208) X(216) \
-    X(220) X(224) X(240) X(256) X(288) X(320) X(384) X(416) X(448) X(480) X(512)
+#define QSM_FIXED_LENGTHS(X) X(48) X(64) X(72) X(80) X(96) X(112) X(120) X(128) X(144) X(150) X(160) X(168) \
+    X(176) X(180) X(192) X(200) X(208) X(216) X(220) X(224) X(240) X(256) X(288) X(320) X(384) X(416) X(448) \
+    X(480) X(512)
 #define QSM_DECLARE(n) void launch_fixed_##n(int kind, const LaunchTable& lt, const PassArgs& a);
 QSM_FIXED_LENGTHS(QSM_DECLARE)
 void launch_generic(int kind, int emax, const LaunchTable& lt, const PassArgs& a);

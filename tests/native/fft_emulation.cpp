@@ -311,7 +311,9 @@ int main() {
                     check_fixed<256>() + check_fixed<384>() + check_fixed<96>() + check_fixed<112>() +
                     check_fixed<180>() + check_fixed<200>() + check_fixed<216>() + check_fixed<220>() +
                     check_fixed<320>() + check_fixed<512>() + check_fixed<288>() +
-                    check_fixed<416>() + check_fixed<448>() + check_fixed<480>();
+                    check_fixed<416>() + check_fixed<448>() + check_fixed<480>() + check_fixed<48>() +
+                    check_fixed<72>() + check_fixed<80>() + check_fixed<120>() + check_fixed<150>() +
+                    check_fixed<168>();
     const int lens[] = {1, 2, 3, 4, 5, 6, 7, 8, 10, 11, 12, 13, 14, 16, 22, 26, 32, 64,
                         72, 80, 88, 96, 104, 112, 120, 128, 144, 160, 176, 192, 208, 224,
                         240, 256, 384};

@@ -41,6 +41,21 @@ def test_noalloc_equals_dipole_forward_twice(qsm):
         assert np.array_equal(out, ref), dims
 
 
+def test_noalloc_long_and_chirp_lengths(qsm):
+    # the fixed twiddle storage at its largest: a 1000-point axis (direct,
+    # 8-column z tiles) and a 509-point chirp-z axis (convolution length
+    # 1024) in one volume, then a smaller call with a 263-point chirp-z axis
+    rng = np.random.default_rng(78)
+    cases = [((1000, 12, 509), (1.0, 0.9, 1.1), (0.1, 0.2, 1.0)),
+             ((20, 263, 16), (1.0, 1.0, 1.0), (0.0, 0.3, 1.0))]
+    for dims, vs, bdir in cases:
+        chi = rng.standard_normal(int(np.prod(dims))) * 0.05
+        ref = _dipole(qsm, "qsm_dipole_forward", chi, dims, vs, bdir)
+        out = _dipole(qsm, "qsm_dipole_forward_noalloc", chi, dims, vs, bdir)
+        print(f"{dims}: max |noalloc - forward| = {np.max(np.abs(out - ref)):.3e}", flush=True)
+        assert np.array_equal(out, ref), dims
+
+
 def test_noalloc_rejects_volumes_beyond_its_workspace(qsm):
     # 512 x 512 x 128 needs a 134 MB half spectrum, more than the fixed 72 MiB:
     # refused before any device work, with the message naming the limit
