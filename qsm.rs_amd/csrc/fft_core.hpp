@@ -313,14 +313,12 @@ template <int R, bool COLFAST, int LOG2C, bool INV>
 __host__ __device__ __forceinline__ void stage_pp(const float2* src, float2* dst, int tid, const StageGeom& g,
                                                   const float2* tw) {
     const int total = g.nb * g.ncols;
-    // more butterflies in flight per thread for column tiles (measured)
-    if constexpr (COLFAST) {
-#pragma unroll 4
-        for (int b = tid; b < total; b += kThreads) stage_bfly<R, COLFAST, LOG2C, INV>(src, dst, b, g, tw);
-    } else {
-#pragma unroll 2
-        for (int b = tid; b < total; b += kThreads) stage_bfly<R, COLFAST, LOG2C, INV>(src, dst, b, g, tw);
-    }
+    // butterflies in flight per thread: 4 on column tiles, 2 on rows
+    // (measured), 1 for radices 11 and 13, whose register footprint would
+    // otherwise set the whole kernel's
+    constexpr int U = R > 8 ? 1 : (COLFAST ? 4 : 2);
+#pragma unroll U
+    for (int b = tid; b < total; b += kThreads) stage_bfly<R, COLFAST, LOG2C, INV>(src, dst, b, g, tw);
 }
 
 // Stage s of plan P from src to dst (the radix chosen at run time).

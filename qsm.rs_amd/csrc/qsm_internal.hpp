@@ -21,11 +21,18 @@ namespace qsm {
 constexpr int kMaxVolumes = 64;
 constexpr int kTile = 16;      // columns per y/z-pass tile (16 * 8 B = 128 B rows)
 constexpr int kLog2Tile = 4;
-// Runtime-planned column lengths above this run in tiles of kTileLong
-// columns (two LDS buffers of kTileLong * 1024 elements fit a workgroup).
+// Runtime-planned column lengths above 256 run in tiles of kTileLong columns
+// (two LDS buffers of kTileLong * 1024 elements fit a workgroup); compiled
+// lengths, at most kWideTileMaxLen, in tiles of kTile.
 constexpr int kTileLong = 8;
 constexpr int kWideTileMaxLen = 512;
 constexpr int tile_log2(int tw) { return tw == 16 ? 4 : (tw == 8 ? 3 : -1); }
+// Added to a runtime-plan column kernel's elements per thread (EG) when it
+// runs in tiles of kTileLong columns.
+constexpr int kColLongTile = 256;
+// Elements per thread of the runtime-plan row kernels (tiles of
+// kThreads * kRowEmaxRt elements; the compiled ones take 16).
+constexpr int kRowEmaxRt = 8;
 
 struct LaunchTable {
     int nvol;

@@ -54,15 +54,15 @@ __device__ __forceinline__ void stg2(QSM_GLOBAL float2* p, float2 a) {
 
 // elements per thread of an x (row) pass / a column pass
 __host__ __device__ constexpr int x_emax(int n) {
-    return n > 0 ? (qsmfft::fixed_rows(n) * n + kThreads - 1) / kThreads : 16;
+    return n > 0 ? (qsmfft::fixed_rows(n) * n + kThreads - 1) / kThreads : kRowEmaxRt;
 }
 __host__ __device__ constexpr int xc_emax(int n) {
-    return n > 0 ? (qsmfft::fixed_rows_c(n) * n + kThreads - 1) / kThreads : 16;
+    return n > 0 ? (qsmfft::fixed_rows_c(n) * n + kThreads - 1) / kThreads : kRowEmaxRt;
 }
 __host__ __device__ constexpr int col_emax(int n, int eg) {
     return n > 0 ? (kTile * n + kThreads - 1) / kThreads : eg;
 }
-__host__ __device__ constexpr int x_lds(int n) { return n > 0 ? qsmfft::fixed_rows(n) * n : kThreads * 16; }
+__host__ __device__ constexpr int x_lds(int n) { return n > 0 ? qsmfft::fixed_rows(n) * n : kThreads * kRowEmaxRt; }
 __host__ __device__ constexpr int col_lds(int n, int eg) { return n > 0 ? kTile * n : kThreads * eg; }
 __host__ __device__ constexpr int tw_lds(int n) { return n > 0 ? n : kMaxLen; }
 // column passes of the lengths fft_col2.hpp covers use its padded layout
@@ -596,7 +596,7 @@ __device__ __forceinline__ void fill_tvdiv(float2* lds, const VolDesc& V, int nx
                                            QSM_GLOBAL int8_t* __restrict__ rec,
                                            const QSM_GLOBAL float* __restrict__ w = nullptr,
                                            float wnorm = 1.0f) {
-    constexpr int CH = W ? 1 : 2;
+    constexpr int CH = (W || EMAX <= kRowEmaxRt) ? 1 : 2;
     constexpr int NR = (EMAX + CH - 1) / CH;
     const int total = RB * nx;
     TvDivRound<CH, W> A, B;
@@ -1041,7 +1041,7 @@ __global__ __launch_bounds__(kThreads) void k_r2c_x(const VolDesc* __restrict__ 
                                                     VolStatus* mirror, int ck) {
     constexpr int RBX = N > 0 ? (qsmfft::fixed_rows(N) >> RBS) : 0;
     static_assert(N == 0 || RBX >= 1, "row shift too large for this length");
-    constexpr int EMAX = N > 0 ? (RBX * N + kThreads - 1) / kThreads : 16;
+    constexpr int EMAX = N > 0 ? (RBX * N + kThreads - 1) / kThreads : kRowEmaxRt;
     __shared__ float2 lds[N > 0 ? RBX * N : x_lds(0)];
     __shared__ float2 twl[tw_lds(N)];
     const int bid = xcd_remap(blockIdx.x, gridDim.x);
@@ -1123,7 +1123,7 @@ __global__ __launch_bounds__(kThreads) void k_r2c_x(const VolDesc* __restrict__ 
 
 // ------------------------------------------------------------- C2C y pass --
 // Block = one tile of TW consecutive kx columns at one kz, full ny (TW =
-// kTile; kTileLong for runtime-planned lengths above kWideTileMaxLen).
+// kTile; kTileLong for runtime-planned lengths above 256).
 // sel2 >= 0 transforms a second spectrum with a second set of blocks.
 // SCHED 0: two-stage column FFT (fft_col2.hpp) where the length allows;
 // SCHED 1: the LDS-staged Stockham stages (less LDS per block).
@@ -1758,8 +1758,8 @@ __global__ __launch_bounds__(kThreads) void k_c2r_tv(const VolDesc* __restrict__
                                                      const VolStatus* __restrict__ status) {
     // half the R2C pass's pair-rows (after its row shift RBS), at least 1
     constexpr int RBF = N > 0 ? ((qsmfft::fixed_rows(N) >> RBS) / 2 > 0 ? (qsmfft::fixed_rows(N) >> RBS) / 2 : 1) : 0;
-    constexpr int EMAX = N > 0 ? (RBF * N + kThreads - 1) / kThreads : 16;
-    __shared__ float2 lds[N > 0 ? RBF * N : kThreads * 16];
+    constexpr int EMAX = N > 0 ? (RBF * N + kThreads - 1) / kThreads : kRowEmaxRt;
+    __shared__ float2 lds[N > 0 ? RBF * N : x_lds(0)];
     __shared__ float2 twl[tw_lds(N)];
     __shared__ double sh[kThreads / 64];
     const int bid = xcd_remap(blockIdx.x, gridDim.x);

@@ -131,10 +131,9 @@ void QSM_CAT(launch_fixed_, QSM_PASS_N)(int kind, const LaunchTable& lt, const P
 #else
 void launch_generic(int kind, int emax, const LaunchTable& lt, const PassArgs& a) {
     if (is_row_kind(kind)) launch_rows<0, 16>(kind, lt, a);
-    else if (emax <= 16) launch_cols<0, 16>(kind, lt, a);
-    else if (emax <= 24) launch_cols<0, 24>(kind, lt, a);
-    else if (emax <= 32) launch_cols<0, 32>(kind, lt, a);
-    else launch_cols<0, 32, kTileLong>(kind, lt, a);  // lengths above kWideTileMaxLen
+    else if (emax == kColLongTile + 16) launch_cols<0, 16, kTileLong>(kind, lt, a);  // work length <= 512
+    else if (emax > kColLongTile) launch_cols<0, 32, kTileLong>(kind, lt, a);        // <= 1024
+    else launch_cols<0, 16>(kind, lt, a);                                            // <= 256
 }
 #endif
 

@@ -76,16 +76,23 @@ bool length_ok(size_t n) {
 // convolution length), which sizes the LDS tiles.
 int work_length(int n) { return direct_length(n) ? n : chirp_length(n); }
 
-// EMAX (elements per thread) needed for a column pass of length n in tiles
-// of kTile columns; above kWideTileMaxLen the pass runs in tiles of
-// kTileLong columns (EMAX 32), marked by the value 64.
-int emax_for_col(int n) {
-    const int w = work_length(n);
-    return w <= 256 ? 16 : (w <= 384 ? 24 : (w <= kWideTileMaxLen ? 32 : 64));
+// Columns per tile of a column pass of length n: kTile for the compiled
+// lengths and for runtime plans of work length <= 256; kTileLong above (half
+// the LDS per block of the two-buffer transform, so twice the blocks per
+// CU).  Measured, one volume: 164x205x205 (chirp-z on every axis) 4.29 ->
+// 5.14 solves/s and 232x232x160 4.30 -> 4.69 with kTileLong, 130^3 (work
+// length 130) 27.8 -> 26.7, so kTile stays there.
+int col_tile(int n) {
+    if (has_fixed_kernels(n) || work_length(n) <= 256) return kTile;
+    return kTileLong;
 }
 
-// Columns per tile of a column pass of length n.
-int col_tile(int n) { return work_length(n) <= kWideTileMaxLen ? kTile : kTileLong; }
+// Runtime-plan column kernel of length n: its elements per thread (EG),
+// plus kColLongTile when it runs in tiles of kTileLong columns.
+int emax_for_col(int n) {
+    if (col_tile(n) == kTileLong) return kColLongTile + (work_length(n) <= 512 ? 16 : 32);
+    return 16;
+}
 
 // Elements of the tile-major TV / RTS z-pass operands (f^, q) of a volume:
 // ny * nz rows of whole z-pass tiles (at least the half spectrum).
@@ -216,8 +223,11 @@ struct PassBlocks {
 };
 
 // Pair-rows per block of the x passes: as many as 16 elements per thread
-// hold, at most 32.
-static int default_rows_x(int nx) { return std::max(1, std::min(32, (qsmfft::kThreads * 16) / work_length(nx))); }
+// hold (kRowEmaxRt for the runtime-plan kernels), at most 32.
+static int default_rows_x(int nx) {
+    const int tile = qsmfft::kThreads * (has_fixed_kernels(nx) ? 16 : kRowEmaxRt);
+    return std::max(1, std::min(32, tile / work_length(nx)));
+}
 
 // xshift / cshift 2 (compiled lengths only): the R2C / TV C2R passes take a
 // quarter of their default rows per block, as the kernels launched with
