@@ -237,7 +237,11 @@ PassBlocks fill_geometry(VolDesc& V, const qsm_grid& g, TB& tb, int xshift = 0, 
     V.nx = (int)g.nx; V.ny = (int)g.ny; V.nz = (int)g.nz;
     V.nxh = (V.nx + 1) / 2;  // even nx: DC and Nyquist share column 0
     V.rows_x = default_rows_x(V.nx) >> xshift;
-    V.rows_c = std::max(1, (default_rows_x(V.nx) >> cshift) / 2);
+    // compiled C2R kernels take half the R2C pass's pair-rows (the previous
+    // iterate stays in registers across the transform); the runtime-plan one
+    // its whole tile, whose 8 elements per thread cover it
+    V.rows_c = has_fixed_kernels(V.nx) ? std::max(1, (default_rows_x(V.nx) >> cshift) / 2)
+                                       : default_rows_x(V.nx);
     V.ntx = (V.nxh + kTile - 1) / kTile;
     V.ntx_long = (V.nxh + kTileLong - 1) / kTileLong;
     V.m_nx = qsmfft::magic_for((uint32_t)V.nx);
