@@ -21,12 +21,38 @@ inline bool direct_length(int n) {
     return factorize(n, r, &s);
 }
 
-// Convolution length of the chirp-z transform of length n: the smallest
-// length >= 2n - 1 the stages handle directly (0 if none <= kMaxDirectLen).
+// Relative cost of one runtime-plan stage of radix r per element: every
+// stage is one LDS round trip and one barrier; radices 11 and 13 also keep
+// one butterfly in flight per thread and a long register DFT.
+inline double stage_cost(int r) {
+    switch (r) {
+        case 2: case 4: return 1.0;
+        case 3: case 8: return 1.1;
+        case 5: return 1.2;
+        case 7: return 1.5;
+        default: return 2.2;  // 11, 13
+    }
+}
+
+// Convolution length of the chirp-z transform of length n: of the lengths
+// M >= 2n - 1 the stages handle directly (up to kMaxDirectLen), the one with
+// the least M * sum of its stage costs, so 512 = 8 8 8 is taken over
+// 468 = 4 3 3 13 for n = 232 (0 if there is none).
 inline int chirp_length(int n) {
-    for (int m = 2 * n - 1; m <= kMaxDirectLen; ++m)
-        if (direct_length(m)) return m;
-    return 0;
+    int best = 0;
+    double best_cost = 0.0;
+    for (int m = 2 * n - 1; m <= kMaxDirectLen; ++m) {
+        int r[kMaxStages], st;
+        if (!direct_length(m) || !factorize(m, r, &st)) continue;
+        double c = 0.0;
+        for (int i = 0; i < st; ++i) c += stage_cost(r[i]);
+        c *= m;
+        if (!best || c < best_cost) {
+            best = m;
+            best_cost = c;
+        }
+    }
+    return best;
 }
 
 // Chirp c[j] = exp(-i pi j^2 / n), j < n (j^2 reduced mod 2n, exact), and the
