@@ -16,12 +16,13 @@ CSRC = os.path.join(HERE, "csrc")
 BUILD = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libqsm_gpu.so")
 
-# qsm_pass.hip is compiled once per specialised axis length (and once with
-# length 0 for the runtime-plan fallback); keep in step with
-# QSM_FIXED_LENGTHS in qsm_misc.hip.
+# qsm_pass.hip is compiled once per specialised axis length, and with length
+# 0 in four parts (QSM_GENERIC_PART) for the runtime-plan kernels; keep in
+# step with QSM_FIXED_LENGTHS in qsm_misc.hip.
 FIXED_LENGTHS = [48, 64, 72, 80, 96, 112, 120, 128, 144, 150, 160, 168, 176, 180, 192, 200, 208, 216, 220,
                  224, 240, 256, 288, 320, 384, 416, 448, 480, 512]
-UNITS = ([("qsm_pass.hip", f"qsm_pass_{n}", [f"-DQSM_PASS_N={n}"]) for n in [0] + FIXED_LENGTHS]
+UNITS = ([("qsm_pass.hip", f"qsm_pass_0_{k}", ["-DQSM_PASS_N=0", f"-DQSM_GENERIC_PART={k}"]) for k in range(4)]
+         + [("qsm_pass.hip", f"qsm_pass_{n}", [f"-DQSM_PASS_N={n}"]) for n in FIXED_LENGTHS]
          + [("qsm_misc.hip", "qsm_misc", []), ("qsm_runtime.hip", "qsm_runtime", []),
             ("qsm_nifti.hip", "qsm_nifti", [])])
 

@@ -37,7 +37,8 @@ inline double stage_cost(int r) {
 // Convolution length of the chirp-z transform of length n: of the lengths
 // M >= 2n - 1 the stages handle directly (up to kMaxDirectLen), the one with
 // the least M * sum of its stage costs, so 512 = 8 8 8 is taken over
-// 468 = 4 3 3 13 for n = 232 (0 if there is none).
+// 468 = 4 3 3 13 for n = 232 (0 if there is none).  128, 256, 512 and 1024
+// count half: the runtime-plan kernels run them on compile-time stages.
 inline int chirp_length(int n) {
     int best = 0;
     double best_cost = 0.0;
@@ -47,6 +48,7 @@ inline int chirp_length(int n) {
         double c = 0.0;
         for (int i = 0; i < st; ++i) c += stage_cost(r[i]);
         c *= m;
+        if (m == 128 || m == 256 || m == 512 || m == 1024) c *= 0.5;
         if (!best || c < best_cost) {
             best = m;
             best_cost = c;

@@ -20,6 +20,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <type_traits>
+
 namespace qsmfft {
 
 constexpr int kMaxStages = 10;
@@ -408,26 +410,32 @@ __host__ __device__ inline void bl_out(const float2* src, float2* dst, int tid, 
 }
 
 #ifdef __HIPCC__
+// The runtime-plan stages of P from src to dst and back, one barrier after
+// each; returns the buffer that holds the result.
+template <bool COLFAST, int LOG2C, bool INV>
+__device__ __forceinline__ float2* plan_stages(float2* src, float2* dst, const Plan& P, int ncols, int es, int cs,
+                                               const float2* tw) {
+    for (int s = 0; s < P.nstages; ++s) {
+        plan_stage<COLFAST, LOG2C, INV>(src, dst, threadIdx.x, P, s, ncols, es, cs, tw);
+        __syncthreads();
+        float2* t = src;
+        src = dst;
+        dst = t;
+    }
+    return src;
+}
+
 // Full transform of the ncols sequences in buffer a on the device, buffer b
 // as the other half of each stage (both hold ncols * n, or ncols * M for a
-// chirp-z length, elements).  Caller must have synchronised after filling a;
+// chirp-z length, elements).  inner(src, dst, inv, es, cs) runs a P.n-point
+// transform (inv: std::true_type / std::false_type) and returns the buffer
+// that holds its result.  Caller must have synchronised after filling a;
 // returns after a final barrier with the result in a.
-template <bool COLFAST, int LOG2C, bool INV>
+template <bool COLFAST, int LOG2C, bool INV, class Inner>
 __device__ __forceinline__ void lds_fft_any(float2* a, float2* b, const Plan& P, int ncols, int es, int cs,
-                                            const float2* twl, const float2* twg) {
+                                            const float2* twg, Inner&& inner) {
     const int tid = threadIdx.x;
-    float2* src = a;
-    float2* dst = b;
-    auto stages = [&](int wes, int wcs, bool inv) {
-        for (int s = 0; s < P.nstages; ++s) {
-            if (inv) plan_stage<COLFAST, LOG2C, true>(src, dst, tid, P, s, ncols, wes, wcs, twl);
-            else plan_stage<COLFAST, LOG2C, false>(src, dst, tid, P, s, ncols, wes, wcs, twl);
-            __syncthreads();
-            float2* t = src;
-            src = dst;
-            dst = t;
-        }
-    };
+    float2* src;
     int n = P.n;
     if (P.bl_n) {
         const BlGeom g = bl_geom(P, ncols, es, cs, COLFAST);
@@ -435,17 +443,15 @@ __device__ __forceinline__ void lds_fft_any(float2* a, float2* b, const Plan& P,
         n = g.n;
         bl_in<COLFAST, LOG2C, INV>(a, b, tid, g, chirp);
         __syncthreads();
-        src = b;
-        dst = a;
-        stages(g.wes, g.wcs, false);
-        bl_filter<COLFAST, LOG2C>(src, tid, g, twg + P.bl_filt);
+        float2* r = inner(b, a, std::false_type{}, g.wes, g.wcs);
+        bl_filter<COLFAST, LOG2C>(r, tid, g, twg + P.bl_filt);
         __syncthreads();
-        stages(g.wes, g.wcs, true);
-        bl_out<COLFAST, LOG2C, INV>(src, dst, tid, g, chirp);
+        r = inner(r, r == a ? b : a, std::true_type{}, g.wes, g.wcs);
+        src = r == a ? b : a;
+        bl_out<COLFAST, LOG2C, INV>(r, src, tid, g, chirp);
         __syncthreads();
-        src = dst;
     } else {
-        stages(es, cs, INV);
+        src = inner(a, b, std::integral_constant<bool, INV>{}, es, cs);
     }
     if (src != a) {
         // the caller's region is contiguous: ncols rows of n (es 1, cs n)

@@ -78,6 +78,25 @@ __device__ __forceinline__ float2* pp_buffer() {
     return pp;
 }
 
+// The M-point transform of a runtime plan (P.n == M: a chirp-z convolution
+// or a direct length) on the compile-time stages (fft_fixed.hpp), in place
+// in buf, when the tile holds the sequences those stages are built for:
+// column tiles of 2^LOG2C columns (layout e * 2^LOG2C + c), or row tiles of
+// kThreads * EMAX / M sequences at stride M.  Returns false (nothing done)
+// otherwise.
+template <int EMAX, bool COLFAST, int LOG2C, bool INV, int M>
+__device__ __forceinline__ bool fixed_pow2_stages(float2* buf, const qsmfft::Plan& P, int ncols,
+                                                  const float2* twl) {
+    constexpr int NC = COLFAST ? (1 << LOG2C) : (kThreads * EMAX) / M;
+    if constexpr (NC < 1 || NC * M > kThreads * EMAX) {
+        return false;
+    } else {
+        if (P.n != M || ncols != NC) return false;
+        qsmfft::fixed_fft<M, NC, COLFAST, INV>(buf, twl);
+        return true;
+    }
+}
+
 // twl: the pass's twiddle table in LDS; twg: the twiddle bank in global
 // memory (the chirp-z tables of runtime-planned lengths).
 template <int N, int EMAX, bool COLFAST, int LOG2C, bool INV, int NC = 0>
@@ -88,8 +107,20 @@ __device__ __forceinline__ void run_fft(float2* lds, const qsmfft::Plan& P, int 
     } else {
         // runtime-planned lengths: out-of-place stages between the kernel's
         // tile (kThreads * EMAX elements in every such kernel) and a second
-        // buffer of that size
-        qsmfft::lds_fft_any<COLFAST, LOG2C, INV>(lds, pp_buffer<kThreads * EMAX>(), P, ncols, es, cs, twl, twg);
+        // buffer of that size; a transform of length 128, 256, 512 or 1024 (a
+        // chirp-z convolution, or a direct 1024) runs on the compile-time
+        // stages instead, in place, when the tile holds exactly the sequences
+        // they are built for
+        auto inner = [&](float2* src, float2* dst, auto inv, int wes, int wcs) -> float2* {
+            constexpr bool I = decltype(inv)::value;
+            if (fixed_pow2_stages<EMAX, COLFAST, LOG2C, I, 128>(src, P, ncols, twl) ||
+                fixed_pow2_stages<EMAX, COLFAST, LOG2C, I, 256>(src, P, ncols, twl) ||
+                fixed_pow2_stages<EMAX, COLFAST, LOG2C, I, 512>(src, P, ncols, twl) ||
+                fixed_pow2_stages<EMAX, COLFAST, LOG2C, I, 1024>(src, P, ncols, twl))
+                return src;
+            return qsmfft::plan_stages<COLFAST, LOG2C, I>(src, dst, P, ncols, wes, wcs, twl);
+        };
+        qsmfft::lds_fft_any<COLFAST, LOG2C, INV>(lds, pp_buffer<kThreads * EMAX>(), P, ncols, es, cs, twg, inner);
     }
 }
 

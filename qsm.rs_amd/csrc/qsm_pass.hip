@@ -6,10 +6,13 @@
 #ifndef QSM_PASS_N
 #error "compile with -DQSM_PASS_N=<axis length> (0 for the runtime-plan kernels)"
 #endif
+#if QSM_PASS_N == 0 && !defined(QSM_GENERIC_PART)
+#error "the runtime-plan kernels compile with -DQSM_GENERIC_PART=0..3"
+#endif
 
 namespace qsm {
 
-static bool is_row_kind(int kind) {
+[[maybe_unused]] static bool is_row_kind(int kind) {
     return kind == P_R2C_TVDIV || kind == P_R2C_F64 || kind == P_R2C_U8 || kind == P_R2C_F32 ||
            kind == P_R2C_SPHERE || kind == P_C2R_TV || kind == P_C2R_VS_SELECT || kind == P_C2R_VS_OUT ||
            kind == P_R2C_TVDIV_REC || kind == P_R2C_TVADJ_FIRST || kind == P_R2C_TVADJ ||
@@ -128,12 +131,27 @@ void QSM_CAT(launch_fixed_, QSM_PASS_N)(int kind, const LaunchTable& lt, const P
     if (is_row_kind(kind)) launch_rows<QSM_PASS_N, 0>(kind, lt, a);
     else launch_cols<QSM_PASS_N, 0>(kind, lt, a);
 }
-#else
+#elif QSM_GENERIC_PART == 0
+// The runtime-plan kernels are compiled in four parts (this one with the row
+// kernels and the dispatcher) so that they build in parallel.
+void launch_generic_cols(int kind, const LaunchTable& lt, const PassArgs& a);
+void launch_generic_cols_long16(int kind, const LaunchTable& lt, const PassArgs& a);
+void launch_generic_cols_long32(int kind, const LaunchTable& lt, const PassArgs& a);
 void launch_generic(int kind, int emax, const LaunchTable& lt, const PassArgs& a) {
     if (is_row_kind(kind)) launch_rows<0, 16>(kind, lt, a);
-    else if (emax == kColLongTile + 16) launch_cols<0, 16, kTileLong>(kind, lt, a);  // work length <= 512
-    else if (emax > kColLongTile) launch_cols<0, 32, kTileLong>(kind, lt, a);        // <= 1024
-    else launch_cols<0, 16>(kind, lt, a);                                            // <= 256
+    else if (emax == kColLongTile + 16) launch_generic_cols_long16(kind, lt, a);  // work length <= 512
+    else if (emax > kColLongTile) launch_generic_cols_long32(kind, lt, a);        // <= 1024
+    else launch_generic_cols(kind, lt, a);                                        // <= 256
+}
+#elif QSM_GENERIC_PART == 1
+void launch_generic_cols(int kind, const LaunchTable& lt, const PassArgs& a) { launch_cols<0, 16>(kind, lt, a); }
+#elif QSM_GENERIC_PART == 2
+void launch_generic_cols_long16(int kind, const LaunchTable& lt, const PassArgs& a) {
+    launch_cols<0, 16, kTileLong>(kind, lt, a);
+}
+#else
+void launch_generic_cols_long32(int kind, const LaunchTable& lt, const PassArgs& a) {
+    launch_cols<0, 32, kTileLong>(kind, lt, a);
 }
 #endif
 
