@@ -705,11 +705,12 @@ __device__ __forceinline__ void tvrow_load(TvRow<NX>& R, int r, int k, int nz, i
 }
 
 // fill_tvdiv with the block's RB pair-rows split over its four waves: wave w
-// takes pair-rows [w RB / 4, (w + 1) RB / 4) (at most PMAX), lane l the
-// columns l + 64 m.  Row indices are then the same in every lane (scalar),
-// the in-row neighbours (x at i +- 1, u_x at i - 1) come from lane rotations
-// (the periodic wrap between columns NX - 1 and 0 from lane broadcasts) and
-// the y neighbours from the wave's neighbouring rows in registers; only the
+// takes pair-rows [w RB / 4, (w + 1) RB / 4), lane l the columns l + 64 m.
+// Row indices are then the same in every lane (scalar), the in-row
+// neighbours (x at i +- 1, u_x at i - 1) come from lane rotations (with a
+// partial last column group, the periodic wrap between columns NX - 1 and 0
+// from lane broadcasts) and the y neighbours from the wave's neighbouring
+// rows in registers; only the
 // rows above and below the wave's rows, and y-wrapping rows, are loaded
 // again.  Loads per voxel: 7 plus 3 / (rows per wave), against 13.  The
 // arithmetic per voxel is that of tvdiv_compute.
@@ -720,7 +721,6 @@ __device__ __forceinline__ void fill_tvdiv_rows(float2* lds, const VolDesc& V, i
                                                 QSM_GLOBAL float* __restrict__ uo) {
     constexpr int C = TvRow<NX>::C;
     constexpr int LV = NX - 64 * (C - 1);  // valid lanes of the last column group
-    constexpr int PMAX = (RB + 3) / 4;
     static_assert(kThreads == 256, "four waves per block");
     const int ny = V.ny, nz = V.nz;
     const int nxy = NX * ny, N = nxy * nz;
@@ -891,14 +891,13 @@ __device__ __forceinline__ void fill_tvdiv_rows(float2* lds, const VolDesc& V, i
         j = jn;
         k = kn;
     };
-    // pair-rows of this wave (nrw / 2 <= PMAX, wave-uniform): the row
+    // pair-rows of this wave (nrw / 2, wave-uniform): the row
     // buffers keep fixed roles, so nothing is indexed at run time
 #pragma unroll
     for (int s = 0; s < (nrw >> 1); ++s) {
         step(A, B, 2 * s, std::false_type{});
         step(B, A, 2 * s + 1, std::true_type{});
     }
-    (void)PMAX;
 }
 
 // First step of an NLTV Newton round after the first (src/inversion/

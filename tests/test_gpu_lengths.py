@@ -173,3 +173,60 @@ def test_dipole_long_lengths(qsm, oracle, dims, bdir):
     err = float(np.max(np.abs(out - ref)) / np.max(np.abs(ref)))
     print(dims, f"max rel err {err:.3e}", flush=True)
     assert err < 2e-5, err
+
+
+def test_batch_mixing_runtime_and_compiled_lengths(qsm, oracle):
+    # one TV batch with a chirp-z volume (232: convolution length 512 on the
+    # compile-time stages), a direct 1024-point axis (8-column tiles) and a
+    # compiled volume: every volume equals its single-volume solve bit for
+    # bit (the launch groups do not change the arithmetic), and the chirp-z
+    # one matches the oracle at 10 iterations
+    rng = np.random.default_rng(31)
+    shapes = [((232, 24, 16), (1.0, 0.9, 1.1), (0.1, 0.2, 1.0)),
+              ((8, 1024, 6), (1.0, 1.0, 1.0), (0.0, 0.1, 1.0)),
+              ((64, 48, 40), (0.9, 0.9, 1.0), (0.0, 0.0, 1.0))]
+    p = qsm.TvParams(lambda_=1e-3, rho=0.1, tol=0.0, max_iter=10)
+    keep, vols, singles, outs, cases = [], [], [], [], []
+    for dims, vs, b in shapes:
+        f, m = _case(rng, dims)
+        cases.append((f, m))
+        singles.append(qsm.tv_admm(f, m, qsm.Grid(dims, vs), b, p))
+        tf, tm = qsm.DeviceArray.from_numpy(f), qsm.DeviceArray.from_numpy(m)
+        tc = qsm.DeviceArray(f.size, np.float64)
+        keep.append((tf, tm))
+        outs.append(tc)
+        vols.append((qsm.Grid(dims, vs), b, tf.ptr, tm.ptr, tc.ptr))
+    batch = qsm.TvBatch(vols, p)
+    try:
+        batch.run()
+        for v in range(len(shapes)):
+            assert np.array_equal(outs[v].numpy(), singles[v]), shapes[v][0]
+    finally:
+        batch.close()
+    (dims, vs, b), (f, m) = shapes[0], cases[0]
+    xo = oracle.tv_admm(f, m, dims, vs, b, 1e-3, 0.1, 0.0, 10)
+    dc = nrmse_dc(singles[0], xo, m)
+    print(dims, f"nrmse dc-removed {dc:.3e} raw {float(oracle.nrmse(singles[0], xo, m)):.3e}", flush=True)
+    assert dc < 2e-5, dc
+
+
+@pytest.mark.parametrize("dims", [(68, 40, 24), (24, 164, 20)])
+def test_rts_nltv_chirp_lengths(qsm, oracle, dims):
+    # RTS (stored q operand, src/inversion/rts.rs:71-212) and NLTV
+    # (src/inversion/nltv.rs:63-133) through the chirp-z transforms (68 and
+    # 164 run convolutions of length 256 and 512 on the compile-time stages)
+    rng = np.random.default_rng(dims[1])
+    f, m = _case(rng, dims)
+    vs, b = (1.0, 1.0, 1.0), (0.1, 0.0, 1.0)
+    g = qsm.Grid(dims, vs)
+    x = qsm.rts(f, m, g, b, qsm.RtsParams(tol=0.0, max_iter=10))
+    xo = oracle.rts(f, m, dims, vs, b, 0.15, 1e5, 10.0, 0.0, 10, 4)
+    e = float(oracle.nrmse(x, xo, m))
+    print(dims, f"rts nrmse {e:.3e}", flush=True)
+    assert e < 2e-5, e
+    p = qsm.NltvParams(lambda_=2e-3, mu=1.0, tol=0.0, max_iter=5, newton_iter=3)
+    x = qsm.nltv(f, m, g, b, p)
+    xo = oracle.nltv(f, m, dims, vs, b, 2e-3, 1.0, 0.0, 5, 3)
+    e = float(oracle.nrmse(x, xo, m))
+    print(dims, f"nltv nrmse {e:.3e}", flush=True)
+    assert e < 2e-5, e
