@@ -247,7 +247,7 @@ def main():
     ap.add_argument("--skip-plain", action="store_true",
                     help="time only the batched path (so a profiler sees batched launches only)")
     ap.add_argument("--dims", type=str, default="256,256,160")
-    ap.add_argument("--workload", choices=["tv256", "uneven32", "chain8", "vjp"], default="tv256")
+    ap.add_argument("--workload", choices=["tv256", "uneven32", "chain8", "vjp", "hipfft"], default="tv256")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
                     help="before timing, copy BYTES device-to-device once (a plain copy "
                          "kernel of known size for scaling HBM counters)")
@@ -266,6 +266,8 @@ def main():
         return bench_chain(a, qsm, phantom)
     if a.workload == "vjp":
         return bench_vjp(a, qsm, phantom)
+    if a.workload == "hipfft":
+        return bench_vendor_fft(a, qsm, phantom)
 
     dims = tuple(int(v) for v in a.dims.split(","))
     vs = (0.9, 0.9, 1.0)
@@ -559,6 +561,94 @@ def bench_vjp(a, qsm, phantom):
                                "K=20, loss ||chi_K - chi_true||^2", "iterations": K},
         "forward_ms": round(fwd_ms, 3), "backward_ms": round(ms_per_step - fwd_ms, 3),
         "alg_GBs": round((fwd_bytes + bwd_bytes) / (ms_per_step * 1e-3) / 1e9, 1),
+    }))
+
+
+def bench_vendor_fft(a, qsm, phantom):
+    """Timing comparator: the installed hipFFT's 3-D R2C and C2R (FP32, out of
+    place, nothing fused) on the batch this bench times, against one whole
+    TV-ADMM iteration of this library on the same volumes (both 3-D
+    transforms plus every element-wise step).  hipFFT is only run here, never
+    linked into the library."""
+    import ctypes as C
+    dims = tuple(int(v) for v in a.dims.split(","))
+    nb = max(1, a.batch)
+    vs = (0.9, 0.9, 1.0)
+    params = qsm.TvParams(lambda_=1e-3, rho=0.1, tol=0.0, max_iter=a.iters)
+    grid = qsm.Grid(dims, vs)
+    keep, vols = [], []
+    for v in range(nb):
+        p_ = phantom.make_phantom(dims, voxel_size=vs, bdir=(0.0, 0.0, 1.0), n_ellipsoids=40,
+                                  seed=1001 + 1000 * v)
+        tf, tm = qsm.DeviceArray.from_numpy(p_.local_field), qsm.DeviceArray.from_numpy(p_.mask)
+        tc = qsm.DeviceArray(p_.local_field.size, np.float64)
+        keep.append((tf, tm, tc))
+        vols.append((grid, p_.bdir, tf.ptr, tm.ptr, tc.ptr))
+    batch = qsm.TvBatch(vols, params)
+    batch.run()
+    ms = batch.profile(a.profile_iters)[:len(KERNELS)]
+    batch.close()
+    iter_ms = float(sum(ms))
+
+    lib = None
+    for name in ("libhipfft.so", "/opt/rocm/lib/libhipfft.so"):
+        try:
+            lib = C.CDLL(name)
+            break
+        except OSError:
+            continue
+    if lib is None:
+        print(json.dumps({"metric": "hipFFT comparator", "error": "libhipfft.so not found"}))
+        return
+    HIPFFT_R2C, HIPFFT_C2R = 0x2A, 0x2C
+    nx, ny, nz = dims
+    N, Nh = nx * ny * nz, (nx // 2 + 1) * ny * nz
+    x = qsm.DeviceArray(nb * N, np.float32)
+    X = qsm.DeviceArray(2 * nb * Nh, np.float32)
+    n = (C.c_int * 3)(nz, ny, nx)  # C order: x fastest
+    pf, pi = C.c_void_p(), C.c_void_p()
+    for plan, typ, idist, odist in ((pf, HIPFFT_R2C, N, Nh), (pi, HIPFFT_C2R, Nh, N)):
+        rc = lib.hipfftPlanMany(C.byref(plan), 3, n, None, 1, idist, None, 1, odist, typ, nb)
+        if rc != 0:
+            print(json.dumps({"metric": "hipFFT comparator", "error": f"hipfftPlanMany returned {rc}"}))
+            return
+
+    def r2c():
+        assert lib.hipfftExecR2C(pf, C.c_void_p(x.ptr), C.c_void_p(X.ptr)) == 0
+
+    def c2r():
+        assert lib.hipfftExecC2R(pi, C.c_void_p(X.ptr), C.c_void_p(x.ptr)) == 0
+
+    def pair():
+        r2c()
+        c2r()
+    reps = max(a.steps, 20)
+    out = {}
+    for name, f in (("r2c", r2c), ("c2r", c2r), ("pair", pair)):
+        for _ in range(3):
+            f()
+        qsm.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            f()
+        qsm.synchronize()
+        out[name] = (time.perf_counter() - t0) / reps * 1e3
+    lib.hipfftDestroy(pf)
+    lib.hipfftDestroy(pi)
+    x.free()
+    X.free()
+    print(json.dumps({
+        "metric": f"one TV-ADMM iteration vs hipFFT's bare 3-D R2C + C2R, {nb} x "
+                  f"{dims[0]}x{dims[1]}x{dims[2]} FP32",
+        "value": round(iter_ms, 4), "unit": "ms per iteration of the batch", "n_gpus": 1,
+        "steps": reps, "warmup": 3, "ms_per_step": round(iter_ms, 4), "higher_is_better": False,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+        "config": {"workload": f"{nb} x {dims[0]}x{dims[1]}x{dims[2]} @0.9x0.9x1.0 mm"},
+        "tv_iteration_ms": round(iter_ms, 4),
+        "tv_per_kernel_ms": {k: round(float(v), 5) for k, v in zip(KERNELS, ms)},
+        "hipfft_r2c_ms": round(out["r2c"], 4), "hipfft_c2r_ms": round(out["c2r"], 4),
+        "hipfft_pair_ms": round(out["pair"], 4),
+        "pair_over_iteration": round(out["pair"] / iter_ms, 3),
     }))
 
 
