@@ -1241,12 +1241,33 @@ __global__ __launch_bounds__(kThreads) void k_y_pass(const VolDesc* __restrict__
 }
 
 // ---------------------------------------------------- TV spectral passes --
+// The periodic second difference along one axis, (2 - 2 cos(2 pi k / n)) /
+// vs^2: the Poisson eigenvalue of src/unwrap/laplacian.rs:93-118 (negated),
+// 0 at k = 0 and on every length-1 axis.
+__device__ inline double lap_closed(int k, int n, double vs) {
+#pragma clang fp contract(off)
+    const double two_pi = 6.283185307179586476925286766559;
+    return (2.0 - 2.0 * cos(two_pi * k / n)) / (vs * vs);
+}
+
+// One axis's share of the DFT of the negative Laplacian stencil of
+// src/kernels/laplacian.rs:20-62.  That stencil is written by assignment:
+// the centre holds 2/vs^2 for every axis, the two neighbours -1/vs^2 only
+// where n > 1, and where n = 2 both neighbours are the same voxel, written
+// once.  So the share is the periodic second difference where n > 2, but
+// 2/vs^2 (at every k, DC included) where n = 1 and (2 - cos(pi k))/vs^2,
+// i.e. 1/vs^2 at k = 0 and 3/vs^2 at k = 1, where n = 2.
+__device__ inline double lap_stencil(int k, int n, double vs) {
+#pragma clang fp contract(off)
+    if (n > 2) return lap_closed(k, n, vs);
+    return (n == 1 ? 2.0 : (k == 0 ? 1.0 : 3.0)) / (vs * vs);
+}
+
 // Spectral terms at one index, in FP64 with every operation rounded (no
 // contraction) so threshold decisions match the reference's f64 arithmetic:
 // D = 1/3 - (k.b)^2/|k|^2, 0 where |k|^2 <= 1e-20 (src/kernels/dipole.rs:
-// 24-72), and the DFT of the negative Laplacian stencil of
-// src/kernels/laplacian.rs:20-62 in closed form,
-// L = sum_a (2 - 2 cos(2 pi k_a / n_a)) / vs_a^2 (exactly 0 at DC).
+// 24-72), and L, the DFT of the negative Laplacian stencil (lap_stencil per
+// axis; exactly 0 at DC unless an axis has length 1).
 __device__ inline void spec_terms(const VolDesc& V, int kx, int ky, int kz, double& D, double& L) {
 #pragma clang fp contract(off)
     const double fx = fftfreq_at(kx, V.nx, V.vsx);
@@ -1255,10 +1276,7 @@ __device__ inline void spec_terms(const VolDesc& V, int kx, int ky, int kz, doub
     const double kdb = fx * V.bx + fy * V.by + fz * V.bz;
     const double k2 = fx * fx + fy * fy + fz * fz;
     D = (k2 > 1e-20) ? (1.0 / 3.0) - (kdb * kdb) / k2 : 0.0;
-    const double two_pi = 6.283185307179586476925286766559;
-    L = (2.0 - 2.0 * cos(two_pi * kx / V.nx)) / (V.vsx * V.vsx) +
-        (2.0 - 2.0 * cos(two_pi * ky / V.ny)) / (V.vsy * V.vsy) +
-        (2.0 - 2.0 * cos(two_pi * kz / V.nz)) / (V.vsz * V.vsz);
+    L = lap_stencil(kx, V.nx, V.vsx) + lap_stencil(ky, V.ny, V.vsy) + lap_stencil(kz, V.nz, V.vsz);
 }
 
 // TV: D inv_a and inv_a, inv_a = 1/(D^2 + rho L) or 0 where |.| <= 1e-20
@@ -1302,7 +1320,8 @@ __device__ inline double dipole_at(const VolDesc& V, int kx, int ky, int kz) {
 //   RTS step 1: lsmr_iter rounds of w += (1 - w D) D/(D^2 + 1e-6) from
 //     w = D/(D^2 + 1e-6) on the cone |D| > delta, 0 off it (the per-frequency
 //     refinement of src/inversion/rts.rs:117-150, applied to FFT(f))
-//   Poisson: 1/lambda with lambda = -L the periodic Laplacian's eigenvalue, 0
+//   Poisson: 1/lambda with lambda the periodic Laplacian's eigenvalue
+//     (lap_closed summed over the axes, negated), 0
 //     where |lambda| <= 1e-20 (src/unwrap/laplacian.rs:93-133)
 enum Filter : int { FILT_TKD = 0, FILT_TSVD = 1, FILT_TIK_ID = 2, FILT_TIK_GRAD = 3, FILT_TIK_LAP = 4,
                     FILT_RTS_X0 = 5, FILT_POISSON = 6 };
@@ -1326,7 +1345,8 @@ __device__ inline double filter_mult(const VolDesc& V, int kx, int ky, int kz) {
             return fabs(denom) > 1e-20 ? D / denom : 0.0;
         }
         case FILT_POISSON: {
-            const double lam = -L;
+            const double lam = -(lap_closed(kx, V.nx, V.vsx) + lap_closed(ky, V.ny, V.vsy) +
+                                 lap_closed(kz, V.nz, V.vsz));
             return fabs(lam) > 1e-20 ? 1.0 / lam : 0.0;
         }
         default: {  // FILT_RTS_X0
@@ -1368,9 +1388,7 @@ struct TvQ {
         sxy = (float)(fx * V.bx + fy * V.by);
         sxym = (float)(fxm * V.bx + fym * V.by);
         k2xy = (float)(fx * fx + fy * fy);
-        const double two_pi = 6.283185307179586476925286766559;
-        lxy = (float)((2.0 - 2.0 * cos(two_pi * kx / V.nx)) / (V.vsx * V.vsx) +
-                      (2.0 - 2.0 * cos(two_pi * ky / V.ny)) / (V.vsy * V.vsy));
+        lxy = (float)(lap_stencil(kx, V.nx, V.vsx) + lap_stencil(ky, V.ny, V.vsy));
         dfz = (float)(1.0 / ((double)V.nz * V.vsz));
         bz = (float)V.bz;
         hz2 = (float)(1.0 / (V.vsz * V.vsz));
@@ -1383,9 +1401,12 @@ struct TvQ {
         const float fz = (float)sz * dfz, fzm = (float)szm * dfz;
         const float kdb = sxy + fz * bz, kdbm = sxym + fzm * bz;
         const float k2 = k2xy + fz * fz;
-        // 2 - 2 cos(t) = 2 sin^2(t) / (1 + cos(t)) where cos(t) > 0 (no cancellation)
+        // 2 - 2 cos(t) = 2 sin^2(t) / (1 + cos(t)) where cos(t) > 0 (no
+        // cancellation); lap_stencil's shares where nz <= 2 (twz.x = cos(pi kz)
+        // where nz = 2)
         const float cz = twz.x;
-        const float lz = (cz > 0.0f ? 2.0f * (twz.y * twz.y) / (1.0f + cz) : 2.0f - 2.0f * cz) * hz2;
+        const float lz = (nz > 2 ? (cz > 0.0f ? 2.0f * (twz.y * twz.y) / (1.0f + cz) : 2.0f - 2.0f * cz)
+                                 : (nz == 1 ? 2.0f : 2.0f - cz)) * hz2;
         const float rl = rho * (lxy + lz);
         const float rk = k2 > 1e-20f ? __builtin_amdgcn_rcpf(k2) : 0.0f;  // D = 0 at DC
         const float D = k2 > 1e-20f ? (1.0f / 3.0f) - kdb * kdb * rk : 0.0f;

@@ -52,6 +52,32 @@ def test_closed_forms_match_oracle(qsm, oracle, dims, vs, b):
         assert np.all(got[k][m == 0] == 0.0)
 
 
+@pytest.mark.parametrize("dims", [(64, 1, 8), (20, 2, 1), (1, 16, 12), (2, 24, 2)])
+def test_degenerate_axes_match_oracle(qsm, oracle, dims):
+    # axes of length 1 or 2, where the reference's Laplacian stencil is not
+    # the periodic second difference (its DFT keeps 2/vs^2 on a length-1
+    # axis and 1/vs^2 or 3/vs^2 on a length-2 one): Tikhonov with the
+    # gradient and Laplacian terms, RTS, and the Poisson unwrap, whose
+    # eigenvalues stay the periodic ones
+    vs, b = (0.9, 1.1, 1.2), (0.1, 0.2, 1.0)
+    f, m = _case(5, dims)
+    g = qsm.Grid(dims, vs)
+    for reg, ri in [(qsm.Regularization.Gradient, 1), (qsm.Regularization.Laplacian, 2)]:
+        e = _rel_l2(qsm.tikhonov(f, m, g, b, qsm.TikhonovParams(0.01, reg)),
+                    oracle.tikhonov(f, m, dims, vs, b, 0.01, ri))
+        print(dims, reg, "rel L2", e, flush=True)
+        assert e <= 1e-5
+    x = qsm.rts(f, m, g, b, qsm.RtsParams(tol=0.0, max_iter=6))
+    xo = oracle.rts(f, m, dims, vs, b, 0.15, 1e5, 10.0, 0.0, 6, 4)
+    e = float(oracle.nrmse(x, xo, m))
+    print(dims, "rts nrmse", e, flush=True)
+    assert e <= 2e-5
+    ph = np.mod(f * 400.0 + np.pi, 2 * np.pi) - np.pi
+    e = _rel_l2(qsm.laplacian_unwrap(ph, m, g), oracle.laplacian_unwrap(ph, m, dims, vs))
+    print(dims, "unwrap rel L2", e, flush=True)
+    assert e <= 1e-5
+
+
 @pytest.mark.parametrize("dims,vs,b", SHAPES[:3])
 def test_rts_matches_oracle(qsm, oracle, dims, vs, b):
     f, m = _case(4, dims)

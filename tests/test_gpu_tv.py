@@ -311,3 +311,21 @@ def test_tv_uneven_batch_fixed_lengths(qsm, oracle, phantom):
     e = nrmse_dc(singles[1], xo, ph.mask)
     print("uneven batch volume 1 nrmse(dc removed)", e, "raw", oracle.nrmse(singles[1], xo, ph.mask))
     assert e <= 1e-3
+
+
+@pytest.mark.parametrize("dims", [(256, 1, 8), (128, 6, 1), (160, 1, 1), (144, 2, 3), (240, 3, 2)])
+def test_tv_degenerate_axes(qsm, oracle, dims):
+    # compiled x lengths (the wave-row divergence fill: 256 and 128 in whole
+    # 64-column groups, 160 / 144 / 240 with a partial last group) with y or
+    # z of length 1 or 2, where every y or z neighbour is the voxel itself
+    # or wraps at once; 5 iterations against the oracle
+    rng = np.random.default_rng(sum(dims))
+    f, m = _case(rng, dims)
+    vs, b = (1.0, 1.0, 1.0), (0.1, 0.2, 1.0)
+    p = qsm.TvParams(lambda_=1e-3, rho=0.1, tol=0.0, max_iter=5)
+    x = qsm.tv_admm(f, m, qsm.Grid(dims, vs), b, p)
+    xo = oracle.tv_admm(f, m, dims, vs, b, 1e-3, 0.1, 0.0, 5)
+    e = float(oracle.nrmse(x, xo, m))
+    print(dims, f"nrmse {e:.3e}", flush=True)
+    assert e < 2e-5, e
+    assert np.all(x[m == 0] == 0.0)

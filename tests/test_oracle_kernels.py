@@ -57,6 +57,27 @@ def test_laplacian_spectrum_closed_form(oracle):
     assert np.max(np.abs(spec - closed.ravel(order="F"))) < 1e-12
 
 
+def test_laplacian_degenerate_axes(oracle):
+    # src/kernels/laplacian.rs:20-62 writes the centre for every axis but the
+    # neighbours only where n > 1, by assignment: on a length-2 axis both
+    # neighbours are voxel 1, written once; on a length-1 axis there is none.
+    # So the stencil no longer sums to zero, and its DFT per axis is 2h
+    # (n = 1) and 2h - h cos(pi k) (n = 2), h = 1/vs^2, against the periodic
+    # 2h - 2h cos(2 pi k / n) of longer axes.
+    dims, vs = (1, 2, 5), (0.9, 1.1, 1.3)
+    h = [1.0 / v ** 2 for v in vs]
+    l = oracle.laplacian_kernel(dims, vs, True).reshape(dims, order="F")
+    assert abs(l[0, 0, 0] - 2 * sum(h)) < 1e-12
+    assert abs(l[0, 1, 0] + h[1]) < 1e-12
+    assert abs(l[0, 0, 1] + h[2]) < 1e-12 and abs(l[0, 0, 4] + h[2]) < 1e-12
+    assert np.count_nonzero(l) == 4
+    assert abs(l.sum() - (2 * h[0] + h[1])) < 1e-12
+    spec = oracle.fft3d(l.ravel(order="F"), dims).real
+    _, j, k = np.meshgrid(*[np.arange(n) for n in dims], indexing="ij")
+    want = 2 * h[0] + (2 - np.cos(np.pi * j)) * h[1] + (2 - 2 * np.cos(2 * np.pi * k / 5)) * h[2]
+    assert np.max(np.abs(spec - want.ravel(order="F"))) < 1e-12
+
+
 def test_smv_reference_tests_and_counts(oracle):
     s, _ = oracle.smv_kernel((16, 16, 16), (1, 1, 1), 3.0)
     assert abs(s.sum() - 1.0) < 1e-10
