@@ -656,71 +656,135 @@ __device__ __forceinline__ float lane_prev(float v) {
 }
 
 // Row lengths and pair-rows per block (RB) the wave-row form takes: at
-// least one pair-row per wave, at most 4 column groups of 64 per lane.
-// Measured: faster than the element-wise fill at 144 ... 256; at 384 (6
-// groups, 222 VGPRs) slower, and with 7 or 8 groups its two row buffers take
-// all 256 VGPRs (one wave per SIMD, where the element-wise fill keeps two).
+// least one pair-row per wave, at most 4 column groups per lane.  Measured
+// (one column per lane): faster than the element-wise fill at 144 ... 256;
+// at 384 (6 groups of 64, 222 VGPRs) slower, and with 7 or 8 groups its two
+// row buffers take all 256 VGPRs (one wave per SIMD, where the element-wise
+// fill keeps two).
 __host__ __device__ constexpr bool tvdiv_rows_form(int n, int rb) {
     return n > 0 && n <= 256 && rb >= 4;
 }
 
-// One row of the wave-row form below: the values at the row's own voxels,
-// columns lane + 64 m (m < C = ceil(NX / 64); in a partial last group the
-// lanes past NX hold column NX - 1 and are never stored).
-template <int NX>
+// Columns per lane of the wave-row form (VW): 2 where whole groups of 128
+// columns cover the row, so that every load and store moves 8 bytes per
+// lane (measured at 256: 277.9 -> 274.8 us per 4-volume launch, 151 -> 118
+// VGPRs); otherwise 1 (a partial last group of 128 leaves more lanes idle,
+// and at 208 / 240 the pair form takes 168 VGPRs against 132).
+__host__ __device__ constexpr int tvdiv_rows_vw(int n) {
+    return n % 128 == 0 ? 2 : 1;
+}
+
+template <int VW>
+using lanev = std::conditional_t<VW == 1, float, float2>;
+__device__ __forceinline__ float vfirst(float v) { return v; }
+__device__ __forceinline__ float vfirst(float2 v) { return v.x; }
+__device__ __forceinline__ float vlast(float v) { return v; }
+__device__ __forceinline__ float vlast(float2 v) { return v.y; }
+template <int VW>
+__device__ __forceinline__ lanev<VW> vzero() {
+    if constexpr (VW == 1) return 0.0f;
+    else return make_float2(0.0f, 0.0f);
+}
+template <int VW>
+__device__ __forceinline__ lanev<VW> ldv(const QSM_GLOBAL float* p) {
+    if constexpr (VW == 1) return *p;
+    else return ldg2((const QSM_GLOBAL float2*)p);
+}
+template <int VW>
+__device__ __forceinline__ void stv(QSM_GLOBAL float* p, lanev<VW> v) {
+    if constexpr (VW == 1) *p = v;
+    else stg2((QSM_GLOBAL float2*)p, v);
+}
+
+// One row of the wave-row form below: lane l holds columns VW l ... VW l +
+// VW - 1 of each group of G = 64 VW columns, m < C = ceil(NX / G) (in a
+// partial last group the lanes past its last column hold that column's
+// lane-set and are never stored).
+template <int NX, int VW>
 struct TvRow {
-    static constexpr int C = (NX + 63) / 64;
-    float xc[C], ux[C], uy[C], uz[C], xzp[C], xzm[C], uzm[C];
+    static constexpr int G = 64 * VW;
+    static constexpr int C = (NX + G - 1) / G;
+    lanev<VW> xc[C], ux[C], uy[C], uz[C], xzp[C], xzm[C], uzm[C];
 };
 
-template <int NX>
+// first column of the lane's set in group m
+template <int NX, int VW>
 __device__ __forceinline__ int tv_col(int lane, int m) {
-    return (NX % 64 != 0 && m == (NX + 63) / 64 - 1) ? min(lane + 64 * m, NX - 1) : lane + 64 * m;
+    constexpr int G = 64 * VW, C = (NX + G - 1) / G;
+    return (NX % G != 0 && m == C - 1) ? min(VW * lane + G * m, NX - VW) : VW * lane + G * m;
 }
 
 // Row r of the volume (r = j + k ny, k the plane): its own x and u, x on the
 // planes k +- 1 and u_z on plane k - 1 (periodic).
-template <int NX>
-__device__ __forceinline__ void tvrow_load(TvRow<NX>& R, int r, int k, int nz, int nxy, int N, int lane,
+template <int NX, int VW>
+__device__ __forceinline__ void tvrow_load(TvRow<NX, VW>& R, int r, int k, int nz, int nxy, int N, int lane,
                                            const QSM_GLOBAL float* __restrict__ x,
                                            const QSM_GLOBAL float* __restrict__ u) {
-    constexpr int C = TvRow<NX>::C;
+    constexpr int C = TvRow<NX, VW>::C;
     const int base = r * NX;
     const int dzp = (k + 1 < nz) ? nxy : nxy - N;
     const int dzm = (k > 0) ? -nxy : N - nxy;
     const QSM_GLOBAL float* xr = x + base;
     const QSM_GLOBAL float* ur = u + base;
 #pragma unroll
-    for (int m = 0; m < C; ++m) R.xc[m] = xr[tv_col<NX>(lane, m)];
+    for (int m = 0; m < C; ++m) R.xc[m] = ldv<VW>(xr + tv_col<NX, VW>(lane, m));
 #pragma unroll
     for (int m = 0; m < C; ++m) {
-        const int i = tv_col<NX>(lane, m);
-        R.ux[m] = ur[i];
-        R.uy[m] = ur[N + i];
-        R.uz[m] = ur[2 * N + i];
-        R.xzp[m] = xr[dzp + i];
-        R.xzm[m] = xr[dzm + i];
-        R.uzm[m] = ur[2 * N + dzm + i];
+        const int i = tv_col<NX, VW>(lane, m);
+        R.ux[m] = ldv<VW>(ur + i);
+        R.uy[m] = ldv<VW>(ur + N + i);
+        R.uz[m] = ldv<VW>(ur + 2 * N + i);
+        R.xzp[m] = ldv<VW>(xr + dzp + i);
+        R.xzm[m] = ldv<VW>(xr + dzm + i);
+        R.uzm[m] = ldv<VW>(ur + 2 * N + dzm + i);
     }
 }
 
+// The z/u update and the divergence at one voxel (the arithmetic of
+// tvdiv_compute): x and its six neighbours, u and the components of u at
+// the three lower neighbours; writes u' = v - shrink(v), returns div(2z - v).
+__device__ __forceinline__ float tv_voxel(float xc, float xp, float xm, float xyp, float xym, float xzp,
+                                          float xzm, float ux, float uy, float uz, float uxm, float uym,
+                                          float uzm, float t, float hx, float hy, float hz, float& ox,
+                                          float& oy, float& oz) {
+    const float vx = (xp - xc) * hx + ux;
+    const float vy = (xyp - xc) * hy + uy;
+    const float vz = (xzp - xc) * hz + uz;
+    const float vxm = (xc - xm) * hx + uxm;
+    const float vym = (xc - xym) * hy + uym;
+    const float vzm = (xc - xzm) * hz + uzm;
+    const float zx = shrinkf(vx, t), zy = shrinkf(vy, t), zz = shrinkf(vz, t);
+    ox = vx - zx;
+    oy = vy - zy;
+    oz = vz - zz;
+    const float gx = 2.0f * zx - vx, gy = 2.0f * zy - vy, gz = 2.0f * zz - vz;
+    const float gxm = 2.0f * shrinkf(vxm, t) - vxm;
+    const float gym = 2.0f * shrinkf(vym, t) - vym;
+    const float gzm = 2.0f * shrinkf(vzm, t) - vzm;
+    return (gx - gxm) * hx + (gy - gym) * hy + (gz - gzm) * hz;
+}
+
 // fill_tvdiv with the block's RB pair-rows split over its four waves: wave w
-// takes pair-rows [w RB / 4, (w + 1) RB / 4), lane l the columns l + 64 m.
-// Row indices are then the same in every lane (scalar), the in-row
-// neighbours (x at i +- 1, u_x at i - 1) come from lane rotations (with a
+// takes pair-rows [w RB / 4, (w + 1) RB / 4), lane l the columns VW l ...
+// VW l + VW - 1 (+ 64 VW m).  Row indices are then the same in every lane
+// (scalar), the in-row neighbours come from the lane's own columns and, at
+// their ends, from lane rotations (x and u_x left of the lane's first
+// column from lane l - 1, x right of its last from lane l + 1; with a
 // partial last column group, the periodic wrap between columns NX - 1 and 0
 // from lane broadcasts) and the y neighbours from the wave's neighbouring
-// rows in registers; only the
-// rows above and below the wave's rows, and y-wrapping rows, are loaded
-// again.  Loads per voxel: 7 plus 3 / (rows per wave), against 13.  The
-// arithmetic per voxel is that of tvdiv_compute.
+// rows in registers; only the rows above and below the wave's rows, and
+// y-wrapping rows, are loaded again.  Loads per voxel: 7 plus 3 / (rows per
+// wave), against 13.
 template <int NX, int RB>
 __device__ __forceinline__ void fill_tvdiv_rows(float2* lds, const VolDesc& V, int row0, int nrows,
                                                 const QSM_GLOBAL float* __restrict__ x,
                                                 const QSM_GLOBAL float* __restrict__ u,
                                                 QSM_GLOBAL float* __restrict__ uo) {
-    constexpr int C = TvRow<NX>::C;
-    constexpr int LV = NX - 64 * (C - 1);  // valid lanes of the last column group
+    constexpr int VW = tvdiv_rows_vw(NX);
+    using vt = lanev<VW>;
+    constexpr int G = TvRow<NX, VW>::G;
+    constexpr int C = TvRow<NX, VW>::C;
+    constexpr int LV = (NX - G * (C - 1)) / VW;  // valid lanes of the last column group
     static_assert(kThreads == 256, "four waves per block");
     const int ny = V.ny, nz = V.nz;
     const int nxy = NX * ny, N = nxy * nz;
@@ -735,27 +799,27 @@ __device__ __forceinline__ void fill_tvdiv_rows(float2* lds, const VolDesc& V, i
     // plane and in-plane index of the wave's first row, then stepped
     int k = r0 / ny;
     int j = r0 - k * ny;
-    TvRow<NX> A, B;
-    float hym_x[C], hym_uy[C], hyp_x[C];  // rows above / below the wave's rows
+    TvRow<NX, VW> A, B;
+    vt hym_x[C], hym_uy[C], hyp_x[C];  // rows above / below the wave's rows
     if (r0 < nrows) {
-        tvrow_load<NX>(A, r0, k, nz, nxy, N, lane, x, u);
+        tvrow_load<NX, VW>(A, r0, k, nz, nxy, N, lane, x, u);
         // the row before the wave's first row, periodic in y
         const int ra = j > 0 ? r0 - 1 : r0 + ny - 1;
         const QSM_GLOBAL float* xr = x + ra * NX;
         const QSM_GLOBAL float* ur = u + N + ra * NX;
 #pragma unroll
         for (int m = 0; m < C; ++m) {
-            hym_x[m] = xr[tv_col<NX>(lane, m)];
-            hym_uy[m] = ur[tv_col<NX>(lane, m)];
+            hym_x[m] = ldv<VW>(xr + tv_col<NX, VW>(lane, m));
+            hym_uy[m] = ldv<VW>(ur + tv_col<NX, VW>(lane, m));
         }
     } else {
 #pragma unroll
-        for (int m = 0; m < C; ++m) hym_x[m] = hym_uy[m] = 0.0f;
+        for (int m = 0; m < C; ++m) hym_x[m] = hym_uy[m] = vzero<VW>();
     }
-    float pxc[C], puy[C], dev[C];
+    vt pxc[C], puy[C], dev[C];
     // row q of the wave from cur, with the next row's loads issued into nxt;
     // ODD: the second row of a pair-row (the pair goes to LDS)
-    auto step = [&](TvRow<NX>& cur, TvRow<NX>& nxt, const int q, auto odd) {
+    auto step = [&](TvRow<NX, VW>& cur, TvRow<NX, VW>& nxt, const int q, auto odd) {
         constexpr bool ODD = decltype(odd)::value;
         const int r = r0 + q;
         const bool ok = r < nrows;
@@ -765,7 +829,7 @@ __device__ __forceinline__ void fill_tvdiv_rows(float2* lds, const VolDesc& V, i
             jn = 0;
             ++kn;
         }
-        if (q + 1 < nrw && r + 1 < nrows) tvrow_load<NX>(nxt, r + 1, kn, nz, nxy, N, lane, x, u);
+        if (q + 1 < nrw && r + 1 < nrows) tvrow_load<NX, VW>(nxt, r + 1, kn, nz, nxy, N, lane, x, u);
         if (q + 2 == nrw && ok) {
             // the row after the wave's last row rl = r + 1 (in-plane index
             // jn), periodic in y; issued one row ahead
@@ -774,16 +838,16 @@ __device__ __forceinline__ void fill_tvdiv_rows(float2* lds, const VolDesc& V, i
             const int rb = (jn + 1 < ny) ? rl + 1 : rl - ny + 1;
             const QSM_GLOBAL float* xr = x + (okl ? rb : 0) * NX;
 #pragma unroll
-            for (int m = 0; m < C; ++m) hyp_x[m] = okl ? xr[tv_col<NX>(lane, m)] : 0.0f;
+            for (int m = 0; m < C; ++m) hyp_x[m] = okl ? ldv<VW>(xr + tv_col<NX, VW>(lane, m)) : vzero<VW>();
         }
-        float xym[C], uym[C], xyp[C];
+        vt xym[C], uym[C], xyp[C];
         if (!ok) {
             // rows past the volume (the last block): d = 0, nothing stored
 #pragma unroll
             for (int m = 0; m < C; ++m) {
-                xym[m] = uym[m] = xyp[m] = 0.0f;
-                cur.xc[m] = cur.ux[m] = cur.uy[m] = cur.uz[m] = 0.0f;
-                cur.xzp[m] = cur.xzm[m] = cur.uzm[m] = 0.0f;
+                xym[m] = uym[m] = xyp[m] = vzero<VW>();
+                cur.xc[m] = cur.ux[m] = cur.uy[m] = cur.uz[m] = vzero<VW>();
+                cur.xzp[m] = cur.xzm[m] = cur.uzm[m] = vzero<VW>();
             }
         } else {
             // y - 1
@@ -797,8 +861,8 @@ __device__ __forceinline__ void fill_tvdiv_rows(float2* lds, const VolDesc& V, i
                 const int ra = r + ny - 1;
 #pragma unroll
                 for (int m = 0; m < C; ++m) {
-                    xym[m] = x[ra * NX + tv_col<NX>(lane, m)];
-                    uym[m] = u[N + ra * NX + tv_col<NX>(lane, m)];
+                    xym[m] = ldv<VW>(x + ra * NX + tv_col<NX, VW>(lane, m));
+                    uym[m] = ldv<VW>(u + N + ra * NX + tv_col<NX, VW>(lane, m));
                 }
             } else {
 #pragma unroll
@@ -814,70 +878,99 @@ __device__ __forceinline__ void fill_tvdiv_rows(float2* lds, const VolDesc& V, i
             } else if (j + 1 == ny) {
                 const int rb = r - ny + 1;
 #pragma unroll
-                for (int m = 0; m < C; ++m) xyp[m] = x[rb * NX + tv_col<NX>(lane, m)];
+                for (int m = 0; m < C; ++m) xyp[m] = ldv<VW>(x + rb * NX + tv_col<NX, VW>(lane, m));
             } else {
 #pragma unroll
                 for (int m = 0; m < C; ++m) xyp[m] = nxt.xc[m];
             }
         }
-        // in-row neighbours: lane l + 1 (lane 63: lane 0 of the next column
-        // group) and lane l - 1 (lane 0: lane 63 of the previous group);
-        // columns NX - 1 and 0 are neighbours (periodic), through lane
-        // broadcasts when the last group is partial
+        // in-row neighbours at the ends of the lane's columns: x right of
+        // its last column from lane l + 1 (lane 63: lane 0 of the next
+        // column group), x and u_x left of its first column from lane l - 1
+        // (lane 0: lane 63 of the previous group); columns NX - 1 and 0 are
+        // neighbours (periodic), through lane broadcasts when the last group
+        // is partial
         float rn[C], rp[C], rpu[C];
 #pragma unroll
         for (int m = 0; m < C; ++m) {
-            rn[m] = lane_next(cur.xc[m]);
-            rp[m] = lane_prev(cur.xc[m]);
-            rpu[m] = lane_prev(cur.ux[m]);
+            rn[m] = lane_next(vfirst(cur.xc[m]));
+            rp[m] = lane_prev(vlast(cur.xc[m]));
+            rpu[m] = lane_prev(vlast(cur.ux[m]));
         }
         // (whole groups: the rotations already wrap column NX - 1 to 0)
         float x0 = 0.0f, xl = 0.0f, uxl = 0.0f;
-        if constexpr (NX % 64 != 0) {
-            x0 = __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, cur.xc[0]), 0));
-            xl = __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, cur.xc[C - 1]), LV - 1));
-            uxl = __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, cur.ux[C - 1]), LV - 1));
+        if constexpr (NX % G != 0) {
+            x0 = __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, vfirst(cur.xc[0])), 0));
+            xl = __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, vlast(cur.xc[C - 1])), LV - 1));
+            uxl = __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, vlast(cur.ux[C - 1])), LV - 1));
         }
-        float d[C];
+        vt d[C];
 #pragma unroll
         for (int m = 0; m < C; ++m) {
             float xp = lane == 63 ? rn[(m + 1) % C] : rn[m];
             float xm = lane == 0 ? rp[(m + C - 1) % C] : rp[m];
             float uxm = lane == 0 ? rpu[(m + C - 1) % C] : rpu[m];
-            if constexpr (NX % 64 != 0) {
+            if constexpr (NX % G != 0) {
                 if (m == C - 1 && lane == LV - 1) xp = x0;  // column NX - 1
                 if (m == 0 && lane == 0) {                  // column 0
                     xm = xl;
                     uxm = uxl;
                 }
             }
-            const float xc = cur.xc[m];
-            const float vx = (xp - xc) * hx + cur.ux[m];
-            const float vy = (xyp[m] - xc) * hy + cur.uy[m];
-            const float vz = (cur.xzp[m] - xc) * hz + cur.uz[m];
-            const float vxm = (xc - xm) * hx + uxm;
-            const float vym = (xc - xym[m]) * hy + uym[m];
-            const float vzm = (xc - cur.xzm[m]) * hz + cur.uzm[m];
-            const float zx = shrinkf(vx, t), zy = shrinkf(vy, t), zz = shrinkf(vz, t);
-            const bool col_ok = NX % 64 == 0 || m < C - 1 || lane < LV;
-            if (ok && col_ok) {
-                QSM_GLOBAL float* o = uo + r * NX + lane + 64 * m;
-                o[0] = vx - zx;
-                o[N] = vy - zy;
-                o[2 * N] = vz - zz;
+            const bool col_ok = NX % G == 0 || m < C - 1 || lane < LV;
+            QSM_GLOBAL float* o = uo + r * NX + VW * lane + G * m;
+            if constexpr (VW == 1) {
+                // (tv_voxel written out: the same arithmetic, with the stores
+                // between the two halves, in fewer registers)
+                const float xc = cur.xc[m];
+                const float vx = (xp - xc) * hx + cur.ux[m];
+                const float vy = (xyp[m] - xc) * hy + cur.uy[m];
+                const float vz = (cur.xzp[m] - xc) * hz + cur.uz[m];
+                const float vxm = (xc - xm) * hx + uxm;
+                const float vym = (xc - xym[m]) * hy + uym[m];
+                const float vzm = (xc - cur.xzm[m]) * hz + cur.uzm[m];
+                const float zx = shrinkf(vx, t), zy = shrinkf(vy, t), zz = shrinkf(vz, t);
+                if (ok && col_ok) {
+                    o[0] = vx - zx;
+                    o[N] = vy - zy;
+                    o[2 * N] = vz - zz;
+                }
+                const float gx = 2.0f * zx - vx, gy = 2.0f * zy - vy, gz = 2.0f * zz - vz;
+                const float gxm = 2.0f * shrinkf(vxm, t) - vxm;
+                const float gym = 2.0f * shrinkf(vym, t) - vym;
+                const float gzm = 2.0f * shrinkf(vzm, t) - vzm;
+                d[m] = ok ? (gx - gxm) * hx + (gy - gym) * hy + (gz - gzm) * hz : 0.0f;
+            } else {
+                const float2 xc = cur.xc[m], ux = cur.ux[m], uy = cur.uy[m], uz = cur.uz[m];
+                float2 ox, oy, oz;
+                const float d0 = tv_voxel(xc.x, xc.y, xm, xyp[m].x, xym[m].x, cur.xzp[m].x, cur.xzm[m].x, ux.x,
+                                          uy.x, uz.x, uxm, uym[m].x, cur.uzm[m].x, t, hx, hy, hz, ox.x, oy.x,
+                                          oz.x);
+                const float d1 = tv_voxel(xc.y, xp, xc.x, xyp[m].y, xym[m].y, cur.xzp[m].y, cur.xzm[m].y, ux.y,
+                                          uy.y, uz.y, ux.x, uym[m].y, cur.uzm[m].y, t, hx, hy, hz, ox.y, oy.y,
+                                          oz.y);
+                if (ok && col_ok) {
+                    stv<2>(o, ox);
+                    stv<2>(o + N, oy);
+                    stv<2>(o + 2 * N, oz);
+                }
+                d[m] = ok ? make_float2(d0, d1) : vzero<2>();
             }
-            const float gx = 2.0f * zx - vx, gy = 2.0f * zy - vy, gz = 2.0f * zz - vz;
-            const float gxm = 2.0f * shrinkf(vxm, t) - vxm;
-            const float gym = 2.0f * shrinkf(vym, t) - vym;
-            const float gzm = 2.0f * shrinkf(vzm, t) - vzm;
-            d[m] = ok ? (gx - gxm) * hx + (gy - gym) * hy + (gz - gzm) * hz : 0.0f;
         }
         if constexpr (ODD) {
             const int p = p0 + (q >> 1);
 #pragma unroll
             for (int m = 0; m < C; ++m) {
-                const bool col_ok = NX % 64 == 0 || m < C - 1 || lane < LV;
-                if (col_ok) lds[p * NX + lane + 64 * m] = make_float2(dev[m], d[m]);
+                const bool col_ok = NX % G == 0 || m < C - 1 || lane < LV;
+                if (col_ok) {
+                    float2* l2 = lds + p * NX + VW * lane + G * m;
+                    if constexpr (VW == 1) {
+                        l2[0] = make_float2(dev[m], d[m]);
+                    } else {
+                        l2[0] = make_float2(dev[m].x, d[m].x);
+                        l2[1] = make_float2(dev[m].y, d[m].y);
+                    }
+                }
             }
         } else {
 #pragma unroll
@@ -1056,6 +1149,19 @@ __device__ __forceinline__ void fill_tvadj(float2* lds, const VolDesc& V, int nx
 }
 
 // ------------------------------------------------------------- R2C x pass --
+// Row-block index of dispatch slot lb when every plane holds a whole number
+// bpp = ny / R of R-row blocks: slots run plane-fastest (lb = yb nz + k), so
+// blocks dispatched together (and sharing an XCD) hold the same rows of
+// neighbouring planes, and the divergence's loads of x at k +- 1 and u_z at
+// k - 1 find the rows their neighbours just read in L2.  Otherwise slots are
+// blocks in row order.  A bijection on [0, bpp nz) either way.
+__device__ __forceinline__ int plane_fast_block(int lb, int ny, int nz, int R) {
+    const int bpp = ny / R;
+    if (bpp * R != ny) return lb;
+    const int yb = lb / nz;
+    return (lb - yb * nz) * bpp + yb;
+}
+
 // Block = RB pairs of real rows.  Pair p packs rows (2p, 2p+1) as
 // z = a + i b; after a complex FFT of length nx the half spectra are
 //   A[k] = (Z[k] + conj Z[-k]) / 2,   B[k] = (Z[k] - conj Z[-k]) / (2i).
@@ -1092,7 +1198,10 @@ __global__ __launch_bounds__(kThreads) void k_r2c_x(const VolDesc* __restrict__ 
     const int nxh = (nx + 1) / 2;
     const int RB = N > 0 ? RBX : V.rows_x;
     const int nrows = V.ny * V.nz;
-    const int row0 = lb * RB * 2;
+    const int rb = (SRC == SRC_TVDIV || SRC == SRC_TVDIV_REC || SRC == SRC_TVDIV_W || SRC == SRC_NLTV_BND)
+                       ? plane_fast_block(lb, V.ny, V.nz, 2 * RB)
+                       : lb;
+    const int row0 = rb * RB * 2;
     TwStage<N> tws;
     tws.fetch(tw, V.px);
     if constexpr (SRC == SRC_TVDIV && tvdiv_rows_form(N, RBX)) {
