@@ -57,7 +57,7 @@ def test_degenerate_axes_match_oracle(qsm, oracle, dims):
     # axes of length 1 or 2, where the reference's Laplacian stencil is not
     # the periodic second difference (its DFT keeps 2/vs^2 on a length-1
     # axis and 1/vs^2 or 3/vs^2 on a length-2 one): Tikhonov with the
-    # gradient and Laplacian terms, RTS, and the Poisson unwrap, whose
+    # gradient and Laplacian terms, RTS, NLTV, and the Poisson unwrap, whose
     # eigenvalues stay the periodic ones
     vs, b = (0.9, 1.1, 1.2), (0.1, 0.2, 1.0)
     f, m = _case(5, dims)
@@ -76,6 +76,10 @@ def test_degenerate_axes_match_oracle(qsm, oracle, dims):
     e = _rel_l2(qsm.laplacian_unwrap(ph, m, g), oracle.laplacian_unwrap(ph, m, dims, vs))
     print(dims, "unwrap rel L2", e, flush=True)
     assert e <= 1e-5
+    p = qsm.NltvParams(lambda_=2e-3, mu=1.0, tol=0.0, max_iter=4, newton_iter=2)
+    e = float(oracle.nrmse(qsm.nltv(f, m, g, b, p), oracle.nltv(f, m, dims, vs, b, 2e-3, 1.0, 0.0, 4, 2), m))
+    print(dims, "nltv nrmse", e, flush=True)
+    assert e <= 2e-5
 
 
 @pytest.mark.parametrize("dims,vs,b", SHAPES[:3])
