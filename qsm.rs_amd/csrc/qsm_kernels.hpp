@@ -667,35 +667,61 @@ __host__ __device__ constexpr bool tvdiv_rows_form(int n, int rb) {
     return n > 0 && n <= 256 && rb >= 4;
 }
 
-// Columns per lane of the wave-row form (VW): 2 where whole groups of 128
-// columns cover the row, so that every load and store moves 8 bytes per
-// lane (measured at 256: 277.9 -> 274.8 us per 4-volume launch, 151 -> 118
-// VGPRs); otherwise 1 (a partial last group of 128 leaves more lanes idle,
-// and at 208 / 240 the pair form takes 168 VGPRs against 132).
+// Columns per lane of the wave-row form (VW): 4 or 2 where whole groups of
+// 256 or 128 columns cover the row, so that every load and store moves 16
+// or 8 bytes per lane (measured at 256: 277.9 -> 274.8 us per 4-volume
+// launch with 2, 268.2 with 4; 151 -> 118 -> 104 VGPRs); otherwise 1 (at
+// 208 / 240 the 2-column form takes 168 VGPRs against 132).
 __host__ __device__ constexpr int tvdiv_rows_vw(int n) {
-    return n % 128 == 0 ? 2 : 1;
+    return n % 256 == 0 ? 4 : (n % 128 == 0 ? 2 : 1);
 }
 
 template <int VW>
-using lanev = std::conditional_t<VW == 1, float, float2>;
-__device__ __forceinline__ float vfirst(float v) { return v; }
-__device__ __forceinline__ float vfirst(float2 v) { return v.x; }
-__device__ __forceinline__ float vlast(float v) { return v; }
-__device__ __forceinline__ float vlast(float2 v) { return v.y; }
+using lanev = std::conditional_t<VW == 1, float, std::conditional_t<VW == 2, float2, float4>>;
+typedef float qsm_f4v __attribute__((ext_vector_type(4)));
+// element e (a compile-time constant after unrolling) of a lane's columns
+template <int VW>
+__device__ __forceinline__ float vel(const lanev<VW>& v, int e) {
+    if constexpr (VW == 1) return v;
+    else if constexpr (VW == 2) return e == 0 ? v.x : v.y;
+    else return e == 0 ? v.x : (e == 1 ? v.y : (e == 2 ? v.z : v.w));
+}
+template <int VW>
+__device__ __forceinline__ lanev<VW> vpack(const float (&a)[VW]) {
+    if constexpr (VW == 1) return a[0];
+    else if constexpr (VW == 2) return make_float2(a[0], a[1]);
+    else return make_float4(a[0], a[1], a[2], a[3]);
+}
+template <int VW>
+__device__ __forceinline__ float vfirst(const lanev<VW>& v) { return vel<VW>(v, 0); }
+template <int VW>
+__device__ __forceinline__ float vlast(const lanev<VW>& v) { return vel<VW>(v, VW - 1); }
 template <int VW>
 __device__ __forceinline__ lanev<VW> vzero() {
-    if constexpr (VW == 1) return 0.0f;
-    else return make_float2(0.0f, 0.0f);
+    const float z[VW] = {};
+    return vpack<VW>(z);
 }
 template <int VW>
 __device__ __forceinline__ lanev<VW> ldv(const QSM_GLOBAL float* p) {
     if constexpr (VW == 1) return *p;
-    else return ldg2((const QSM_GLOBAL float2*)p);
+    else if constexpr (VW == 2) return ldg2((const QSM_GLOBAL float2*)p);
+    else {
+        const qsm_f4v v = *(const QSM_GLOBAL qsm_f4v*)p;
+        return make_float4(v.x, v.y, v.z, v.w);
+    }
 }
 template <int VW>
 __device__ __forceinline__ void stv(QSM_GLOBAL float* p, lanev<VW> v) {
     if constexpr (VW == 1) *p = v;
-    else stg2((QSM_GLOBAL float2*)p, v);
+    else if constexpr (VW == 2) stg2((QSM_GLOBAL float2*)p, v);
+    else {
+        qsm_f4v w;
+        w.x = v.x;
+        w.y = v.y;
+        w.z = v.z;
+        w.w = v.w;
+        *(QSM_GLOBAL qsm_f4v*)p = w;
+    }
 }
 
 // One row of the wave-row form below: lane l holds columns VW l ... VW l +
@@ -895,16 +921,16 @@ __device__ __forceinline__ void fill_tvdiv_rows(float2* lds, const VolDesc& V, i
         float rn[C], rp[C], rpu[C];
 #pragma unroll
         for (int m = 0; m < C; ++m) {
-            rn[m] = lane_next(vfirst(cur.xc[m]));
-            rp[m] = lane_prev(vlast(cur.xc[m]));
-            rpu[m] = lane_prev(vlast(cur.ux[m]));
+            rn[m] = lane_next(vfirst<VW>(cur.xc[m]));
+            rp[m] = lane_prev(vlast<VW>(cur.xc[m]));
+            rpu[m] = lane_prev(vlast<VW>(cur.ux[m]));
         }
         // (whole groups: the rotations already wrap column NX - 1 to 0)
         float x0 = 0.0f, xl = 0.0f, uxl = 0.0f;
         if constexpr (NX % G != 0) {
-            x0 = __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, vfirst(cur.xc[0])), 0));
-            xl = __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, vlast(cur.xc[C - 1])), LV - 1));
-            uxl = __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, vlast(cur.ux[C - 1])), LV - 1));
+            x0 = __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, vfirst<VW>(cur.xc[0])), 0));
+            xl = __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, vlast<VW>(cur.xc[C - 1])), LV - 1));
+            uxl = __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, vlast<VW>(cur.ux[C - 1])), LV - 1));
         }
         vt d[C];
 #pragma unroll
@@ -943,20 +969,25 @@ __device__ __forceinline__ void fill_tvdiv_rows(float2* lds, const VolDesc& V, i
                 const float gzm = 2.0f * shrinkf(vzm, t) - vzm;
                 d[m] = ok ? (gx - gxm) * hx + (gy - gym) * hy + (gz - gzm) * hz : 0.0f;
             } else {
-                const float2 xc = cur.xc[m], ux = cur.ux[m], uy = cur.uy[m], uz = cur.uz[m];
-                float2 ox, oy, oz;
-                const float d0 = tv_voxel(xc.x, xc.y, xm, xyp[m].x, xym[m].x, cur.xzp[m].x, cur.xzm[m].x, ux.x,
-                                          uy.x, uz.x, uxm, uym[m].x, cur.uzm[m].x, t, hx, hy, hz, ox.x, oy.x,
-                                          oz.x);
-                const float d1 = tv_voxel(xc.y, xp, xc.x, xyp[m].y, xym[m].y, cur.xzp[m].y, cur.xzm[m].y, ux.y,
-                                          uy.y, uz.y, ux.x, uym[m].y, cur.uzm[m].y, t, hx, hy, hz, ox.y, oy.y,
-                                          oz.y);
-                if (ok && col_ok) {
-                    stv<2>(o, ox);
-                    stv<2>(o + N, oy);
-                    stv<2>(o + 2 * N, oz);
+                // the lane's columns e = 0 ... VW - 1: in-row neighbours
+                // from its own registers except at the two ends
+                float ox[VW], oy[VW], oz[VW], dd[VW];
+#pragma unroll
+                for (int e = 0; e < VW; ++e) {
+                    const float xpe = e + 1 < VW ? vel<VW>(cur.xc[m], e + 1) : xp;
+                    const float xme = e > 0 ? vel<VW>(cur.xc[m], e - 1) : xm;
+                    const float uxme = e > 0 ? vel<VW>(cur.ux[m], e - 1) : uxm;
+                    dd[e] = tv_voxel(vel<VW>(cur.xc[m], e), xpe, xme, vel<VW>(xyp[m], e), vel<VW>(xym[m], e),
+                                     vel<VW>(cur.xzp[m], e), vel<VW>(cur.xzm[m], e), vel<VW>(cur.ux[m], e),
+                                     vel<VW>(cur.uy[m], e), vel<VW>(cur.uz[m], e), uxme, vel<VW>(uym[m], e),
+                                     vel<VW>(cur.uzm[m], e), t, hx, hy, hz, ox[e], oy[e], oz[e]);
                 }
-                d[m] = ok ? make_float2(d0, d1) : vzero<2>();
+                if (ok && col_ok) {
+                    stv<VW>(o, vpack<VW>(ox));
+                    stv<VW>(o + N, vpack<VW>(oy));
+                    stv<VW>(o + 2 * N, vpack<VW>(oz));
+                }
+                d[m] = ok ? vpack<VW>(dd) : vzero<VW>();
             }
         }
         if constexpr (ODD) {
@@ -966,12 +997,8 @@ __device__ __forceinline__ void fill_tvdiv_rows(float2* lds, const VolDesc& V, i
                 const bool col_ok = NX % G == 0 || m < C - 1 || lane < LV;
                 if (col_ok) {
                     float2* l2 = lds + p * NX + VW * lane + G * m;
-                    if constexpr (VW == 1) {
-                        l2[0] = make_float2(dev[m], d[m]);
-                    } else {
-                        l2[0] = make_float2(dev[m].x, d[m].x);
-                        l2[1] = make_float2(dev[m].y, d[m].y);
-                    }
+#pragma unroll
+                    for (int e = 0; e < VW; ++e) l2[e] = make_float2(vel<VW>(dev[m], e), vel<VW>(d[m], e));
                 }
             }
         } else {
