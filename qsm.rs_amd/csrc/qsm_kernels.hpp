@@ -667,13 +667,16 @@ __host__ __device__ constexpr bool tvdiv_rows_form(int n, int rb) {
     return n > 0 && n <= 256 && rb >= 4;
 }
 
-// Columns per lane of the wave-row form (VW): 4 or 2 where whole groups of
-// 256 or 128 columns cover the row, so that every load and store moves 16
-// or 8 bytes per lane (measured at 256: 277.9 -> 274.8 us per 4-volume
-// launch with 2, 268.2 with 4; 151 -> 118 -> 104 VGPRs); otherwise 1 (at
-// 208 / 240 the 2-column form takes 168 VGPRs against 132).
+// Columns per lane of the wave-row form (VW): 4 or 2 where groups of 256 or
+// 128 columns pad the row no further than groups of 64 do (no more idle
+// lanes), so that every load and store moves 16 or 8 bytes per lane
+// (measured at 256: 277.9 -> 274.8 us per 4-volume launch with 2, 268.2
+// with 4; 151 -> 118 -> 104 VGPRs); otherwise 1.
+__host__ __device__ constexpr int tv_pad(int n, int g) { return (n + g - 1) / g * g; }
 __host__ __device__ constexpr int tvdiv_rows_vw(int n) {
-    return n % 256 == 0 ? 4 : (n % 128 == 0 ? 2 : 1);
+    return (n % 4 == 0 && tv_pad(n, 256) == tv_pad(n, 64))
+               ? 4
+               : ((n % 2 == 0 && tv_pad(n, 128) == tv_pad(n, 64)) ? 2 : 1);
 }
 
 template <int VW>
