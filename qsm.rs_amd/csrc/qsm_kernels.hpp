@@ -2008,6 +2008,68 @@ __global__ __launch_bounds__(kThreads) void k_c2r_tv(const VolDesc* __restrict__
     }
     const QSM_GLOBAL float* __restrict__ xo = glb(V.x[parity]);
     QSM_GLOBAL float* __restrict__ xn = glb(V.x[parity ^ 1]);
+    if constexpr (N > 0 && N % 4 == 0) {
+        // four consecutive columns per thread: 16-byte loads of the previous
+        // iterate and stores of the new one
+        constexpr int E4 = (RBF * N / 4 + kThreads - 1) / kThreads;
+        float4 xa[E4], xb[E4];  // previous iterate, in flight during the inverse FFT
+#pragma unroll
+        for (int t = 0; t < E4; ++t) {
+            const int e = 4 * (threadIdx.x + t * kThreads);
+            const int p = e / N;
+            const int i = e - p * N;
+            // rows past the volume read its last row: always in bounds, never used
+            const int ra = min(row0 + 2 * p, nrows - 1), rb = min(row0 + 2 * p + 1, nrows - 1);
+            xa[t] = ldv<4>(xo + i + (size_t)ra * N);
+            xb[t] = ldv<4>(xo + i + (size_t)rb * N);
+        }
+        tws.commit(twl);
+        __syncthreads();
+        run_fft<N, EMAX, false, 0, true, RBF>(lds, V.px, RB, 1, nx, twl, tw);
+        double sd = 0.0, sx = 0.0;
+#pragma unroll
+        for (int t = 0; t < E4; ++t) {
+            const int e = 4 * (threadIdx.x + t * kThreads);
+            if (e < total) {
+                const int p = e / N;
+                const int i = e - p * N;
+                const int ra = row0 + 2 * p, rb = ra + 1;
+                float za[4], zb[4];
+#pragma unroll
+                for (int c = 0; c < 4; ++c) {
+                    const float2 z = lds[e + c];
+                    za[c] = z.x;
+                    zb[c] = z.y;
+                }
+                if (ra < nrows) {
+                    stv<4>(xn + i + (size_t)ra * N, vpack<4>(za));
+#pragma unroll
+                    for (int c = 0; c < 4; ++c) {
+                        const double d = (double)za[c] - (double)vel<4>(xa[t], c);
+                        sd += d * d;
+                        sx += (double)za[c] * (double)za[c];
+                    }
+                }
+                if (rb < nrows) {
+                    stv<4>(xn + i + (size_t)rb * N, vpack<4>(zb));
+#pragma unroll
+                    for (int c = 0; c < 4; ++c) {
+                        const double d = (double)zb[c] - (double)vel<4>(xb[t], c);
+                        sd += d * d;
+                        sx += (double)zb[c] * (double)zb[c];
+                    }
+                }
+            }
+        }
+        const double tsd = block_sum(sd, sh);
+        const double tsx = block_sum(sx, sh);
+        if (threadIdx.x == 0) {
+            QSM_GLOBAL double* part = glb(V.partial);
+            part[2 * lb] = tsd;
+            part[2 * lb + 1] = tsx;
+        }
+        return;
+    }
     float2 xp[EMAX];  // previous iterate, in flight during the inverse FFT
 #pragma unroll
     for (int t = 0; t < EMAX; ++t) {
