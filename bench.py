@@ -235,8 +235,8 @@ def cpu_baseline(ph, params, threads, x_gpu=None, cpu_info=None):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=5)
-    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--batch", type=int, default=4,
                     help="volumes per launch on the batched path (value); 1 = plain only")
     ap.add_argument("--iters", type=int, default=200)
