@@ -729,8 +729,8 @@ __device__ __forceinline__ void stv(QSM_GLOBAL float* p, lanev<VW> v) {
 
 // One row of the wave-row form below: lane l holds columns VW l ... VW l +
 // VW - 1 of each group of G = 64 VW columns, m < C = ceil(NX / G) (in a
-// partial last group the lanes past its last column hold that column's
-// lane-set and are never stored).
+// partial last group the lanes past the row's end hold its last VW columns
+// again and are never stored).
 template <int NX, int VW>
 struct TvRow {
     static constexpr int G = 64 * VW;
