@@ -1269,6 +1269,42 @@ __global__ __launch_bounds__(kThreads) void k_r2c_x(const VolDesc* __restrict__ 
     run_fft<N, EMAX, false, 0, false, RBX>(lds, V.px, RB, 1, nx, twl, tw);
     QSM_GLOBAL float2* __restrict__ out = glb(V.spec[out_sel]);
     const int total = RB * nxh;
+    if constexpr (N > 0 && N % 4 == 0) {
+        // two consecutive half-spectrum entries per thread and row: 16-byte
+        // stores (nxh is even, so every pair is 16-byte aligned)
+        constexpr int NH2 = N / 4, TOT2 = RBX * NH2;
+#pragma unroll
+        for (int t = 0; t < (TOT2 + kThreads - 1) / kThreads; ++t) {
+            const int e = threadIdx.x + t * kThreads;
+            if (e < TOT2) {
+                const int p = e / NH2;
+                const int k0 = 2 * (e - p * NH2);
+                const int ra = row0 + 2 * p, rb = ra + 1;
+                float a4[4], b4[4];
+#pragma unroll
+                for (int c = 0; c < 2; ++c) {
+                    const int k = k0 + c;
+                    const float2 zk = lds[p * N + k];
+                    const float2 zm = lds[p * N + (k == 0 ? 0 : N - k)];
+                    float2 A = make_float2(0.5f * (zk.x + zm.x), 0.5f * (zk.y - zm.y));
+                    float2 B = make_float2(0.5f * (zk.y + zm.y), -0.5f * (zk.x - zm.x));
+                    if (k == 0) {
+                        // (DC, Nyquist) of both rows: real parts of Z[0] and Z[N/2]
+                        const float2 zn = lds[p * N + N / 2];
+                        A = make_float2(zk.x, zn.x);
+                        B = make_float2(zk.y, zn.y);
+                    }
+                    a4[2 * c] = A.x;
+                    a4[2 * c + 1] = A.y;
+                    b4[2 * c] = B.x;
+                    b4[2 * c + 1] = B.y;
+                }
+                if (ra < nrows) stv<4>((QSM_GLOBAL float*)(out + k0 + (size_t)ra * nxh), vpack<4>(a4));
+                if (rb < nrows) stv<4>((QSM_GLOBAL float*)(out + k0 + (size_t)rb * nxh), vpack<4>(b4));
+            }
+        }
+        return;
+    }
 #pragma unroll
     for (int t = 0; t < EMAX; ++t) {
         const int e = threadIdx.x + t * kThreads;
