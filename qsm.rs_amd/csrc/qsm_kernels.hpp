@@ -1970,6 +1970,65 @@ __device__ __forceinline__ void c2r_load(float2* lds, int nx, int nxh, int RB,
     }
 }
 
+// c2r_load for compiled lengths divisible by 4: a thread takes two
+// consecutive half-spectrum entries k0, k0 + 1 of a row pair (16-byte loads,
+// every entry read once) and writes the full-spectrum elements they give:
+// k and N - k for 0 < k < N / 2, and for k = 0 the DC element and the
+// Nyquist element N / 2 (the imaginary parts of the packed column 0).
+template <int N, int RB>
+__device__ __forceinline__ void c2r_load_pairs(float2* lds, const QSM_GLOBAL float2* __restrict__ sa,
+                                               const QSM_GLOBAL float2* __restrict__ sb, int row0, int step,
+                                               int nrows) {
+    static_assert(N % 4 == 0, "pairs of half-spectrum entries");
+    constexpr int NH = N / 2, NH2 = N / 4, TOT2 = RB * NH2;
+    constexpr int T = (TOT2 + kThreads - 1) / kThreads;
+    float4 a[T], b[T];
+#pragma unroll
+    for (int t = 0; t < T; ++t) {
+        const int e = threadIdx.x + t * kThreads;
+        const int ec = e < TOT2 ? e : 0;  // idle threads load a valid pair (unused)
+        const int p = ec / NH2;
+        const int k0 = 2 * (ec - p * NH2);
+        // rows past the volume read its last row: in bounds, never stored
+        const int ra = min(row0 + step * p, nrows - 1);
+        const int rb = min((step == 2) ? row0 + step * p + 1 : row0 + step * p, nrows - 1);
+        a[t] = ldv<4>((const QSM_GLOBAL float*)(sa + k0 + (size_t)ra * NH));
+        b[t] = ldv<4>((const QSM_GLOBAL float*)(sb + k0 + (size_t)rb * NH));
+    }
+#pragma unroll
+    for (int t = 0; t < T; ++t) {
+        const int e = threadIdx.x + t * kThreads;
+        if (e < TOT2) {
+            const int p = e / NH2;
+            const int k0 = 2 * (e - p * NH2);
+            float2* row = lds + p * N;
+#pragma unroll
+            for (int c = 0; c < 2; ++c) {
+                const int k = k0 + c;
+                const float2 A = c == 0 ? make_float2(a[t].x, a[t].y) : make_float2(a[t].z, a[t].w);
+                const float2 B = c == 0 ? make_float2(b[t].x, b[t].y) : make_float2(b[t].z, b[t].w);
+                if (k == 0) {
+                    row[0] = make_float2(A.x, B.x);
+                    row[NH] = make_float2(A.y, B.y);
+                } else {
+                    row[k] = make_float2(A.x - B.y, A.y + B.x);
+                    row[N - k] = make_float2(A.x + B.y, B.x - A.y);
+                }
+            }
+        }
+    }
+}
+
+// c2r_load_pairs where it applies, else c2r_load.
+template <int N, int RBC, int EMAX>
+__device__ __forceinline__ void c2r_load_rows(float2* lds, int nx, int nxh, int RB,
+                                              const QSM_GLOBAL float2* __restrict__ sa,
+                                              const QSM_GLOBAL float2* __restrict__ sb, int row0, int step,
+                                              int nrows) {
+    if constexpr (N > 0 && N % 4 == 0) c2r_load_pairs<N, RBC>(lds, sa, sb, row0, step, nrows);
+    else c2r_load<EMAX>(lds, nx, nxh, RB, sa, sb, row0, step, nrows);
+}
+
 // TV: x_it = Re IFFT(x^) (src/inversion/admm.rs:111-114) and the two sums of
 // relative_change (src/utils/mod.rs:69-77) accumulated in FP64 per block;
 // the next pass's first block (or k_tv_reduce after the last iteration)
@@ -2004,7 +2063,7 @@ __global__ __launch_bounds__(kThreads) void k_c2r_tv(const VolDesc* __restrict__
     const QSM_GLOBAL float2* s = glb(V.spec[SPEC_WORK]);
     TwStage<N> tws;
     tws.fetch(tw, V.px);
-    c2r_load<EMAX>(lds, nx, nxh, RB, s, s, row0, 2, nrows);
+    c2r_load_rows<N, RBF, EMAX>(lds, nx, nxh, RB, s, s, row0, 2, nrows);
     if constexpr (MODE != C2R_TV) {
         tws.commit(twl);
         __syncthreads();
@@ -2326,7 +2385,8 @@ __global__ __launch_bounds__(kThreads) void k_c2r_vs_select(const VolDesc* __res
     const int row0 = lb * RB;
     TwStage<N> tws;
     tws.fetch(tw, V.px);
-    c2r_load<EMAX>(lds, nx, nxh, RB, glb(V.spec[SPEC_WORK]), glb(V.spec[SPEC_C]), row0, 1, nrows);
+    c2r_load_rows<N, (N > 0 ? qsmfft::fixed_rows(N) : 0), EMAX>(lds, nx, nxh, RB, glb(V.spec[SPEC_WORK]),
+                                                            glb(V.spec[SPEC_C]), row0, 1, nrows);
     tws.commit(twl);
     QSM_GLOBAL uint8_t* fin = glb(V.vs_final);
     QSM_GLOBAL float* loc = glb(V.vs_local);
@@ -2374,7 +2434,7 @@ __global__ __launch_bounds__(kThreads) void k_c2r_vs_out(const VolDesc* __restri
     const QSM_GLOBAL float2* s = glb(V.spec[SPEC_WORK]);
     TwStage<N> tws;
     tws.fetch(tw, V.px);
-    c2r_load<EMAX>(lds, nx, nxh, RB, s, s, row0, 2, nrows);
+    c2r_load_rows<N, (N > 0 ? qsmfft::fixed_rows(N) : 0), EMAX>(lds, nx, nxh, RB, s, s, row0, 2, nrows);
     tws.commit(twl);
     __syncthreads();
     run_fft<N, EMAX, false, 0, true>(lds, V.px, RB, 1, nx, twl, tw);
