@@ -1558,10 +1558,12 @@ enum ZOp : int {
 // with twz = exp(-2 pi i kz / nz) from the z pass's twiddle table.
 struct TvQ {
     float sxy, sxym, k2xy, lxy, dfz, bz, hz2, rho, scale;
+    bool sym;  // the mirror of (kx, ky) is (-kx, -ky): no Nyquist index in it
     __device__ __forceinline__ void init(const VolDesc& V, int kx, int ky) {
         const double fx = fftfreq_at(kx, V.nx, V.vsx), fy = fftfreq_at(ky, V.ny, V.vsy);
         const double fxm = fftfreq_at(mirror_idx(kx, V.nx), V.nx, V.vsx);
         const double fym = fftfreq_at(mirror_idx(ky, V.ny), V.ny, V.vsy);
+        sym = fxm == -fx && fym == -fy;
         sxy = (float)(fx * V.bx + fy * V.by);
         sxym = (float)(fxm * V.bx + fym * V.by);
         k2xy = (float)(fx * fx + fy * fy);
@@ -1582,14 +1584,19 @@ struct TvQ {
         // cancellation); lap_stencil's shares where nz <= 2 (twz.x = cos(pi kz)
         // where nz = 2)
         const float cz = twz.x;
-        const float lz = (nz > 2 ? (cz > 0.0f ? 2.0f * (twz.y * twz.y) / (1.0f + cz) : 2.0f - 2.0f * cz)
+        const float lz = (nz > 2 ? (cz > 0.0f ? 2.0f * (twz.y * twz.y) * __builtin_amdgcn_rcpf(1.0f + cz)
+                                              : 2.0f - 2.0f * cz)
                                  : (nz == 1 ? 2.0f : 2.0f - cz)) * hz2;
         const float rl = rho * (lxy + lz);
         const float rk = k2 > 1e-20f ? __builtin_amdgcn_rcpf(k2) : 0.0f;  // D = 0 at DC
         const float D = k2 > 1e-20f ? (1.0f / 3.0f) - kdb * kdb * rk : 0.0f;
-        const float Dm = k2 > 1e-20f ? (1.0f / 3.0f) - kdbm * kdbm * rk : 0.0f;
-        const float a = D * D + rl, am = Dm * Dm + rl;
+        const float a = D * D + rl;
         const float ia = a > 1e-20f ? __builtin_amdgcn_rcpf(a) : 0.0f;
+        // where the mirror is -k (sym, kz not the Nyquist plane) sxym = -sxy
+        // and fzm = -fz exactly, so kdbm = -kdb and its term is ia bit for bit
+        if (sym && 2 * kz != nz) return scale * (ia + ia);
+        const float Dm = k2 > 1e-20f ? (1.0f / 3.0f) - kdbm * kdbm * rk : 0.0f;
+        const float am = Dm * Dm + rl;
         const float iam = am > 1e-20f ? __builtin_amdgcn_rcpf(am) : 0.0f;
         return scale * (ia + iam);
     }
