@@ -1557,7 +1557,7 @@ enum ZOp : int {
 // init: the x and y terms of one column (FP64, once per thread); at: one kz,
 // with twz = exp(-2 pi i kz / nz) from the z pass's twiddle table.
 struct TvQ {
-    float sxy, sxym, k2xy, lxy, dfz, bz, hz2, rho, scale;
+    float sxy, sxym, k2xy, lxy, rlxy, dfz, bz, hz2, rho, scale;
     bool sym;  // the mirror of (kx, ky) is (-kx, -ky): no Nyquist index in it
     __device__ __forceinline__ void init(const VolDesc& V, int kx, int ky) {
         const double fx = fftfreq_at(kx, V.nx, V.vsx), fy = fftfreq_at(ky, V.ny, V.vsy);
@@ -1568,6 +1568,7 @@ struct TvQ {
         sxym = (float)(fxm * V.bx + fym * V.by);
         k2xy = (float)(fx * fx + fy * fy);
         lxy = (float)(lap_stencil(kx, V.nx, V.vsx) + lap_stencil(ky, V.ny, V.vsy));
+        rlxy = (float)(V.rho * (lap_stencil(kx, V.nx, V.vsx) + lap_stencil(ky, V.ny, V.vsy)));
         dfz = (float)(1.0 / ((double)V.nz * V.vsz));
         bz = (float)V.bz;
         hz2 = (float)(1.0 / (V.vsz * V.vsz));
@@ -1595,6 +1596,37 @@ struct TvQ {
         // where the mirror is -k (sym, kz not the Nyquist plane) sxym = -sxy
         // and fzm = -fz exactly, so kdbm = -kdb and its term is ia bit for bit
         if (sym && 2 * kz != nz) return scale * (ia + ia);
+        const float Dm = k2 > 1e-20f ? (1.0f / 3.0f) - kdbm * kdbm * rk : 0.0f;
+        const float am = Dm * Dm + rl;
+        const float iam = am > 1e-20f ? __builtin_amdgcn_rcpf(am) : 0.0f;
+        return scale * (ia + iam);
+    }
+    // The terms of at() that depend on kz only, shared by every column of a
+    // z tile: (fz bz, fz^2, rho lz / vs_z^2) with fz the plane's frequency
+    // and lz its share of the Laplacian (cos and sin of 2 pi kz / nz here).
+    __device__ static float4 plane(const VolDesc& V, int kz, int nz) {
+        const float dfz = (float)(1.0 / ((double)nz * V.vsz));
+        const int sz = kz < (nz + 1) / 2 ? kz : kz - nz;
+        const float fz = (float)sz * dfz;
+        float sn, cs;
+        sincospif(2.0f * (float)kz / (float)nz, &sn, &cs);
+        const float lz = (nz > 2 ? (cs > 0.0f ? 2.0f * (sn * sn) * __builtin_amdgcn_rcpf(1.0f + cs)
+                                              : 2.0f - 2.0f * cs)
+                                 : (nz == 1 ? 2.0f : 2.0f - cs)) *
+                         (float)(1.0 / (V.vsz * V.vsz));
+        return make_float4(fz * (float)V.bz, fz * fz, (float)V.rho * lz, 0.0f);
+    }
+    // at() from the plane's terms P = plane(V, kz, nz)
+    __device__ __forceinline__ float at_plane(int kz, int nz, float4 P) const {
+        const float kdb = sxy + P.x;
+        const float k2 = k2xy + P.y;
+        const float rl = rlxy + P.z;
+        const float rk = k2 > 1e-20f ? __builtin_amdgcn_rcpf(k2) : 0.0f;  // D = 0 at DC
+        const float D = k2 > 1e-20f ? (1.0f / 3.0f) - kdb * kdb * rk : 0.0f;
+        const float a = D * D + rl;
+        const float ia = a > 1e-20f ? __builtin_amdgcn_rcpf(a) : 0.0f;
+        if (sym && 2 * kz != nz) return scale * (ia + ia);
+        const float kdbm = sxym + (2 * kz == nz ? P.x : -P.x);  // the mirror's kz is -kz but on Nyquist
         const float Dm = k2 > 1e-20f ? (1.0f / 3.0f) - kdbm * kdbm * rk : 0.0f;
         const float am = Dm * Dm + rl;
         const float iam = am > 1e-20f ? __builtin_amdgcn_rcpf(am) : 0.0f;
@@ -1702,6 +1734,11 @@ __device__ __forceinline__ void z_tv_col2(const VolDesc& V, float2* lds, float2*
         }
     }
     tws.commit(twl);
+    // the multiplier's per-plane terms, once per block for every column
+    __shared__ float4 zqt[Q_MEM ? 1 : N];
+    if constexpr (!Q_MEM) {
+        for (int kz = threadIdx.x; kz < N; kz += kThreads) zqt[kz] = TvQ::plane(V, kz, N);
+    }
     C::template a_fwd<false>(threadIdx.x, v, lds);
     __syncthreads();
     if constexpr (!Q_MEM) {
@@ -1711,11 +1748,11 @@ __device__ __forceinline__ void z_tv_col2(const VolDesc& V, float2* lds, float2*
         pk_freq(V, min(kx0 + c, nxh - 1), ky, fx, fy);
         tq.init(V, fx, fy);
         if (act) {
-            float2 tz[16];  // all LDS reads issued before the first use
+            float4 pz[16];  // all LDS reads issued before the first use
 #pragma unroll
-            for (int r = 0; r < 16; ++r) tz[r] = twl[j + R0 * r];
+            for (int r = 0; r < 16; ++r) pz[r] = zqt[j + R0 * r];
 #pragma unroll
-            for (int r = 0; r < 16; ++r) qr[r] = tq.at(j + R0 * r, N, tz[r]);
+            for (int r = 0; r < 16; ++r) qr[r] = tq.at_plane(j + R0 * r, N, pz[r]);
         }
     }
     C::template b_fwd<false>(threadIdx.x, v, lds, twl);
@@ -1743,7 +1780,7 @@ __device__ __forceinline__ void z_tv_col2(const VolDesc& V, float2* lds, float2*
                 const int kz = j + R0 * r;
                 float qB;
                 if constexpr (Q_MEM) qB = rts_q(V, V.nx / 2, ky, kz);
-                else qB = tqB.at(kz, N, twl[kz]);
+                else qB = tqB.at_plane(kz, N, zqt[kz]);
                 qs = pk_apply(v[r], lds[mirror_idx(kz, N)], qr[r], qB);
             }
             if constexpr (HAS_F)
