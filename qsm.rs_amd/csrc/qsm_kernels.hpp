@@ -1559,16 +1559,29 @@ enum ZOp : int {
 struct TvQ {
     float sxy, sxym, k2xy, lxy, rlxy, dfz, bz, hz2, rho, scale;
     bool sym;  // the mirror of (kx, ky) is (-kx, -ky): no Nyquist index in it
+    // In FP32 (the multiplier is FP32): fftfreq (src/fft.rs:544-566), whose
+    // mirror is its negative except at a Nyquist index, and lap_stencil with
+    // 2 - 2 cos t written as 4 sin^2(t / 2) (no cancellation).
     __device__ __forceinline__ void init(const VolDesc& V, int kx, int ky) {
-        const double fx = fftfreq_at(kx, V.nx, V.vsx), fy = fftfreq_at(ky, V.ny, V.vsy);
-        const double fxm = fftfreq_at(mirror_idx(kx, V.nx), V.nx, V.vsx);
-        const double fym = fftfreq_at(mirror_idx(ky, V.ny), V.ny, V.vsy);
-        sym = fxm == -fx && fym == -fy;
-        sxy = (float)(fx * V.bx + fy * V.by);
-        sxym = (float)(fxm * V.bx + fym * V.by);
-        k2xy = (float)(fx * fx + fy * fy);
-        lxy = (float)(lap_stencil(kx, V.nx, V.vsx) + lap_stencil(ky, V.ny, V.vsy));
-        rlxy = (float)(V.rho * (lap_stencil(kx, V.nx, V.vsx) + lap_stencil(ky, V.ny, V.vsy)));
+        const bool nyx = 2 * kx == V.nx, nyy = 2 * ky == V.ny;
+        sym = !nyx && !nyy;
+        const float fx = (float)(kx < (V.nx + 1) / 2 ? kx : kx - V.nx) * (float)(1.0 / ((double)V.nx * V.vsx));
+        const float fy = (float)(ky < (V.ny + 1) / 2 ? ky : ky - V.ny) * (float)(1.0 / ((double)V.ny * V.vsy));
+        const float fxm = nyx ? fx : -fx, fym = nyy ? fy : -fy;
+        const float bx = (float)V.bx, by = (float)V.by;
+        sxy = fx * bx + fy * by;
+        sxym = fxm * bx + fym * by;
+        k2xy = fx * fx + fy * fy;
+        auto lap = [](int k, int n, double vs) {
+            const float h2 = (float)(1.0 / (vs * vs));
+            if (n > 2) {
+                const float sn = sinpif((float)k / (float)n);
+                return 4.0f * (sn * sn) * h2;
+            }
+            return (n == 1 ? 2.0f : (k == 0 ? 1.0f : 3.0f)) * h2;
+        };
+        lxy = lap(kx, V.nx, V.vsx) + lap(ky, V.ny, V.vsy);
+        rlxy = (float)V.rho * lxy;
         dfz = (float)(1.0 / ((double)V.nz * V.vsz));
         bz = (float)V.bz;
         hz2 = (float)(1.0 / (V.vsz * V.vsz));
