@@ -1565,28 +1565,29 @@ struct TvQ {
     __device__ __forceinline__ void init(const VolDesc& V, int kx, int ky) {
         const bool nyx = 2 * kx == V.nx, nyy = 2 * ky == V.ny;
         sym = !nyx && !nyy;
-        const float fx = (float)(kx < (V.nx + 1) / 2 ? kx : kx - V.nx) * (float)(1.0 / ((double)V.nx * V.vsx));
-        const float fy = (float)(ky < (V.ny + 1) / 2 ? ky : ky - V.ny) * (float)(1.0 / ((double)V.ny * V.vsy));
+        // 1/(n vs) and 1/vs^2 from the FP32 1/vs (V.hx ...)
+        const float fx = (float)(kx < (V.nx + 1) / 2 ? kx : kx - V.nx) * (V.hx * __builtin_amdgcn_rcpf((float)V.nx));
+        const float fy = (float)(ky < (V.ny + 1) / 2 ? ky : ky - V.ny) * (V.hy * __builtin_amdgcn_rcpf((float)V.ny));
         const float fxm = nyx ? fx : -fx, fym = nyy ? fy : -fy;
         const float bx = (float)V.bx, by = (float)V.by;
         sxy = fx * bx + fy * by;
         sxym = fxm * bx + fym * by;
         k2xy = fx * fx + fy * fy;
-        auto lap = [](int k, int n, double vs) {
-            const float h2 = (float)(1.0 / (vs * vs));
+        auto lap = [](int k, int n, float h) {
+            const float h2 = h * h;
             if (n > 2) {
                 const float sn = sinpif((float)k / (float)n);
                 return 4.0f * (sn * sn) * h2;
             }
             return (n == 1 ? 2.0f : (k == 0 ? 1.0f : 3.0f)) * h2;
         };
-        lxy = lap(kx, V.nx, V.vsx) + lap(ky, V.ny, V.vsy);
-        rlxy = (float)V.rho * lxy;
-        dfz = (float)(1.0 / ((double)V.nz * V.vsz));
-        bz = (float)V.bz;
-        hz2 = (float)(1.0 / (V.vsz * V.vsz));
+        lxy = lap(kx, V.nx, V.hx) + lap(ky, V.ny, V.hy);
         rho = (float)V.rho;
-        scale = (float)(0.5 * V.rho / ((double)V.nx * V.ny * V.nz));
+        rlxy = rho * lxy;
+        dfz = V.hz * __builtin_amdgcn_rcpf((float)V.nz);
+        bz = (float)V.bz;
+        hz2 = V.hz * V.hz;
+        scale = 0.5f * rho * __builtin_amdgcn_rcpf((float)V.nx * (float)V.ny * (float)V.nz);
     }
     __device__ __forceinline__ float at(int kz, int nz, float2 twz) const {
         const int sz = kz < (nz + 1) / 2 ? kz : kz - nz;           // fftfreq index
@@ -1618,7 +1619,7 @@ struct TvQ {
     // z tile: (fz bz, fz^2, rho lz / vs_z^2) with fz the plane's frequency
     // and lz its share of the Laplacian (cos and sin of 2 pi kz / nz here).
     __device__ static float4 plane(const VolDesc& V, int kz, int nz) {
-        const float dfz = (float)(1.0 / ((double)nz * V.vsz));
+        const float dfz = V.hz * __builtin_amdgcn_rcpf((float)nz);
         const int sz = kz < (nz + 1) / 2 ? kz : kz - nz;
         const float fz = (float)sz * dfz;
         float sn, cs;
@@ -1626,7 +1627,7 @@ struct TvQ {
         const float lz = (nz > 2 ? (cs > 0.0f ? 2.0f * (sn * sn) * __builtin_amdgcn_rcpf(1.0f + cs)
                                               : 2.0f - 2.0f * cs)
                                  : (nz == 1 ? 2.0f : 2.0f - cs)) *
-                         (float)(1.0 / (V.vsz * V.vsz));
+                         (V.hz * V.hz);
         return make_float4(fz * (float)V.bz, fz * fz, (float)V.rho * lz, 0.0f);
     }
     // at() from the plane's terms P = plane(V, kz, nz)
